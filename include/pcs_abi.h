@@ -1,0 +1,228 @@
+/*
+ * pcs_abi.h -- C ABI of the MI355X simulation hot path for ABC-PMC particle
+ * concentration estimation (walk -> detect -> bin -> cumulative-histogram
+ * distance).
+ *
+ * Plain C types only: caller-owned host buffers, library-owned device
+ * buffers, int return codes (PCS_OK == 0, negative on error), no exceptions
+ * across the boundary. One pcs_ctx / pcs_batch / pcs_server must not be used
+ * by two threads at once.
+ *
+ * The reference has no plugin or FFI layer; its hot path sits behind two
+ * internal seams that these entry points replace (citations relative to the
+ * reference repository):
+ *   - per-job launch pair K_simulate_system_unr + K_distance issued by
+ *     dev_worker_ (src/gpu/detail/simulation_impl.inl:1064-1097), with
+ *     SimulateSetup / SimulateRun / HistogramRecorder arguments
+ *     (src/gpu/cusim/kernel-simulate.cuh:13-63, src/gpu/cusim/histogram2d.cuh:42-60);
+ *   - the reference histogram + SAT upload
+ *     (src/gpu/detail/simulation_impl.inl:605-629, src/gpu/cusim/histogram2d.inl:6-18);
+ *   - the multi-GPU queue pool and completion callback
+ *     (src/gpu/detail/simulation_impl.inl:631-730, 889-901, 1121-1136),
+ *     replaced by resident batches and a serving loop on one GPU.
+ *
+ * Histogram layout (differs from the reference by one row): row r, column b
+ * of an n_k x n_de row-major uint32 matrix counts tracks of length k = r + 1
+ * whose diffusion estimate falls in bin b = trunc(DE / de_bin_width). The
+ * reference keeps an always-empty row 0 (kmax = 1 + max frames,
+ * src/gpu/detail/simulation_impl.inl:513); the distance is unchanged by it.
+ */
+#ifndef PCS_ABI_H
+#define PCS_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCS_ABI_VERSION 1
+#define PCS_MAX_SPECIES 8
+#define PCS_MAX_NK 4096
+#define PCS_MAX_NDE 4096
+
+/* Return codes. */
+#define PCS_OK 0
+#define PCS_EINVAL (-1)    /* invalid argument */
+#define PCS_ENOMEM (-2)    /* device or host allocation failed */
+#define PCS_EHIP (-3)      /* HIP runtime error (see pcs_last_error) */
+#define PCS_EAGAIN (-4)    /* serving: no room now, try again */
+#define PCS_EOVERFLOW (-5) /* a record arena overflowed and could not be rerun */
+#define PCS_ENODEV (-6)    /* no usable GPU */
+#define PCS_ESTATE (-7)    /* call not valid in the object's current state */
+
+/* Boundary condition in z (x and y are always periodic). */
+#define PCS_Z_PERIODIC 0  /* reference: src/gpu/cusim/kernel-simulate-unroll.inl:176 */
+#define PCS_Z_REFLECT 1   /* model extension */
+
+/* Detection model. */
+#define PCS_DETECT_SLAB 0     /* |z| <= az/2 (src/gpu/cusim/utility.inl:102-153) */
+#define PCS_DETECT_GAUSS_Z 1  /* probability exp(-z^2 / (2 sigma^2)); extension */
+
+/* Distance kind; both return log10 of an exact integer sum over the 2D
+ * cumulative histograms S (simulated) and R (observed). */
+#define PCS_DIST_L2SQ 0  /* log10 sum (S-R)^2: src/cpu/distance.jl:1-8 */
+#define PCS_DIST_L1 1    /* log10 sum |S-R|: extension */
+
+/* Execution forms of pcs_simulate / pcs_batch_run. */
+#define PCS_MODE_PLAIN 0    /* one launch pair per simulation */
+#define PCS_MODE_BATCHED 1  /* one persistent launch per batch in a hipGraph */
+#define PCS_MODE_BATCHED_DIRECT 2 /* the batched kernels launched without the graph,
+                                     with events between them (kernel timing) */
+
+/* Model shared by all simulations of one ABC run
+ * (SimulateSetup, src/gpu/cusim/kernel-simulate.cuh:13-38, filled at
+ * src/gpu/detail/simulation_impl.inl:568-579). */
+typedef struct pcs_system {
+    uint32_t n_species;    /* C in [1, PCS_MAX_SPECIES] */
+    uint32_t kmin;         /* shortest recorded track, >= 2 */
+    uint32_t n_k;          /* histogram rows, [1, PCS_MAX_NK] */
+    uint32_t n_de;         /* histogram columns, [1, PCS_MAX_NDE] */
+    float de_bin_width;    /* ub_de / n_de */
+    float four_dt;         /* 4 * deltat */
+    float half_l[3];       /* half box extents Lx/2, Ly/2, Lz/2 (um) */
+    float half_a[2];       /* half detection extents ax/2, ay/2 (um) */
+    uint32_t z_boundary;   /* PCS_Z_* */
+    uint32_t detect_model; /* PCS_DETECT_* */
+    float detect_sigma;    /* sigma of PCS_DETECT_GAUSS_Z (um) */
+    uint32_t dist_kind;    /* PCS_DIST_* */
+} pcs_system;
+
+/* One simulation (SimulateRun, src/gpu/cusim/kernel-simulate.cuh:46-63,
+ * precomputed at src/gpu/detail/simulation_impl.inl:306-341). Videos are
+ * given as a range into the video arrays passed alongside. */
+typedef struct pcs_proposal {
+    uint32_t sim_id;       /* Philox counter word; unique per simulation and seed */
+    uint32_t first_video;  /* index of the first video in the video arrays */
+    uint32_t n_videos;     /* number of videos */
+    uint32_t reserved;     /* must be 0 */
+    float half_az[PCS_MAX_SPECIES];  /* az_i / 2 per species (fixed depth: all equal) */
+    float cum_prob[PCS_MAX_SPECIES]; /* cumulative abundance; entry C-1 >= 1 */
+    float sigma[PCS_MAX_SPECIES];    /* random-walk step sqrt(2 m_i dt) per species */
+} pcs_proposal;
+
+typedef struct pcs_ctx pcs_ctx;
+typedef struct pcs_batch pcs_batch;
+typedef struct pcs_server pcs_server;
+
+/* ----------------------------------------------------------------- basics */
+
+int pcs_abi_version(void);
+const char* pcs_strerror(int code);
+/* Last error message recorded on this thread (HIP error text, argument). */
+const char* pcs_last_error(void);
+/* Number of visible GPUs (0 when none). */
+int pcs_device_count(void);
+
+/* Observed histogram from track lists, as built by prepare_
+ * (src/gpu/detail/simulation_impl.inl:605-616) with row K-1: DE >= ub_de is
+ * skipped, bin = trunc(DE / (ub_de / n_de)). hist (n_k*n_de) is overwritten.
+ * Tracks outside the matrix are counted in *dropped (nullable). Host only. */
+int pcs_reference_hist(const uint64_t* K, const double* DE, uint64_t n, double ub_de,
+                       uint32_t n_k, uint32_t n_de, uint32_t* hist, uint64_t* dropped);
+
+/* ---------------------------------------------------------------- context */
+
+/* Open a context on GPU `device` for one model and its observed histogram
+ * (plain counts, n_k*n_de; the cumulative table is built internally, replacing
+ * the host SAT + upload of src/gpu/detail/simulation_impl.inl:617-629). */
+int pcs_open(const pcs_system* sys, const uint32_t* obs_hist, int device, pcs_ctx** out);
+int pcs_close(pcs_ctx* ctx);
+
+/* One-shot: simulate n proposals and write log10 distances (n doubles).
+ * hist_out (nullable) receives n*n_k*n_de histograms; tracks_out (nullable)
+ * the number of binned tracks per proposal. Blocks until done. */
+int pcs_simulate(pcs_ctx* ctx, const pcs_proposal* props, uint32_t n,
+                 const uint32_t* video_frames, const uint32_t* video_particles, uint32_t n_videos,
+                 uint64_t seed, int mode, double* dist_out, uint32_t* hist_out,
+                 uint32_t* tracks_out);
+
+/* ---------------------------------------------------------------- batches */
+
+/* A batch holds proposals resident in HBM; load uploads and plans it,
+ * run enqueues the walk + distance asynchronously on the batch stream,
+ * sync waits, read copies results back. */
+int pcs_batch_create(pcs_ctx* ctx, pcs_batch** out);
+int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n,
+                   const uint32_t* video_frames, const uint32_t* video_particles, uint32_t n_videos);
+int pcs_batch_run(pcs_batch* b, uint64_t seed, int mode);
+int pcs_batch_sync(pcs_batch* b);
+int pcs_batch_read(pcs_batch* b, double* dist_out, uint32_t* hist_out, uint32_t* tracks_out);
+/* Exact integer distance sums of the last run (n uint64: sum (S-R)^2 or
+ * sum |S-R|), of which the log10 distances are taken. Call after
+ * pcs_batch_read. */
+int pcs_batch_read_exact(pcs_batch* b, uint64_t* exact_out);
+/* Device time of the last run (HIP events on the batch stream), ms:
+ * out[0] whole run, out[1] walk kernel(s), out[2] distance kernel(s). */
+int pcs_batch_timing(pcs_batch* b, float out[3]);
+/* Number of proposals whose record arena overflowed in the last run and
+ * were rerun (informational). */
+int pcs_batch_overflow_reruns(pcs_batch* b, uint32_t* n);
+int pcs_batch_destroy(pcs_batch* b);
+
+/* Launch shape of the batched walk kernel. Zero fields mean "default".
+ * Results do not depend on the shape (the random stream is counter based). */
+typedef struct pcs_tuning {
+    uint32_t particles_per_lane; /* 1 or 2 */
+    uint32_t waves_per_group;    /* 1, 2, 4 */
+    uint32_t groups_per_cu;      /* persistent grid = CUs * groups_per_cu */
+    uint32_t stage_records;      /* LDS record staging per wave: 128, 256, 512 */
+} pcs_tuning;
+int pcs_set_tuning(pcs_ctx* ctx, const pcs_tuning* t);
+int pcs_get_tuning(pcs_ctx* ctx, pcs_tuning* t);
+/* Scale of the per-simulation record slice estimate (default 1.0; the bound
+ * is tracks <= detected particle-frames / kmin). Slices that overflow are
+ * rerun exactly, so this only trades memory for reruns. Takes effect at the
+ * next pcs_batch_load. */
+int pcs_set_record_capacity_factor(pcs_ctx* ctx, double factor);
+
+/* Evaluates the walk kernel's FP32 math on the GPU for n 32-bit words:
+ * radius[i] = sqrt(-2 ln(((w>>8)+1) 2^-24)), (s[i], c[i]) = sincos(2 pi (w>>8) 2^-24).
+ * Test hook for bit-exactness of the device math; no reference counterpart. */
+int pcs_selftest_math(int device, const uint32_t* w, float* radius, float* s, float* c, uint64_t n);
+
+/* ---------------------------------------------------------------- serving */
+
+typedef struct pcs_server_config {
+    uint64_t arena_bytes;    /* HBM budget for track records (admission), 0 = 1 GiB */
+    uint32_t max_inflight;   /* simulations on the GPU at once, 0 = 4096 */
+    uint32_t max_videos;     /* videos per request, 0 = 64 */
+    uint32_t flags;          /* reserved, 0 */
+} pcs_server_config;
+
+typedef struct pcs_request {
+    uint64_t tag;            /* caller's id, echoed in the completion */
+    uint64_t seed;           /* Philox key of this simulation */
+    uint32_t sys_id;         /* from pcs_server_add_system */
+    uint32_t n_videos;
+    const uint32_t* frames;     /* [n_videos], copied at submit */
+    const uint32_t* particles;  /* [n_videos], copied at submit */
+    pcs_proposal prop;       /* first_video is ignored */
+} pcs_request;
+
+typedef struct pcs_completion {
+    uint64_t tag;
+    double dist;             /* log10 distance */
+    uint32_t tracks;         /* binned tracks */
+    int32_t status;          /* PCS_OK or error */
+    double submit_us;        /* host clock at submit (us since server start) */
+    double done_us;          /* host clock when the result was observed */
+} pcs_completion;
+
+int pcs_server_create(int device, const pcs_server_config* cfg, pcs_server** out);
+int pcs_server_add_system(pcs_server* s, const pcs_system* sys, const uint32_t* obs_hist, uint32_t* sys_id);
+/* Non-blocking; PCS_EAGAIN when the request queue is full. */
+int pcs_server_submit(pcs_server* s, const pcs_request* req);
+/* Collect up to max completions; waits up to wait_us microseconds for at
+ * least one (0 = do not wait). */
+int pcs_server_poll(pcs_server* s, pcs_completion* out, uint32_t max, uint32_t* n_out, uint32_t wait_us);
+/* Requests submitted and not yet returned by poll. */
+int pcs_server_pending(pcs_server* s, uint32_t* n);
+int pcs_server_destroy(pcs_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

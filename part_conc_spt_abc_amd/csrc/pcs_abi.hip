@@ -1,0 +1,829 @@
+// pcs_abi.hip -- C ABI of the hot path: contexts, resident batches (plain and
+// batched forms), overflow reruns. The serving loop lives in pcs_server.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pcs_common.h"
+
+using namespace pcs;
+
+// ------------------------------------------------------------ error state
+
+namespace pcs {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_error(hipError_t e, const char* what)
+{
+    return set_error(PCS_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int validate_system(const pcs_system* s)
+{
+    if (!s) return set_error(PCS_EINVAL, "system is NULL");
+    if (s->n_species < 1 || s->n_species > PCS_MAX_SPECIES)
+        return set_error(PCS_EINVAL, "n_species %u outside [1, %d]", s->n_species, PCS_MAX_SPECIES);
+    if (s->kmin < 2) return set_error(PCS_EINVAL, "kmin %u < 2", s->kmin);
+    if (s->n_k < 1 || s->n_k > PCS_MAX_NK) return set_error(PCS_EINVAL, "n_k %u outside [1, %d]", s->n_k, PCS_MAX_NK);
+    if (s->n_de < 1 || s->n_de > PCS_MAX_NDE)
+        return set_error(PCS_EINVAL, "n_de %u outside [1, %d]", s->n_de, PCS_MAX_NDE);
+    if (!(s->de_bin_width > 0.0f) || !std::isfinite(s->de_bin_width))
+        return set_error(PCS_EINVAL, "de_bin_width must be positive");
+    if (!(s->four_dt > 0.0f) || !std::isfinite(s->four_dt)) return set_error(PCS_EINVAL, "four_dt must be positive");
+    for (int i = 0; i < 3; ++i)
+        if (!(s->half_l[i] > 0.0f) || !std::isfinite(s->half_l[i]))
+            return set_error(PCS_EINVAL, "half_l[%d] must be positive", i);
+    for (int i = 0; i < 2; ++i)
+        if (!(s->half_a[i] >= 0.0f) || !std::isfinite(s->half_a[i]))
+            return set_error(PCS_EINVAL, "half_a[%d] must be >= 0", i);
+    if (s->z_boundary > PCS_Z_REFLECT) return set_error(PCS_EINVAL, "z_boundary %u unknown", s->z_boundary);
+    if (s->detect_model > PCS_DETECT_GAUSS_Z) return set_error(PCS_EINVAL, "detect_model %u unknown", s->detect_model);
+    if (s->detect_model == PCS_DETECT_GAUSS_Z && !(s->detect_sigma > 0.0f))
+        return set_error(PCS_EINVAL, "detect_sigma must be positive for gaussian detection");
+    if (s->dist_kind > PCS_DIST_L1) return set_error(PCS_EINVAL, "dist_kind %u unknown", s->dist_kind);
+    return PCS_OK;
+}
+
+int validate_proposal(const pcs_system& s, const pcs_proposal& p)
+{
+    for (uint32_t c = 0; c < s.n_species; ++c) {
+        if (!(p.sigma[c] >= 0.0f) || !std::isfinite(p.sigma[c]))
+            return set_error(PCS_EINVAL, "proposal %u: sigma[%u] invalid", p.sim_id, c);
+        if (!(p.half_az[c] >= 0.0f) || !std::isfinite(p.half_az[c]))
+            return set_error(PCS_EINVAL, "proposal %u: half_az[%u] invalid", p.sim_id, c);
+    }
+    if (!(p.cum_prob[s.n_species - 1] >= 1.0f))
+        return set_error(PCS_EINVAL, "proposal %u: cum_prob[C-1] must be >= 1", p.sim_id);
+    return PCS_OK;
+}
+
+// Summed-area table of the observed histogram (uint32 like the reference's
+// host SAT, src/gpu/detail/simulation_impl.inl:617-628) and its deepest row.
+void build_sat(const uint32_t* H, uint32_t n_k, uint32_t n_de, std::vector<uint32_t>& S, uint32_t& maxrow)
+{
+    S.assign((size_t)n_k * n_de, 0u);
+    maxrow = 0;
+    for (uint32_t i = 0; i < n_k; ++i) {
+        uint32_t run = 0;
+        for (uint32_t j = 0; j < n_de; ++j) {
+            const uint32_t h = H ? H[(size_t)i * n_de + j] : 0u;
+            if (h) maxrow = i;
+            run += h;
+            S[(size_t)i * n_de + j] = run + (i ? S[(size_t)(i - 1) * n_de + j] : 0u);
+        }
+    }
+}
+
+DevSys make_devsys(const pcs_system& s, uint32_t maxrow, const uint32_t* dsat)
+{
+    DevSys d{};
+    d.n_species = s.n_species;
+    d.kmin = s.kmin;
+    d.n_k = s.n_k;
+    d.n_de = s.n_de;
+    d.de_bin_width = s.de_bin_width;
+    d.four_dt = s.four_dt;
+    for (int i = 0; i < 3; ++i) d.half_l[i] = s.half_l[i];
+    d.half_a[0] = s.half_a[0];
+    d.half_a[1] = s.half_a[1];
+    d.z_boundary = s.z_boundary;
+    d.detect_model = s.detect_model;
+    d.detect_sigma = s.detect_sigma;
+    d.dist_kind = s.dist_kind;
+    d.maxrow_obs = maxrow;
+    d.obs_sat = dsat;
+    return d;
+}
+
+// Record slice size for one simulation: every recorded track spans at least
+// kmin detected frames, so tracks <= detected particle-frames / kmin. The
+// expected detected fraction is the detection volume over the box volume;
+// `factor` scales that bound (calibrated, see tests) and a small floor
+// absorbs fluctuations. A slice that still overflows is rerun exactly.
+uint32_t record_capacity(const pcs_system& s, const pcs_proposal& p, const uint32_t* frames,
+                         const uint32_t* particles, double factor)
+{
+    double fxy = std::min(1.0, (double)s.half_a[0] / s.half_l[0]) * std::min(1.0, (double)s.half_a[1] / s.half_l[1]);
+    double fz = 0.0;
+    if (s.detect_model == PCS_DETECT_GAUSS_Z) {
+        fz = std::min(1.0, (double)s.detect_sigma * std::sqrt(2.0 * M_PI) / (2.0 * s.half_l[2]));
+    } else {
+        for (uint32_t c = 0; c < s.n_species; ++c) fz = std::max(fz, std::min(1.0, (double)p.half_az[c] / s.half_l[2]));
+    }
+    double pf = 0.0, maxbound = 0.0;
+    for (uint32_t v = 0; v < p.n_videos; ++v) {
+        pf += (double)particles[v] * (double)frames[v];
+        maxbound += (double)particles[v] * (double)((frames[v] + 1u) / (s.kmin + 1u));
+    }
+    const double e = pf * fxy * fz / (double)s.kmin;
+    double cap = factor * e + 4.0 * std::sqrt(e) + 64.0;
+    cap = std::min(cap, maxbound + 1.0);
+    cap = std::min(cap, 4.0e9);
+    return (uint32_t)std::ceil(cap);
+}
+
+}  // namespace pcs
+
+// --------------------------------------------------------------- context
+
+struct pcs_ctx {
+    int device = 0;
+    int n_cu = 0;
+    pcs_system sys{};
+    DevSys dsys{};
+    DevSys* d_sys = nullptr;
+    uint32_t* d_sat = nullptr;
+    hipStream_t stream = nullptr;
+    pcs_tuning tune{1, 4, 8, 256};
+    double cap_factor = 1.0;
+};
+
+struct pcs_batch {
+    pcs_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    // host copies
+    std::vector<pcs_proposal> props;
+    std::vector<DevProp> hprops;
+    std::vector<uint32_t> vframes, vparticles, vprop, vtile;
+    uint32_t n_props = 0, n_videos = 0, n_tiles = 0;
+    int planned_ppl = 0;
+    uint64_t arena_records = 0;
+    uint32_t* d_rec2 = nullptr;  // rerun arena of the last run (overflowed slices)
+    uint32_t* d_srt2 = nullptr;
+    // device
+    DevProp* d_props = nullptr;
+    uint32_t *d_vframes = nullptr, *d_vparticles = nullptr, *d_vprop = nullptr, *d_vtile = nullptr;
+    uint32_t *d_rec = nullptr, *d_srt = nullptr, *d_count = nullptr, *d_tracks = nullptr, *d_tile_counter = nullptr;
+    double* d_dist = nullptr;
+    uint64_t* d_exact = nullptr;
+    int32_t* d_status = nullptr;
+    uint64_t* d_seed = nullptr;
+    uint64_t* h_seed = nullptr;  // pinned
+    size_t cap_props = 0, cap_videos = 0, cap_arena = 0;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    pcs_tuning graph_tune{};
+    // timing
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int last_mode = -1;
+    bool ran = false;
+    uint32_t reruns = 0;
+};
+
+namespace {
+
+template <typename T>
+int dev_alloc(T** p, size_t n)
+{
+    if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+    }
+    if (n == 0) n = 1;
+    const hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(PCS_ENOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+    }
+    return PCS_OK;
+}
+
+#define PCS_HIP(call)                                         \
+    do {                                                      \
+        const hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return hip_error(e_, #call);    \
+    } while (0)
+
+#define PCS_TRY(call)                   \
+    do {                                \
+        const int r_ = (call);          \
+        if (r_ != PCS_OK) return r_;    \
+    } while (0)
+
+void destroy_graph(pcs_batch* b)
+{
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+int check_tuning(const pcs_tuning& t)
+{
+    if (t.particles_per_lane != 1 && t.particles_per_lane != 2)
+        return set_error(PCS_EINVAL, "particles_per_lane must be 1 or 2");
+    if (t.waves_per_group != 1 && t.waves_per_group != 2 && t.waves_per_group != 4)
+        return set_error(PCS_EINVAL, "waves_per_group must be 1, 2 or 4");
+    if (t.groups_per_cu < 1 || t.groups_per_cu > 32) return set_error(PCS_EINVAL, "groups_per_cu outside [1, 32]");
+    if (t.stage_records != 128 && t.stage_records != 256 && t.stage_records != 512)
+        return set_error(PCS_EINVAL, "stage_records must be 128, 256 or 512");
+    return PCS_OK;
+}
+
+// Tiles of 64*ppl particles per video, prefix-summed.
+void plan_tiles(pcs_batch* b, int ppl)
+{
+    const uint32_t tp = 64u * (uint32_t)ppl;
+    b->vtile.resize(b->n_videos + 1);
+    uint32_t acc = 0;
+    for (uint32_t v = 0; v < b->n_videos; ++v) {
+        b->vtile[v] = acc;
+        acc += (b->vparticles[v] + tp - 1u) / tp;
+    }
+    b->vtile[b->n_videos] = acc;
+    b->n_tiles = acc;
+    b->planned_ppl = ppl;
+}
+
+int upload_plan(pcs_batch* b)
+{
+    PCS_HIP(hipMemcpyAsync(b->d_vtile, b->vtile.data(), b->vtile.size() * 4, hipMemcpyHostToDevice, b->stream));
+    return PCS_OK;
+}
+
+WalkArgs walk_args(pcs_batch* b, uint32_t tile_lo, uint32_t tile_hi)
+{
+    WalkArgs a{};
+    a.sys = b->ctx->d_sys;
+    a.props = b->d_props;
+    a.vframes = b->d_vframes;
+    a.vparticles = b->d_vparticles;
+    a.vprop = b->d_vprop;
+    a.vtile = b->d_vtile;
+    a.n_videos = b->n_videos;
+    a.tile_lo = tile_lo;
+    a.tile_hi = tile_hi;
+    a.stage_cap = b->ctx->tune.stage_records;
+    a.rec_count = b->d_count;
+    a.seed = b->d_seed;
+    a.tile_counter = b->d_tile_counter;
+    return a;
+}
+
+DistArgs dist_args(pcs_batch* b, uint32_t lo, uint32_t n)
+{
+    DistArgs d{};
+    d.sys = b->ctx->d_sys;
+    d.props = b->d_props;
+    d.prop_lo = lo;
+    d.n_props = n;
+    d.rec_count = b->d_count;
+    d.dist = b->d_dist;
+    d.exact = b->d_exact;
+    d.tracks = b->d_tracks;
+    d.status = b->d_status;
+    return d;
+}
+
+// Enqueue the three batched kernels (reset, persistent walk, distance).
+int enqueue_batched(pcs_batch* b, bool with_events)
+{
+    pcs_ctx* c = b->ctx;
+    const pcs_tuning& t = c->tune;
+    const bool gauss = c->sys.detect_model == PCS_DETECT_GAUSS_Z;
+    const bool zref = c->sys.z_boundary == PCS_Z_REFLECT;
+    if (with_events) PCS_HIP(hipEventRecord(b->ev[0], b->stream));
+    PCS_HIP(launch_reset(b->d_count, b->n_props, b->d_tile_counter, b->stream));
+    if (with_events) PCS_HIP(hipEventRecord(b->ev[1], b->stream));
+    const uint32_t grid = (uint32_t)c->n_cu * t.groups_per_cu;
+    PCS_HIP(launch_walk(walk_args(b, 0, b->n_tiles), (int)t.particles_per_lane, (int)t.waves_per_group, true, grid,
+                        gauss, zref, b->stream));
+    if (with_events) PCS_HIP(hipEventRecord(b->ev[2], b->stream));
+    PCS_HIP(launch_distance(dist_args(b, 0, b->n_props), c->sys.n_de, b->stream));
+    if (with_events) PCS_HIP(hipEventRecord(b->ev[3], b->stream));
+    return PCS_OK;
+}
+
+// Plain form: per simulation, one walk launch over its tiles and one
+// distance launch (the reference's per-job pair, src/gpu/detail/simulation_impl.inl:1064-1097).
+int enqueue_plain(pcs_batch* b, const std::vector<uint32_t>* subset)
+{
+    pcs_ctx* c = b->ctx;
+    const pcs_tuning& t = c->tune;
+    const bool gauss = c->sys.detect_model == PCS_DETECT_GAUSS_Z;
+    const bool zref = c->sys.z_boundary == PCS_Z_REFLECT;
+    const uint32_t n = subset ? (uint32_t)subset->size() : b->n_props;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t P = subset ? (*subset)[i] : i;
+        const pcs_proposal& p = b->props[P];
+        const uint32_t lo = b->vtile[p.first_video], hi = b->vtile[p.first_video + p.n_videos];
+        PCS_HIP(launch_walk(walk_args(b, lo, hi), (int)t.particles_per_lane, (int)t.waves_per_group, false, 0, gauss,
+                            zref, b->stream));
+        PCS_HIP(launch_distance(dist_args(b, P, 1), c->sys.n_de, b->stream));
+    }
+    return PCS_OK;
+}
+
+int capture_graph(pcs_batch* b)
+{
+    destroy_graph(b);
+    PCS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int r = enqueue_batched(b, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (r != PCS_OK) {
+        if (g) hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+    b->graph = g;
+    PCS_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->graph_tune = b->ctx->tune;
+    return PCS_OK;
+}
+
+bool same_tune(const pcs_tuning& a, const pcs_tuning& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
+// Proposals whose slice overflowed are rerun (plain form) with a slice of
+// exactly the size the first pass counted (the counter keeps counting past
+// the slice end), placed in a second arena; the counter-based stream makes
+// the rerun reproduce the same tracks. Their table entries point there until
+// the next run restores them.
+int rerun_overflows(pcs_batch* b)
+{
+    std::vector<int32_t> st(b->n_props);
+    std::vector<uint32_t> cnt(b->n_props);
+    PCS_HIP(hipMemcpyAsync(st.data(), b->d_status, b->n_props * 4, hipMemcpyDeviceToHost, b->stream));
+    PCS_HIP(hipMemcpyAsync(cnt.data(), b->d_count, b->n_props * 4, hipMemcpyDeviceToHost, b->stream));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    std::vector<uint32_t> over;
+    uint64_t need = 0;
+    for (uint32_t i = 0; i < b->n_props; ++i)
+        if (st[i] == PCS_EOVERFLOW) {
+            over.push_back(i);
+            need += cnt[i];
+        }
+    b->reruns = (uint32_t)over.size();
+    if (over.empty()) return PCS_OK;
+    PCS_TRY(dev_alloc(&b->d_rec2, need));
+    PCS_TRY(dev_alloc(&b->d_srt2, need));
+    uint64_t off = 0;
+    for (uint32_t P : over) {
+        DevProp dp = b->hprops[P];
+        dp.rec = b->d_rec2 + off;
+        dp.srt = b->d_srt2 + off;
+        dp.rec_cap = cnt[P];
+        off += cnt[P];
+        PCS_HIP(hipMemcpyAsync(b->d_props + P, &dp, sizeof(DevProp), hipMemcpyHostToDevice, b->stream));
+        PCS_HIP(hipMemsetAsync(b->d_count + P, 0, 4, b->stream));
+    }
+    PCS_TRY(enqueue_plain(b, &over));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    return PCS_OK;
+}
+
+// Undo rerun_overflows before a new run.
+int restore_slices(pcs_batch* b)
+{
+    if (!b->d_rec2) return PCS_OK;
+    if (b->n_props)
+        PCS_HIP(hipMemcpyAsync(b->d_props, b->hprops.data(), b->n_props * sizeof(DevProp), hipMemcpyHostToDevice,
+                               b->stream));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    hipFree(b->d_rec2);
+    hipFree(b->d_srt2);
+    b->d_rec2 = b->d_srt2 = nullptr;
+    return PCS_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- ABI
+
+extern "C" {
+
+int pcs_abi_version(void) { return PCS_ABI_VERSION; }
+
+const char* pcs_strerror(int code)
+{
+    switch (code) {
+        case PCS_OK: return "ok";
+        case PCS_EINVAL: return "invalid argument";
+        case PCS_ENOMEM: return "out of memory";
+        case PCS_EHIP: return "HIP runtime error";
+        case PCS_EAGAIN: return "resource temporarily unavailable";
+        case PCS_EOVERFLOW: return "record arena overflow";
+        case PCS_ENODEV: return "no usable GPU";
+        case PCS_ESTATE: return "invalid state";
+        default: return "unknown error";
+    }
+}
+
+const char* pcs_last_error(void) { return g_last_error.c_str(); }
+
+int pcs_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pcs_reference_hist(const uint64_t* K, const double* DE, uint64_t n, double ub_de, uint32_t n_k, uint32_t n_de,
+                       uint32_t* hist, uint64_t* dropped)
+{
+    if ((!K || !DE) && n) return set_error(PCS_EINVAL, "K/DE NULL");
+    if (!hist || n_k == 0 || n_de == 0 || !(ub_de > 0.0)) return set_error(PCS_EINVAL, "bad histogram shape");
+    std::memset(hist, 0, (size_t)n_k * n_de * 4);
+    const double w = ub_de / (double)n_de;
+    uint64_t drop = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!(DE[i] < ub_de)) continue;
+        const long idx = (long)(DE[i] / w);
+        if (K[i] < 1 || K[i] > n_k || idx < 0 || idx >= (long)n_de) {
+            ++drop;
+            continue;
+        }
+        hist[(size_t)(K[i] - 1) * n_de + (size_t)idx] += 1u;
+    }
+    if (dropped) *dropped = drop;
+    return PCS_OK;
+}
+
+int pcs_open(const pcs_system* sys, const uint32_t* obs_hist, int device, pcs_ctx** out)
+{
+    if (!out) return set_error(PCS_EINVAL, "out is NULL");
+    *out = nullptr;
+    PCS_TRY(validate_system(sys));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PCS_ENODEV, "no GPU visible");
+    if (device < 0 || device >= ndev) return set_error(PCS_EINVAL, "device %d outside [0, %d)", device, ndev);
+    PCS_HIP(hipSetDevice(device));
+    pcs_ctx* c = new (std::nothrow) pcs_ctx();
+    if (!c) return set_error(PCS_ENOMEM, "context");
+    c->device = device;
+    c->sys = *sys;
+    hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (c->n_cu <= 0) c->n_cu = 256;
+    std::vector<uint32_t> sat;
+    uint32_t maxrow = 0;
+    build_sat(obs_hist, sys->n_k, sys->n_de, sat, maxrow);
+    int r = dev_alloc(&c->d_sat, sat.size());
+    if (r == PCS_OK) r = dev_alloc(&c->d_sys, 1);
+    if (r == PCS_OK) {
+        c->dsys = make_devsys(*sys, maxrow, c->d_sat);
+        hipError_t e = hipMemcpy(c->d_sat, sat.data(), sat.size() * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c->d_sys, &c->dsys, sizeof(DevSys), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) r = hip_error(e, "pcs_open");
+    }
+    if (r != PCS_OK) {
+        pcs_close(c);
+        return r;
+    }
+    *out = c;
+    return PCS_OK;
+}
+
+int pcs_close(pcs_ctx* c)
+{
+    if (!c) return PCS_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->d_sat) hipFree(c->d_sat);
+    if (c->d_sys) hipFree(c->d_sys);
+    delete c;
+    return PCS_OK;
+}
+
+int pcs_set_tuning(pcs_ctx* c, const pcs_tuning* t)
+{
+    if (!c || !t) return set_error(PCS_EINVAL, "NULL argument");
+    pcs_tuning v = *t;
+    if (!v.particles_per_lane) v.particles_per_lane = 1;
+    if (!v.waves_per_group) v.waves_per_group = 4;
+    if (!v.groups_per_cu) v.groups_per_cu = 8;
+    if (!v.stage_records) v.stage_records = 256;
+    PCS_TRY(check_tuning(v));
+    c->tune = v;
+    return PCS_OK;
+}
+
+int pcs_get_tuning(pcs_ctx* c, pcs_tuning* t)
+{
+    if (!c || !t) return set_error(PCS_EINVAL, "NULL argument");
+    *t = c->tune;
+    return PCS_OK;
+}
+
+int pcs_batch_create(pcs_ctx* c, pcs_batch** out)
+{
+    if (!c || !out) return set_error(PCS_EINVAL, "NULL argument");
+    *out = nullptr;
+    PCS_HIP(hipSetDevice(c->device));
+    pcs_batch* b = new (std::nothrow) pcs_batch();
+    if (!b) return set_error(PCS_ENOMEM, "batch");
+    b->ctx = c;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&b->ev[i]);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_seed, sizeof(uint64_t), hipHostMallocDefault);
+    int r = e == hipSuccess ? PCS_OK : hip_error(e, "pcs_batch_create");
+    if (r == PCS_OK) r = dev_alloc(&b->d_seed, 1);
+    if (r == PCS_OK) r = dev_alloc(&b->d_tile_counter, 1);
+    if (r != PCS_OK) {
+        pcs_batch_destroy(b);
+        return r;
+    }
+    *out = b;
+    return PCS_OK;
+}
+
+int pcs_batch_destroy(pcs_batch* b)
+{
+    if (!b) return PCS_OK;
+    hipSetDevice(b->ctx->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    destroy_graph(b);
+    void* ptrs[] = {b->d_props, b->d_vframes, b->d_vparticles, b->d_vprop, b->d_vtile, b->d_rec, b->d_srt,
+                    b->d_rec2, b->d_srt2, b->d_count, b->d_tracks, b->d_tile_counter, b->d_dist, b->d_exact, b->d_status, b->d_seed};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (b->h_seed) hipHostFree(b->h_seed);
+    for (hipEvent_t& e : b->ev)
+        if (e) hipEventDestroy(e);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return PCS_OK;
+}
+
+int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const uint32_t* vframes,
+                   const uint32_t* vparticles, uint32_t n_videos)
+{
+    if (!b) return set_error(PCS_EINVAL, "batch is NULL");
+    if (n && (!props || !vframes || !vparticles)) return set_error(PCS_EINVAL, "NULL proposal or video arrays");
+    pcs_ctx* c = b->ctx;
+    PCS_HIP(hipSetDevice(c->device));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    PCS_TRY(restore_slices(b));
+    destroy_graph(b);
+    b->ran = false;
+    b->props.assign(props, props + n);
+    b->hprops.resize(n);
+    b->vframes.assign(vframes, vframes + n_videos);
+    b->vparticles.assign(vparticles, vparticles + n_videos);
+    b->vprop.assign(n_videos, 0xffffffffu);
+    b->n_props = n;
+    b->n_videos = n_videos;
+    uint64_t arena = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const pcs_proposal& p = props[i];
+        if (p.reserved != 0) return set_error(PCS_EINVAL, "proposal %u: reserved field must be 0", i);
+        if ((uint64_t)p.first_video + p.n_videos > n_videos)
+            return set_error(PCS_EINVAL, "proposal %u: videos [%u, %u) outside %u", i, p.first_video,
+                             p.first_video + p.n_videos, n_videos);
+        PCS_TRY(validate_proposal(c->sys, p));
+        for (uint32_t v = p.first_video; v < p.first_video + p.n_videos; ++v) {
+            if (b->vprop[v] != 0xffffffffu)
+                return set_error(PCS_EINVAL, "video %u belongs to proposals %u and %u", v, b->vprop[v], i);
+            b->vprop[v] = i;
+        }
+        DevProp& d = b->hprops[i];
+        std::memset(&d, 0, sizeof(d));
+        d.sim_id = p.sim_id;
+        d.first_video = p.first_video;
+        d.n_videos = p.n_videos;
+        d.sys_id = 0;
+        d.rec = nullptr;  // set below, once the arena exists
+        d.srt = nullptr;
+        d.rec_cap = record_capacity(c->sys, p, vframes + p.first_video, vparticles + p.first_video, c->cap_factor);
+        arena += d.rec_cap;
+        for (int s = 0; s < PCS_MAX_SPECIES; ++s) {
+            d.half_az[s] = p.half_az[s];
+            d.cum_prob[s] = p.cum_prob[s];
+            d.sigma[s] = p.sigma[s];
+        }
+    }
+    for (uint32_t v = 0; v < n_videos; ++v)
+        if (b->vprop[v] == 0xffffffffu) return set_error(PCS_EINVAL, "video %u belongs to no proposal", v);
+    b->arena_records = arena;
+    plan_tiles(b, (int)c->tune.particles_per_lane);
+
+    if (n > b->cap_props) {
+        PCS_TRY(dev_alloc(&b->d_props, n));
+        PCS_TRY(dev_alloc(&b->d_count, n));
+        PCS_TRY(dev_alloc(&b->d_tracks, n));
+        PCS_TRY(dev_alloc(&b->d_dist, n));
+        PCS_TRY(dev_alloc(&b->d_exact, n));
+        PCS_TRY(dev_alloc(&b->d_status, n));
+        b->cap_props = n;
+    }
+    if (n_videos + 1 > b->cap_videos) {
+        PCS_TRY(dev_alloc(&b->d_vframes, n_videos + 1));
+        PCS_TRY(dev_alloc(&b->d_vparticles, n_videos + 1));
+        PCS_TRY(dev_alloc(&b->d_vprop, n_videos + 1));
+        PCS_TRY(dev_alloc(&b->d_vtile, n_videos + 1));
+        b->cap_videos = n_videos + 1;
+    }
+    if (arena > b->cap_arena) {
+        PCS_TRY(dev_alloc(&b->d_rec, arena));
+        PCS_TRY(dev_alloc(&b->d_srt, arena));
+        b->cap_arena = arena;
+    }
+    {
+        uint64_t off = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            b->hprops[i].rec = b->d_rec + off;
+            b->hprops[i].srt = b->d_srt + off;
+            off += b->hprops[i].rec_cap;
+        }
+    }
+    if (n) {
+        PCS_HIP(hipMemcpyAsync(b->d_props, b->hprops.data(), n * sizeof(DevProp), hipMemcpyHostToDevice, b->stream));
+    }
+    if (n_videos) {
+        PCS_HIP(hipMemcpyAsync(b->d_vframes, vframes, n_videos * 4, hipMemcpyHostToDevice, b->stream));
+        PCS_HIP(hipMemcpyAsync(b->d_vparticles, vparticles, n_videos * 4, hipMemcpyHostToDevice, b->stream));
+        PCS_HIP(hipMemcpyAsync(b->d_vprop, b->vprop.data(), n_videos * 4, hipMemcpyHostToDevice, b->stream));
+    }
+    PCS_TRY(upload_plan(b));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    return PCS_OK;
+}
+
+int pcs_batch_run(pcs_batch* b, uint64_t seed, int mode)
+{
+    if (!b) return set_error(PCS_EINVAL, "batch is NULL");
+    if (mode != PCS_MODE_PLAIN && mode != PCS_MODE_BATCHED && mode != PCS_MODE_BATCHED_DIRECT)
+        return set_error(PCS_EINVAL, "mode %d unknown", mode);
+    pcs_ctx* c = b->ctx;
+    PCS_HIP(hipSetDevice(c->device));
+    if (b->planned_ppl != (int)c->tune.particles_per_lane) {
+        PCS_HIP(hipStreamSynchronize(b->stream));
+        plan_tiles(b, (int)c->tune.particles_per_lane);
+        PCS_TRY(upload_plan(b));
+        destroy_graph(b);
+    }
+    // The seed lives in device memory so one captured graph serves every seed.
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    PCS_TRY(restore_slices(b));
+    *b->h_seed = seed;
+    PCS_HIP(hipMemcpyAsync(b->d_seed, b->h_seed, sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
+    b->reruns = 0;
+    if (mode == PCS_MODE_BATCHED) {
+        if (!b->gexec || !same_tune(b->graph_tune, c->tune)) PCS_TRY(capture_graph(b));
+        PCS_HIP(hipEventRecord(b->ev[0], b->stream));
+        PCS_HIP(hipGraphLaunch(b->gexec, b->stream));
+        PCS_HIP(hipEventRecord(b->ev[3], b->stream));
+    } else if (mode == PCS_MODE_BATCHED_DIRECT) {
+        PCS_TRY(enqueue_batched(b, true));
+    } else {
+        PCS_HIP(hipEventRecord(b->ev[0], b->stream));
+        PCS_HIP(launch_reset(b->d_count, b->n_props, b->d_tile_counter, b->stream));
+        PCS_TRY(enqueue_plain(b, nullptr));
+        PCS_HIP(hipEventRecord(b->ev[3], b->stream));
+    }
+    b->last_mode = mode;
+    b->ran = true;
+    return PCS_OK;
+}
+
+int pcs_batch_sync(pcs_batch* b)
+{
+    if (!b) return set_error(PCS_EINVAL, "batch is NULL");
+    PCS_HIP(hipSetDevice(b->ctx->device));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    return PCS_OK;
+}
+
+int pcs_batch_timing(pcs_batch* b, float out[3])
+{
+    if (!b || !out) return set_error(PCS_EINVAL, "NULL argument");
+    if (!b->ran) return set_error(PCS_ESTATE, "batch has not run");
+    PCS_HIP(hipEventSynchronize(b->ev[3]));
+    out[0] = out[1] = out[2] = -1.0f;
+    PCS_HIP(hipEventElapsedTime(&out[0], b->ev[0], b->ev[3]));
+    if (b->last_mode == PCS_MODE_BATCHED_DIRECT) {
+        PCS_HIP(hipEventElapsedTime(&out[1], b->ev[1], b->ev[2]));
+        PCS_HIP(hipEventElapsedTime(&out[2], b->ev[2], b->ev[3]));
+    }
+    return PCS_OK;
+}
+
+int pcs_batch_overflow_reruns(pcs_batch* b, uint32_t* n)
+{
+    if (!b || !n) return set_error(PCS_EINVAL, "NULL argument");
+    *n = b->reruns;
+    return PCS_OK;
+}
+
+int pcs_batch_read(pcs_batch* b, double* dist_out, uint32_t* hist_out, uint32_t* tracks_out)
+{
+    if (!b) return set_error(PCS_EINVAL, "batch is NULL");
+    if (!b->ran) return set_error(PCS_ESTATE, "batch has not run");
+    pcs_ctx* c = b->ctx;
+    PCS_HIP(hipSetDevice(c->device));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    PCS_TRY(rerun_overflows(b));
+    if (b->n_props == 0) return PCS_OK;
+    if (dist_out)
+        PCS_HIP(hipMemcpyAsync(dist_out, b->d_dist, b->n_props * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (tracks_out)
+        PCS_HIP(hipMemcpyAsync(tracks_out, b->d_tracks, b->n_props * 4, hipMemcpyDeviceToHost, b->stream));
+    if (hist_out) {
+        // Dense histograms are formed from the records in chunks of proposals.
+        std::vector<uint32_t> cnt(b->n_props);
+        PCS_HIP(hipMemcpyAsync(cnt.data(), b->d_count, b->n_props * 4, hipMemcpyDeviceToHost, b->stream));
+        PCS_HIP(hipStreamSynchronize(b->stream));
+        const size_t per = (size_t)c->sys.n_k * c->sys.n_de;
+        const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(4096, (256u << 20) / (per * 4)));
+        uint32_t* d_h = nullptr;
+        PCS_TRY(dev_alloc(&d_h, per * chunk));
+        int r = PCS_OK;
+        for (uint32_t lo = 0; lo < b->n_props && r == PCS_OK; lo += chunk) {
+            const uint32_t m = std::min(chunk, b->n_props - lo);
+            uint32_t mx = 0;
+            for (uint32_t i = lo; i < lo + m; ++i) mx = std::max(mx, std::min(cnt[i], b->hprops[i].rec_cap));
+            hipError_t e = hipMemsetAsync(d_h, 0, per * m * 4, b->stream);
+            if (e == hipSuccess)
+                e = launch_dense_hist(b->d_props, lo, m, b->d_count, mx, c->sys.n_k, c->sys.n_de, d_h, b->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(hist_out + per * lo, d_h, per * m * 4, hipMemcpyDeviceToHost, b->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+            if (e != hipSuccess) r = hip_error(e, "dense histogram");
+        }
+        hipFree(d_h);
+        PCS_TRY(r);
+    }
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    return PCS_OK;
+}
+
+int pcs_batch_read_exact(pcs_batch* b, uint64_t* exact_out)
+{
+    if (!b || !exact_out) return set_error(PCS_EINVAL, "NULL argument");
+    if (!b->ran) return set_error(PCS_ESTATE, "batch has not run");
+    PCS_HIP(hipSetDevice(b->ctx->device));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    if (b->n_props)
+        PCS_HIP(hipMemcpyAsync(exact_out, b->d_exact, b->n_props * sizeof(uint64_t), hipMemcpyDeviceToHost, b->stream));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    return PCS_OK;
+}
+
+int pcs_set_record_capacity_factor(pcs_ctx* c, double factor)
+{
+    if (!c || !(factor > 0.0) || !std::isfinite(factor)) return set_error(PCS_EINVAL, "factor must be positive");
+    c->cap_factor = factor;
+    return PCS_OK;
+}
+
+int pcs_selftest_math(int device, const uint32_t* w, float* radius, float* s, float* c, uint64_t n)
+{
+    if (n && (!w || !radius || !s || !c)) return set_error(PCS_EINVAL, "NULL argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PCS_ENODEV, "no GPU visible");
+    if (device < 0 || device >= ndev) return set_error(PCS_EINVAL, "device %d outside [0, %d)", device, ndev);
+    PCS_HIP(hipSetDevice(device));
+    if (n == 0) return PCS_OK;
+    uint32_t* dw = nullptr;
+    float* dout = nullptr;
+    PCS_TRY(dev_alloc(&dw, n));
+    if (dev_alloc(&dout, 3 * n) != PCS_OK) {
+        hipFree(dw);
+        return PCS_ENOMEM;
+    }
+    hipError_t e = hipMemcpy(dw, w, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_selftest_math(dw, dout, dout + n, dout + 2 * n, n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(radius, dout, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(s, dout + n, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(c, dout + 2 * n, n * 4, hipMemcpyDeviceToHost);
+    hipFree(dw);
+    hipFree(dout);
+    if (e != hipSuccess) return hip_error(e, "pcs_selftest_math");
+    return PCS_OK;
+}
+
+int pcs_simulate(pcs_ctx* c, const pcs_proposal* props, uint32_t n, const uint32_t* vframes,
+                 const uint32_t* vparticles, uint32_t n_videos, uint64_t seed, int mode, double* dist_out,
+                 uint32_t* hist_out, uint32_t* tracks_out)
+{
+    if (!c) return set_error(PCS_EINVAL, "context is NULL");
+    pcs_batch* b = nullptr;
+    PCS_TRY(pcs_batch_create(c, &b));
+    int r = pcs_batch_load(b, props, n, vframes, vparticles, n_videos);
+    if (r == PCS_OK) r = pcs_batch_run(b, seed, mode);
+    if (r == PCS_OK) r = pcs_batch_read(b, dist_out, hist_out, tracks_out);
+    pcs_batch_destroy(b);
+    return r;
+}
+
+}  // extern "C"

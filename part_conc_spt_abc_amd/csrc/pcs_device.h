@@ -1,0 +1,140 @@
+// pcs_device.h -- gfx950 device primitives of the walk kernel: Philox4x32-10,
+// the project's FP32 log / sqrt / sincos-in-turns, and the per-step walk,
+// wrap, detect and track logic.
+//
+// Every function here fixes its FP32 operation order: the library is built
+// with -ffp-contract=off and correctly rounded division/sqrt, fmaf() is the
+// single-rounding fused multiply-add, so results are reproducible bit for bit
+// on any IEEE-754 host that evaluates the same sequence.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pcs {
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): multipliers and Weyl
+// key increments of the paper's Philox4x32 parameter set.
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+// The counter words that are wave-uniform (block, video, simulation) and the
+// key stay in SGPRs; only the particle word varies per lane, so the first
+// rounds run partly on the scalar unit.
+__device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n3 = (uint32_t)p0;
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// --------------------------------------------------------------- FP32 math
+// ln(1+f) = f + f^2 P(f) on [sqrt(1/2)-1, sqrt(2)-1]; coefficients from
+// fit_math_coeffs.py (reweighted least squares towards minimax), rounded to
+// FP32. ln 2 is split (Cody & Waite) so E * kLn2Hi is exact for |E| < 256.
+constexpr float kLogP0 = -0x1.fffff4p-2f, kLogP1 = 0x1.5556e8p-2f, kLogP2 = -0x1.0006c6p-2f,
+                kLogP3 = 0x1.98da1cp-3f, kLogP4 = -0x1.52fb0cp-3f, kLogP5 = 0x1.30d166p-3f,
+                kLogP6 = -0x1.277654p-3f, kLogP7 = 0x1.6fccd4p-4f;
+constexpr float kLn2Hi = 0x1.62e4p-1f;
+constexpr float kLn2Lo = 0x1.7f7d1cp-20f;
+constexpr float kSqrtHalf = 0x1.6a09e6p-1f;
+// sin(2 pi t) = t S(t^2), cos(2 pi t) = C(t^2) for |t| <= 1/8 turn.
+constexpr float kSin0 = 0x1.921fb6p+2f, kSin1 = -0x1.4abbbap+5f, kSin2 = 0x1.465ec6p+6f,
+                kSin3 = -0x1.2d9bcep+6f;
+constexpr float kCos1 = -0x1.3bd3ccp+4f, kCos2 = 0x1.03c1dep+6f, kCos3 = -0x1.55c664p+6f,
+                kCos4 = 0x1.d9f7bcp+5f;
+
+// -2 ln(u) for u = ((w >> 8) + 1) 2^-24 in (0, 1].
+__device__ __forceinline__ float neg2log_u24(uint32_t w)
+{
+    const uint32_t j = (w >> 8) + 1u;
+    const float xf = (float)j;  // exact: j <= 2^24
+    int e;
+    float m = __builtin_frexpf(xf, &e);  // v_frexp_mant / v_frexp_exp
+    const bool lo = m < kSqrtHalf;
+    m = lo ? m * 2.0f : m;
+    e = lo ? e - 1 : e;
+    const float f = m - 1.0f;  // exact (Sterbenz)
+    const float E = (float)(e - 24);
+    const float f2 = f * f;
+    float P = kLogP7;
+    P = __builtin_fmaf(P, f, kLogP6);
+    P = __builtin_fmaf(P, f, kLogP5);
+    P = __builtin_fmaf(P, f, kLogP4);
+    P = __builtin_fmaf(P, f, kLogP3);
+    P = __builtin_fmaf(P, f, kLogP2);
+    P = __builtin_fmaf(P, f, kLogP1);
+    P = __builtin_fmaf(P, f, kLogP0);
+    const float l1p = __builtin_fmaf(f2, P, f);
+    const float L = __builtin_fmaf(E, kLn2Hi, __builtin_fmaf(E, kLn2Lo, l1p));
+    return __builtin_fmaf(-2.0f, L, 0.0f);
+}
+
+// Box-Muller radius sqrt(-2 ln u), correctly rounded sqrt.
+__device__ __forceinline__ float bm_radius(uint32_t w) { return __builtin_sqrtf(neg2log_u24(w)); }
+
+// sin and cos of 2 pi t, t = (w >> 8) 2^-24: exact reduction to the nearest
+// quarter turn in integers, polynomial on |t| <= 1/8, quadrant rotation by
+// swap and sign flips.
+__device__ __forceinline__ void sincos_turn_u24(uint32_t w, float& s_out, float& c_out)
+{
+    const uint32_t j = (w >> 8) + (1u << 21);
+    const uint32_t q = (j >> 22) & 3u;
+    const int32_t r = (int32_t)(j & ((1u << 22) - 1u)) - (1 << 21);
+    const float t = (float)r * 0x1p-24f;  // exact
+    const float t2 = t * t;
+    float sp = kSin3;
+    sp = __builtin_fmaf(sp, t2, kSin2);
+    sp = __builtin_fmaf(sp, t2, kSin1);
+    sp = __builtin_fmaf(sp, t2, kSin0);
+    const float s = t * sp;
+    float c = kCos4;
+    c = __builtin_fmaf(c, t2, kCos3);
+    c = __builtin_fmaf(c, t2, kCos2);
+    c = __builtin_fmaf(c, t2, kCos1);
+    c = __builtin_fmaf(c, t2, 1.0f);
+    const bool swap = (q & 1u) != 0u;
+    const float so = swap ? c : s;
+    const float co = swap ? s : c;
+    const uint32_t sb = (q & 2u) << 30;
+    const uint32_t cb = ((q + 1u) & 2u) << 30;
+    s_out = __uint_as_float(__float_as_uint(so) ^ sb);
+    c_out = __uint_as_float(__float_as_uint(co) ^ cb);
+}
+
+__device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
+
+// Single periodic wrap of a centred coordinate (the reference's
+// periodic_bounds_: src/gpu/cusim/kernel-simulate-unroll.inl:24-32).
+__device__ __forceinline__ float wrap_periodic(float v, float h, float L)
+{
+    return (__builtin_fabsf(v) > h) ? v - __builtin_copysignf(L, v) : v;
+}
+// Reflecting wall: v' = +-L - v beyond +-L/2.
+__device__ __forceinline__ float wrap_reflect(float v, float h, float L)
+{
+    return (__builtin_fabsf(v) > h) ? __builtin_copysignf(L, v) - v : v;
+}
+
+}  // namespace pcs

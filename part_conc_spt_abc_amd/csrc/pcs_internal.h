@@ -1,0 +1,87 @@
+// pcs_internal.h -- device-side layouts and kernel launchers shared by the
+// C-ABI implementation (pcs_abi.hip) and the kernels (pcs_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pcs_abi.h"
+
+namespace pcs {
+
+constexpr int kWave = 64;
+
+// One model, resident in HBM (one entry per system of a context or server).
+struct DevSys {
+    uint32_t n_species, kmin, n_k, n_de;
+    float de_bin_width, four_dt;
+    float half_l[3];
+    float half_a[2];
+    uint32_t z_boundary, detect_model;
+    float detect_sigma;
+    uint32_t dist_kind;
+    uint32_t maxrow_obs;      // last row holding observed tracks (0 if none)
+    const uint32_t* obs_sat;  // observed summed-area table, n_k * n_de uint32
+    uint64_t pad1;
+};
+static_assert(sizeof(DevSys) == 80, "DevSys layout");
+
+// One simulation, resident in HBM. Its record slice rec[0, rec_cap) holds
+// packed (row << 16 | bin) track records; srt is a scratch slice of the same
+// size for the distance kernel's counting sort. Its Philox key is the launch
+// seed XOR (key0, key1): batches use key 0 and one seed per run, served
+// requests carry their own key.
+struct DevProp {
+    uint32_t sim_id, first_video, n_videos, sys_id;
+    uint32_t* rec;
+    uint32_t* srt;
+    uint32_t rec_cap;
+    uint32_t key0, key1;
+    uint32_t pad;
+    float half_az[PCS_MAX_SPECIES];
+    float cum_prob[PCS_MAX_SPECIES];
+    float sigma[PCS_MAX_SPECIES];
+};
+static_assert(sizeof(DevProp) == 144, "DevProp layout");
+
+// Arguments of the walk kernel. Videos are the unit of the work plan: video v
+// belongs to proposal vprop[v], has vframes[v] frames and vparticles[v]
+// particles, and owns tiles [vtile[v], vtile[v+1]) of 64 * PPL particles.
+struct WalkArgs {
+    const DevSys* sys;
+    const DevProp* props;
+    const uint32_t* vframes;
+    const uint32_t* vparticles;
+    const uint32_t* vprop;
+    const uint32_t* vtile;   // n_videos + 1 entries
+    uint32_t n_videos;
+    uint32_t tile_lo, tile_hi;  // tiles processed by this launch
+    uint32_t stage_cap;         // LDS records per wave
+    uint32_t* rec_count;        // per proposal, atomically bumped
+    const uint64_t* seed;       // device-resident Philox key
+    uint32_t* tile_counter;     // persistent form: next tile (relative to tile_lo)
+};
+
+struct DistArgs {
+    const DevSys* sys;
+    const DevProp* props;
+    uint32_t prop_lo;           // proposal of block 0
+    uint32_t n_props;
+    const uint32_t* rec_count;
+    double* dist;               // log10 distance per proposal
+    uint64_t* exact;            // integer sum per proposal
+    uint32_t* tracks;           // binned tracks per proposal
+    int32_t* status;            // PCS_OK or PCS_EOVERFLOW per proposal
+};
+
+// Launchers (pcs_kernels.hip). waves_per_group in {1,2,4}; ppl in {1,2}.
+hipError_t launch_reset(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter, hipStream_t s);
+hipError_t launch_walk(const WalkArgs& a, int ppl, int waves_per_group, bool persistent,
+                       uint32_t grid_groups, bool gauss, bool zreflect, hipStream_t s);
+hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s);
+hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, float* c, uint64_t n, hipStream_t st);
+hipError_t launch_dense_hist(const DevProp* props, uint32_t prop_lo, uint32_t n_props,
+                             const uint32_t* rec_count, uint32_t max_count,
+                             uint32_t n_k, uint32_t n_de, uint32_t* hist, hipStream_t s);
+
+}  // namespace pcs

@@ -1,0 +1,531 @@
+// pcs_kernels.hip -- gfx950 kernels of the simulation hot path.
+//
+//   walk_kernel      one wave = one tile of 64*PPL particles of one video;
+//                    1 lane = PPL particles with all state in VGPRs. Replaces
+//                    K_simulate_system_unr (src/gpu/cusim/kernel-simulate-unroll.inl:69-298),
+//                    where one 32-wide warp looped over all particles of a video.
+//                    Closed tracks are packed (row<<16 | bin) and staged in a
+//                    per-wave LDS buffer, then appended to the proposal's HBM
+//                    record slice with one atomic per flush (replaces the
+//                    per-track global atomicInc of src/gpu/cusim/histogram2d.inl:98-106).
+//   distance_kernel  one workgroup per proposal: counting sort of the records
+//                    by row in LDS, then a row wavefront over the 2D
+//                    cumulative histogram with the running column sums kept
+//                    in registers and each row's prefix formed by an LDS block
+//                    scan; exact integer sum of (S-R)^2 or |S-R| against the
+//                    observed summed-area table. Replaces K_distance
+//                    (src/gpu/cusim/kernel-distance.inl:81-241), a single 32x32
+//                    block that scanned the whole dense histogram in FP32.
+//   dense_hist       scatter of records into dense histograms (tests, export).
+//   reset_kernel     zeroes per-proposal counters (a kernel rather than a
+//                    memset so it can sit inside a replayed hipGraph).
+#include "pcs_device.h"
+#include "pcs_internal.h"
+
+namespace pcs {
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Orders this wave's LDS accesses across lanes (staging buffer handoff).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------- record stage
+
+struct Stage {
+    uint32_t* buf;      // this wave's LDS slice
+    uint32_t cap;       // entries
+    uint32_t cnt;       // wave-uniform fill
+    uint32_t* rec;      // proposal's record slice
+    uint32_t slice_cap;
+    uint32_t* counter;  // proposal record counter
+};
+
+__device__ __forceinline__ void stage_flush(Stage& st)
+{
+    if (st.cnt == 0u) return;
+    wave_lds_sync();
+    uint32_t base = 0u;
+    if (lane_id() == 0u) base = atomicAdd(st.counter, st.cnt);
+    base = uniform(base);
+    for (uint32_t i = lane_id(); i < st.cnt; i += kWave) {
+        const uint32_t idx = base + i;
+        if (idx < st.slice_cap) st.rec[idx] = st.buf[i];
+    }
+    wave_lds_sync();
+    st.cnt = 0u;
+}
+
+__device__ __forceinline__ void stage_push(Stage& st, bool ok, uint32_t v)
+{
+    const uint64_t m = ballot(ok);
+    if (m == 0ull) return;
+    if (st.cnt + (uint32_t)kWave > st.cap) stage_flush(st);
+    const uint32_t pos = st.cnt + mbcnt64(m);
+    if (ok) st.buf[pos] = v;
+    st.cnt += (uint32_t)__builtin_popcountll(m);
+}
+
+// Track record: de / ((k-1) 4 dt) then bin = trunc(value / dDE), dropped when
+// the bin or the length is outside the matrix
+// (src/gpu/cusim/kernel-simulate-unroll.inl:187-190, src/gpu/cusim/histogram2d.inl:98-106).
+__device__ __forceinline__ bool track_record(uint32_t k, float de, float four_dt, float dde,
+                                             uint32_t n_k, uint32_t n_de, uint32_t& out)
+{
+    const float q = (float)(k - 1u) * four_dt;
+    const float v = de / q;
+    const float tb = v / dde;
+    const bool ok = (k <= n_k) && (tb < (float)n_de);
+    out = ok ? (((k - 1u) << 16) | (uint32_t)tb) : 0u;
+    return ok;
+}
+
+// ------------------------------------------------------------------- walk
+
+struct Particle {
+    float x, y, z, de, sigma, haz;
+    uint32_t k;
+    uint32_t id;
+    bool active;
+};
+
+template <int PPL, bool GAUSS, bool ZREF>
+__device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stage& st, uint32_t s0, uint32_t s1)
+{
+    // Video owning tile t: largest v with vtile[v] <= t (wave-uniform search).
+    uint32_t lo = 0u, hi = a.n_videos;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.vtile[mid] <= t) lo = mid; else hi = mid;
+    }
+    const uint32_t v = lo;
+    const uint32_t P = a.vprop[v];
+    const DevProp& pr = a.props[P];
+    const DevSys& sy = a.sys[pr.sys_id];
+
+    const uint32_t F = a.vframes[v];
+    const uint32_t N = a.vparticles[v];
+    const uint32_t video = v - pr.first_video;
+    const uint32_t sim = pr.sim_id;
+    const uint32_t k0 = s0 ^ pr.key0, k1 = s1 ^ pr.key1;
+    const uint32_t p0 = (t - a.vtile[v]) * (uint32_t)(kWave * PPL);
+
+    const float hx = sy.half_l[0], hy = sy.half_l[1], hz = sy.half_l[2];
+    const float Lx = 2.0f * hx, Ly = 2.0f * hy, Lz = 2.0f * hz;
+    const float hax = sy.half_a[0], hay = sy.half_a[1];
+    const float sig2 = sy.detect_sigma * sy.detect_sigma;
+    const uint32_t kmin = sy.kmin, n_k = sy.n_k, n_de = sy.n_de, C = sy.n_species;
+    const float four_dt = sy.four_dt, dde = sy.de_bin_width;
+    constexpr uint32_t B = GAUSS ? 4u : 3u;
+
+    st.rec = pr.rec;
+    st.slice_cap = pr.rec_cap;
+    st.counter = a.rec_count + P;
+
+    Particle pt[PPL];
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+        Particle& q = pt[i];
+        q.id = p0 + (uint32_t)(i * kWave) + lane_id();
+        q.active = q.id < N;
+        const U4 w = philox10(0u, q.id, video, sim, k0, k1);
+        q.x = hx * (2.0f * u24(w.x) - 1.0f);
+        q.y = hy * (2.0f * u24(w.y) - 1.0f);
+        q.z = hz * (2.0f * u24(w.z) - 1.0f);
+        const float us = u24(w.w);
+        uint32_t s = 0u;
+        for (uint32_t c = 0; c + 1u < C; ++c) s += (us >= pr.cum_prob[c]) ? 1u : 0u;
+        float sg = pr.sigma[0], az = pr.half_az[0];
+#pragma unroll
+        for (uint32_t c = 1; c < PCS_MAX_SPECIES; ++c) {
+            sg = (s == c) ? pr.sigma[c] : sg;
+            az = (s == c) ? pr.half_az[c] : az;
+        }
+        q.sigma = sg;
+        q.haz = az;
+        bool det;
+        if (GAUSS) {
+            const U4 wd = philox10(0xffffffffu, q.id, video, sim, k0, k1);
+            det = __builtin_fabsf(q.x) <= hax && __builtin_fabsf(q.y) <= hay &&
+                  (q.z * q.z) <= sig2 * neg2log_u24(wd.x);
+        } else {
+            det = __builtin_fabsf(q.z) <= q.haz && __builtin_fabsf(q.x) <= hax && __builtin_fabsf(q.y) <= hay;
+        }
+        q.k = det ? 1u : 0u;
+        q.de = 0.0f;
+    }
+
+    const uint32_t steps = F > 0u ? F - 1u : 0u;
+    for (uint32_t c = 0; 4u * c < steps; ++c) {
+        const uint32_t nst = min(4u, steps - 4u * c);
+        float d[PPL][12];
+        uint32_t wdet[PPL][4];
+#pragma unroll
+        for (int i = 0; i < PPL; ++i) {
+            const U4 b0 = philox10(B * c + 1u, pt[i].id, video, sim, k0, k1);
+            const U4 b1 = philox10(B * c + 2u, pt[i].id, video, sim, k0, k1);
+            const U4 b2 = philox10(B * c + 3u, pt[i].id, video, sim, k0, k1);
+            const uint32_t w[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+#pragma unroll
+            for (int p = 0; p < 6; ++p) {
+                const float r = bm_radius(w[2 * p]);
+                float sn, cs;
+                sincos_turn_u24(w[2 * p + 1], sn, cs);
+                const float rs = pt[i].sigma * r;
+                d[i][2 * p] = rs * sn;
+                d[i][2 * p + 1] = rs * cs;
+            }
+            if (GAUSS) {
+                const U4 b3 = philox10(B * c + 4u, pt[i].id, video, sim, k0, k1);
+                wdet[i][0] = b3.x;
+                wdet[i][1] = b3.y;
+                wdet[i][2] = b3.z;
+                wdet[i][3] = b3.w;
+            }
+        }
+#pragma unroll
+        for (uint32_t s = 0; s < 4u; ++s) {
+            if (s >= nst) break;
+#pragma unroll
+            for (int i = 0; i < PPL; ++i) {
+                Particle& q = pt[i];
+                const float dx = d[i][3 * s], dy = d[i][3 * s + 1], dz = d[i][3 * s + 2];
+                q.x = wrap_periodic(q.x + dx, hx, Lx);
+                q.y = wrap_periodic(q.y + dy, hy, Ly);
+                q.z = ZREF ? wrap_reflect(q.z + dz, hz, Lz) : wrap_periodic(q.z + dz, hz, Lz);
+                bool det;
+                if (GAUSS)
+                    det = __builtin_fabsf(q.x) <= hax && __builtin_fabsf(q.y) <= hay &&
+                          (q.z * q.z) <= sig2 * neg2log_u24(wdet[i][s]);
+                else
+                    det = __builtin_fabsf(q.z) <= q.haz && __builtin_fabsf(q.x) <= hax &&
+                          __builtin_fabsf(q.y) <= hay;
+                const bool closing = q.active && !det && q.k >= kmin;
+                if (ballot(closing) != 0ull) {
+                    uint32_t rv = 0u;
+                    bool ok = false;
+                    if (closing) ok = track_record(q.k, q.de, four_dt, dde, n_k, n_de, rv);
+                    stage_push(st, ok, rv);
+                }
+                const float de_n = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, q.de));
+                q.de = (det && q.k >= 1u) ? de_n : 0.0f;
+                q.k = det ? q.k + 1u : 0u;
+            }
+        }
+    }
+    // Tracks still open in the last frame (src/cpu/simulate.jl:114-121).
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+        const bool closing = pt[i].active && pt[i].k >= kmin;
+        if (ballot(closing) != 0ull) {
+            uint32_t rv = 0u;
+            bool ok = false;
+            if (closing) ok = track_record(pt[i].k, pt[i].de, four_dt, dde, n_k, n_de, rv);
+            stage_push(st, ok, rv);
+        }
+    }
+    stage_flush(st);
+}
+
+template <int PPL, bool PERSIST, bool GAUSS, bool ZREF>
+__global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
+{
+    extern __shared__ uint32_t stage_lds[];
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wpg = blockDim.x / kWave;
+    Stage st;
+    st.buf = stage_lds + wave * a.stage_cap;
+    st.cap = a.stage_cap;
+    st.cnt = 0u;
+    const uint64_t seed = *a.seed;
+    const uint32_t k0 = uniform((uint32_t)seed), k1 = uniform((uint32_t)(seed >> 32));
+    if (PERSIST) {
+        for (;;) {
+            uint32_t t = 0u;
+            if (lane_id() == 0u) t = atomicAdd(a.tile_counter, 1u);
+            t = uniform(t) + a.tile_lo;
+            if (t >= a.tile_hi) break;
+            process_tile<PPL, GAUSS, ZREF>(a, t, st, k0, k1);
+        }
+    } else {
+        const uint32_t t = a.tile_lo + blockIdx.x * wpg + wave;
+        if (t < a.tile_hi) process_tile<PPL, GAUSS, ZREF>(a, t, st, k0, k1);
+    }
+}
+
+// --------------------------------------------------------------- distance
+
+constexpr int kDistThreads = 256;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total)
+{
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, kWave);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == kWave - 1) wsum[wave] = x;
+    __syncthreads();
+    uint32_t base = 0u, tot = 0u;
+#pragma unroll
+    for (int w = 0; w < kDistThreads / kWave; ++w) {
+        const uint32_t s = wsum[w];
+        base += ((uint32_t)w < wave) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return base + x - v;
+}
+
+template <int SEG>
+__global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
+{
+    __shared__ uint32_t rowc[PCS_MAX_NK];
+    __shared__ uint32_t colh[PCS_MAX_NDE];
+    __shared__ uint32_t wsum[kDistThreads / kWave];
+    __shared__ uint32_t smax;
+    __shared__ uint64_t red[kDistThreads];
+
+    const uint32_t P = a.prop_lo + blockIdx.x;
+    const DevProp& pr = a.props[P];
+    const DevSys& sy = a.sys[pr.sys_id];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n_k = sy.n_k, n_de = sy.n_de;
+    const uint32_t count = a.rec_count[P];
+    const uint32_t* rec = pr.rec;
+    uint32_t* srt = pr.srt;
+
+    if (count > pr.rec_cap) {  // arena slice overflowed: the host reruns it
+        if (tid == 0) {
+            a.dist[P] = __builtin_nan("");
+            a.exact[P] = 0ull;
+            a.tracks[P] = count;
+            a.status[P] = PCS_EOVERFLOW;
+        }
+        return;
+    }
+    const uint32_t T = count;
+
+    for (uint32_t i = tid; i < n_k; i += kDistThreads) rowc[i] = 0u;
+    for (uint32_t i = tid; i < n_de; i += kDistThreads) colh[i] = 0u;
+    if (tid == 0) smax = 0u;
+    __syncthreads();
+
+    // Row histogram of the records and the deepest occupied row.
+    uint32_t mr = 0u;
+    for (uint32_t t = tid; t < T; t += kDistThreads) {
+        const uint32_t r = rec[t] >> 16;
+        atomicAdd(&rowc[r], 1u);
+        mr = max(mr, r);
+    }
+    atomicMax(&smax, mr);
+    __syncthreads();
+
+    // Exclusive scan of the row counts: rowc[r] becomes the first slot of row r.
+    {
+        const uint32_t ch = (n_k + kDistThreads - 1) / kDistThreads;
+        const uint32_t b0 = min(tid * ch, n_k), b1 = min(b0 + ch, n_k);
+        uint32_t run = 0u;
+        for (uint32_t i = b0; i < b1; ++i) run += rowc[i];
+        uint32_t tot;
+        uint32_t ex = block_excl_scan(run, wsum, tot);
+        for (uint32_t i = b0; i < b1; ++i) {
+            const uint32_t c = rowc[i];
+            rowc[i] = ex;
+            ex += c;
+        }
+    }
+    __syncthreads();
+
+    // Scatter bins by row; afterwards rowc[r] is one past the last slot of row r.
+    for (uint32_t t = tid; t < T; t += kDistThreads) {
+        const uint32_t rv = rec[t];
+        const uint32_t pos = atomicAdd(&rowc[rv >> 16], 1u);
+        srt[pos] = rv & 0xffffu;
+    }
+    __threadfence_block();
+    __syncthreads();
+
+    const uint32_t last = max(smax, sy.maxrow_obs);
+    const uint32_t j0 = tid * SEG;
+    uint32_t Ccol[SEG];
+#pragma unroll
+    for (int m = 0; m < SEG; ++m) Ccol[m] = 0u;
+    uint64_t acc = 0ull;
+    const bool l1 = sy.dist_kind == PCS_DIST_L1;
+    const uint32_t* R = sy.obs_sat;
+
+    for (uint32_t i = 0; i <= last; ++i) {
+        const uint32_t s0 = (i == 0u) ? 0u : rowc[i - 1u];
+        const uint32_t s1 = rowc[i];
+        if (s1 > s0) {
+            for (uint32_t t = s0 + tid; t < s1; t += kDistThreads) atomicAdd(&colh[srt[t]], 1u);
+            __syncthreads();
+            uint32_t pre[SEG];
+            uint32_t run = 0u;
+#pragma unroll
+            for (int m = 0; m < SEG; ++m) {
+                run += (j0 + m < n_de) ? colh[j0 + m] : 0u;
+                pre[m] = run;
+            }
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan(run, wsum, tot);
+#pragma unroll
+            for (int m = 0; m < SEG; ++m) Ccol[m] += ex + pre[m];
+#pragma unroll
+            for (int m = 0; m < SEG; ++m)
+                if (j0 + m < n_de) colh[j0 + m] = 0u;
+            __syncthreads();
+        }
+        uint64_t racc = 0ull;
+        const uint32_t* Rrow = R + (uint64_t)i * n_de;
+#pragma unroll
+        for (int m = 0; m < SEG; ++m) {
+            if (j0 + m < n_de) {
+                const uint32_t rv = Rrow[j0 + m];
+                const uint32_t dd = Ccol[m] > rv ? Ccol[m] - rv : rv - Ccol[m];
+                racc += l1 ? (uint64_t)dd : (uint64_t)dd * dd;
+            }
+        }
+        // Rows past `last` repeat row `last` in both tables.
+        acc += (i == last) ? racc * (uint64_t)(n_k - last) : racc;
+    }
+
+    red[tid] = acc;
+    __syncthreads();
+    for (uint32_t s = kDistThreads / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const uint64_t tot = red[0];
+        a.exact[P] = tot;
+        a.dist[P] = log10((double)tot);
+        a.tracks[P] = T;
+        a.status[P] = PCS_OK;
+    }
+}
+
+// ----------------------------------------------------------------- helpers
+
+__global__ void reset_kernel(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rec_count[i] = 0u;
+    if (i == 0 && tile_counter) *tile_counter = 0u;
+}
+
+__global__ void dense_hist_kernel(const DevProp* props, uint32_t prop_lo, const uint32_t* rec_count, uint32_t n_k, uint32_t n_de, uint32_t* hist)
+{
+    const uint32_t P = prop_lo + blockIdx.y;
+    const DevProp& pr = props[P];
+    const uint32_t T = min(rec_count[P], pr.rec_cap);
+    uint32_t* H = hist + (uint64_t)blockIdx.y * n_k * n_de;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+        const uint32_t rv = pr.rec[t];
+        atomicAdd(&H[(uint64_t)(rv >> 16) * n_de + (rv & 0xffffu)], 1u);
+    }
+}
+
+__global__ void selftest_math_kernel(const uint32_t* w, float* radius, float* s, float* c, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = w[i];
+    radius[i] = bm_radius(v);
+    float sn, cs;
+    sincos_turn_u24(v, sn, cs);
+    s[i] = sn;
+    c[i] = cs;
+}
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, float* c, uint64_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const uint64_t nb = (n + 255u) / 256u;
+    hipLaunchKernelGGL(selftest_math_kernel, dim3((uint32_t)nb), dim3(256), 0, st, w, radius, s, c, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter, hipStream_t s)
+{
+    const uint32_t nb = (n + 255u) / 256u;
+    hipLaunchKernelGGL(reset_kernel, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, rec_count, n, tile_counter);
+    return hipGetLastError();
+}
+
+template <int PPL, bool PERSIST, bool GAUSS, bool ZREF>
+static hipError_t launch_walk_t(const WalkArgs& a, int wpg, uint32_t grid, hipStream_t s)
+{
+    const size_t shmem = (size_t)wpg * a.stage_cap * sizeof(uint32_t);
+    hipLaunchKernelGGL((walk_kernel<PPL, PERSIST, GAUSS, ZREF>), dim3(grid), dim3(wpg * kWave), shmem, s, a);
+    return hipGetLastError();
+}
+
+template <int PPL, bool PERSIST>
+static hipError_t launch_walk_m(const WalkArgs& a, int wpg, uint32_t grid, bool gauss, bool zref, hipStream_t s)
+{
+    if (gauss) return zref ? launch_walk_t<PPL, PERSIST, true, true>(a, wpg, grid, s)
+                           : launch_walk_t<PPL, PERSIST, true, false>(a, wpg, grid, s);
+    return zref ? launch_walk_t<PPL, PERSIST, false, true>(a, wpg, grid, s)
+                : launch_walk_t<PPL, PERSIST, false, false>(a, wpg, grid, s);
+}
+
+hipError_t launch_walk(const WalkArgs& a, int ppl, int wpg, bool persistent, uint32_t grid_groups,
+                       bool gauss, bool zref, hipStream_t s)
+{
+    if (a.tile_hi <= a.tile_lo) return hipSuccess;
+    uint32_t grid = grid_groups;
+    if (!persistent) {
+        const uint32_t n = a.tile_hi - a.tile_lo;
+        grid = (n + (uint32_t)wpg - 1u) / (uint32_t)wpg;
+    }
+    if (ppl == 2)
+        return persistent ? launch_walk_m<2, true>(a, wpg, grid, gauss, zref, s)
+                          : launch_walk_m<2, false>(a, wpg, grid, gauss, zref, s);
+    return persistent ? launch_walk_m<1, true>(a, wpg, grid, gauss, zref, s)
+                      : launch_walk_m<1, false>(a, wpg, grid, gauss, zref, s);
+}
+
+hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s)
+{
+    if (a.n_props == 0) return hipSuccess;
+    const dim3 g(a.n_props), b(kDistThreads);
+    if (n_de_max <= 256) hipLaunchKernelGGL(distance_kernel<1>, g, b, 0, s, a);
+    else if (n_de_max <= 512) hipLaunchKernelGGL(distance_kernel<2>, g, b, 0, s, a);
+    else if (n_de_max <= 1024) hipLaunchKernelGGL(distance_kernel<4>, g, b, 0, s, a);
+    else if (n_de_max <= 2048) hipLaunchKernelGGL(distance_kernel<8>, g, b, 0, s, a);
+    else hipLaunchKernelGGL(distance_kernel<16>, g, b, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_hist(const DevProp* props, uint32_t prop_lo, uint32_t n_props, const uint32_t* rec_count, uint32_t max_count, uint32_t n_k, uint32_t n_de,
+                             uint32_t* hist, hipStream_t s)
+{
+    if (n_props == 0) return hipSuccess;
+    uint32_t gx = (max_count + 255u) / 256u;
+    gx = gx < 1u ? 1u : (gx > 1024u ? 1024u : gx);
+    hipLaunchKernelGGL(dense_hist_kernel, dim3(gx, n_props), dim3(256), 0, s, props, prop_lo, rec_count,
+                       n_k, n_de, hist);
+    return hipGetLastError();
+}
+
+}  // namespace pcs

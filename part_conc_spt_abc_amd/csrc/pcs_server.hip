@@ -1,0 +1,532 @@
+// pcs_server.hip -- serving form of the hot path: continuous batching of
+// simulation requests from many concurrent ABC-PMC chains on one GPU.
+//
+// Replaces the reference's job queue, per-device worker thread, queue pool
+// and stream callback (src/gpu/detail/simulation_impl.inl:631-730, 889-901,
+// 1023-1136). Design:
+//   - models (systems) and their observed summed-area tables are registered
+//     once and stay resident in HBM;
+//   - submit() copies a request into a bounded host queue and returns;
+//   - one scheduler thread admits queued requests while the record arena (the
+//     HBM footprint that grows with a simulation's size) has room and fewer
+//     than max_inflight simulations are on the GPU, packs every request it
+//     admitted in this pass into one launch group (walk + distance kernels),
+//     and issues it on the next of several streams, so groups admitted while
+//     others run overlap on the device;
+//   - the distance kernel writes each result straight into pinned host
+//     memory; the scheduler retires groups whose completion event fired,
+//     frees their arena ranges and hands completions to poll().
+// The random stream is counter based, keyed per request, so a request's
+// result is the same whatever it shares the GPU with.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "pcs_common.h"
+
+using namespace pcs;
+
+namespace {
+
+constexpr int kStreams = 4;
+constexpr uint32_t kMaxSystems = 64;
+
+struct SysEntry {
+    pcs_system sys;
+    uint32_t* d_sat = nullptr;
+};
+
+struct Pending {
+    pcs_request req;
+    std::vector<uint32_t> frames, particles;
+    double submit_us;
+    uint32_t cap;
+};
+
+// First-fit allocator over the record arena (offsets in records).
+class Arena {
+public:
+    void init(uint64_t n) { free_[0] = n; }
+    bool alloc(uint64_t n, uint64_t& off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                off = it->first;
+                const uint64_t rest = it->second - n;
+                const uint64_t noff = it->first + n;
+                free_.erase(it);
+                if (rest) free_[noff] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n)
+    {
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+private:
+    std::map<uint64_t, uint64_t> free_;
+};
+
+struct Group {
+    bool busy = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    // device tables of this group
+    DevProp* d_props = nullptr;
+    uint32_t *d_vframes = nullptr, *d_vparticles = nullptr, *d_vprop = nullptr, *d_vtile = nullptr;
+    uint32_t* d_count = nullptr;
+    uint64_t* d_exact = nullptr;
+    // pinned host staging (uploads) and results (written by the distance kernel)
+    DevProp* h_props = nullptr;
+    uint32_t *h_vframes = nullptr, *h_vparticles = nullptr, *h_vprop = nullptr, *h_vtile = nullptr;
+    double* h_dist = nullptr;
+    uint32_t* h_tracks = nullptr;
+    int32_t* h_status = nullptr;
+    // bookkeeping
+    std::vector<uint64_t> tags;
+    std::vector<double> submit_us;
+    std::vector<uint64_t> arena_off;
+    std::vector<uint32_t> arena_n;
+    uint32_t n = 0;
+};
+
+}  // namespace
+
+struct pcs_server {
+    int device = 0;
+    pcs_server_config cfg{};
+    std::vector<SysEntry> systems;
+    DevSys* d_sys = nullptr;
+    uint64_t* d_zero_seed = nullptr;
+    uint32_t *d_rec = nullptr, *d_srt = nullptr;
+    uint64_t arena_records = 0;
+    Arena arena;
+    Group groups[kStreams];
+    int next_group = 0;
+    uint32_t inflight = 0;
+    uint32_t max_queue = 0;
+
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Pending> queue;
+    std::deque<pcs_completion> done;
+    uint32_t outstanding = 0;  // submitted, not yet polled
+    bool stop = false;
+    int fatal = PCS_OK;
+    std::thread worker;
+    std::chrono::steady_clock::time_point t0;
+
+    double now_us() const
+    {
+        return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t n) { return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)); }
+template <typename T>
+hipError_t hmalloc(T** p, size_t n) { return hipHostMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault); }
+
+hipError_t group_alloc(Group& g, uint32_t max_props, uint32_t max_videos)
+{
+    hipError_t e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = dmalloc(&g.d_props, max_props);
+    if (e == hipSuccess) e = dmalloc(&g.d_vframes, max_videos);
+    if (e == hipSuccess) e = dmalloc(&g.d_vparticles, max_videos);
+    if (e == hipSuccess) e = dmalloc(&g.d_vprop, max_videos);
+    if (e == hipSuccess) e = dmalloc(&g.d_vtile, max_videos + 1);
+    if (e == hipSuccess) e = dmalloc(&g.d_count, max_props);
+    if (e == hipSuccess) e = dmalloc(&g.d_exact, max_props);
+    if (e == hipSuccess) e = hmalloc(&g.h_props, max_props);
+    if (e == hipSuccess) e = hmalloc(&g.h_vframes, max_videos);
+    if (e == hipSuccess) e = hmalloc(&g.h_vparticles, max_videos);
+    if (e == hipSuccess) e = hmalloc(&g.h_vprop, max_videos);
+    if (e == hipSuccess) e = hmalloc(&g.h_vtile, max_videos + 1);
+    if (e == hipSuccess) e = hmalloc(&g.h_dist, max_props);
+    if (e == hipSuccess) e = hmalloc(&g.h_tracks, max_props);
+    if (e == hipSuccess) e = hmalloc(&g.h_status, max_props);
+    return e;
+}
+
+void group_free(Group& g)
+{
+    if (g.stream) hipStreamSynchronize(g.stream);
+    void* dev[] = {g.d_props, g.d_vframes, g.d_vparticles, g.d_vprop, g.d_vtile, g.d_count, g.d_exact};
+    for (void* p : dev)
+        if (p) hipFree(p);
+    void* host[] = {g.h_props, g.h_vframes, g.h_vparticles, g.h_vprop, g.h_vtile, g.h_dist, g.h_tracks, g.h_status};
+    for (void* p : host)
+        if (p) hipHostFree(p);
+    if (g.done) hipEventDestroy(g.done);
+    if (g.stream) hipStreamDestroy(g.stream);
+    g = Group{};
+}
+
+// Issue one launch group on g from the admitted requests.
+int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vector<uint64_t>& offs)
+{
+    const uint32_t n = (uint32_t)batch.size();
+    uint32_t nv = 0;
+    g.tags.resize(n);
+    g.submit_us.resize(n);
+    g.arena_off.assign(offs.begin(), offs.end());
+    g.arena_n.resize(n);
+    uint32_t n_de_max = 1;
+    uint32_t classes = 0;  // bit (gauss*2 + zref) present
+    for (uint32_t i = 0; i < n; ++i) {
+        const Pending& p = batch[i];
+        const pcs_system& sy = s->systems[p.req.sys_id].sys;
+        DevProp& d = g.h_props[i];
+        std::memset(&d, 0, sizeof(d));
+        d.sim_id = p.req.prop.sim_id;
+        d.first_video = nv;
+        d.n_videos = p.req.n_videos;
+        d.sys_id = p.req.sys_id;
+        d.rec = s->d_rec + offs[i];
+        d.srt = s->d_srt + offs[i];
+        d.rec_cap = p.cap;
+        d.key0 = (uint32_t)p.req.seed;
+        d.key1 = (uint32_t)(p.req.seed >> 32);
+        for (int c = 0; c < PCS_MAX_SPECIES; ++c) {
+            d.half_az[c] = p.req.prop.half_az[c];
+            d.cum_prob[c] = p.req.prop.cum_prob[c];
+            d.sigma[c] = p.req.prop.sigma[c];
+        }
+        for (uint32_t v = 0; v < p.req.n_videos; ++v, ++nv) {
+            g.h_vframes[nv] = p.frames[v];
+            g.h_vparticles[nv] = p.particles[v];
+            g.h_vprop[nv] = i;
+        }
+        g.tags[i] = p.req.tag;
+        g.submit_us[i] = p.submit_us;
+        g.arena_n[i] = p.cap;
+        n_de_max = std::max(n_de_max, sy.n_de);
+        classes |= 1u << ((sy.detect_model == PCS_DETECT_GAUSS_Z ? 2 : 0) + (sy.z_boundary == PCS_Z_REFLECT ? 1 : 0));
+    }
+    g.n = n;
+    hipStream_t st = g.stream;
+    hipError_t e = hipMemcpyAsync(g.d_props, g.h_props, n * sizeof(DevProp), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g.d_vframes, g.h_vframes, nv * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g.d_vparticles, g.h_vparticles, nv * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g.d_vprop, g.h_vprop, nv * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_reset(g.d_count, n, nullptr, st);
+    // One walk launch per (detection, boundary) class present: tiles of the
+    // other classes are planned empty for it (zero-tile videos).
+    for (uint32_t cls = 0; cls < 4 && e == hipSuccess; ++cls) {
+        if (!(classes & (1u << cls))) continue;
+        uint32_t acc = 0;
+        for (uint32_t v = 0; v < nv; ++v) {
+            g.h_vtile[v] = acc;
+            const pcs_system& sy = s->systems[g.h_props[g.h_vprop[v]].sys_id].sys;
+            const uint32_t vc = (sy.detect_model == PCS_DETECT_GAUSS_Z ? 2u : 0u) + (sy.z_boundary == PCS_Z_REFLECT ? 1u : 0u);
+            if (vc == cls) acc += (g.h_vparticles[v] + 63u) / 64u;
+        }
+        g.h_vtile[nv] = acc;
+        e = hipMemcpyAsync(g.d_vtile, g.h_vtile, (nv + 1) * 4, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) break;
+        WalkArgs a{};
+        a.sys = s->d_sys;
+        a.props = g.d_props;
+        a.vframes = g.d_vframes;
+        a.vparticles = g.d_vparticles;
+        a.vprop = g.d_vprop;
+        a.vtile = g.d_vtile;
+        a.n_videos = nv;
+        a.tile_lo = 0;
+        a.tile_hi = acc;
+        a.stage_cap = 256;
+        a.rec_count = g.d_count;
+        a.seed = s->d_zero_seed;
+        a.tile_counter = nullptr;
+        e = launch_walk(a, 1, 4, false, 0, (cls & 2u) != 0, (cls & 1u) != 0, st);
+        // the tile table is overwritten by the next class: wait for this walk
+        if (e == hipSuccess && (classes >> (cls + 1))) e = hipStreamSynchronize(st);
+    }
+    if (e == hipSuccess) {
+        DistArgs d{};
+        d.sys = s->d_sys;
+        d.props = g.d_props;
+        d.prop_lo = 0;
+        d.n_props = n;
+        d.rec_count = g.d_count;
+        d.dist = g.h_dist;
+        d.exact = g.d_exact;
+        d.tracks = g.h_tracks;
+        d.status = g.h_status;
+        e = launch_distance(d, n_de_max, st);
+    }
+    if (e == hipSuccess) e = hipEventRecord(g.done, st);
+    if (e != hipSuccess) return hip_error(e, "serving launch");
+    g.busy = true;
+    return PCS_OK;
+}
+
+void retire_group(pcs_server* s, Group& g)
+{
+    const double t = s->now_us();
+    std::vector<pcs_completion> out(g.n);
+    for (uint32_t i = 0; i < g.n; ++i) {
+        pcs_completion& c = out[i];
+        c.tag = g.tags[i];
+        c.dist = g.h_dist[i];
+        c.tracks = g.h_tracks[i];
+        c.status = g.h_status[i];
+        c.submit_us = g.submit_us[i];
+        c.done_us = t;
+        s->arena.release(g.arena_off[i], g.arena_n[i]);
+    }
+    s->inflight -= g.n;
+    g.busy = false;
+    g.n = 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto& c : out) s->done.push_back(c);
+    s->cv.notify_all();
+}
+
+void scheduler(pcs_server* s)
+{
+    hipSetDevice(s->device);
+    std::vector<Pending> batch;
+    std::vector<uint64_t> offs;
+    for (;;) {
+        bool idle = true;
+        // Retire finished groups.
+        for (Group& g : s->groups) {
+            if (!g.busy) continue;
+            const hipError_t q = hipEventQuery(g.done);
+            if (q == hipSuccess) {
+                retire_group(s, g);
+                idle = false;
+            } else if (q != hipErrorNotReady) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->fatal = hip_error(q, "serving group");
+                s->stop = true;
+                s->cv.notify_all();
+                return;
+            }
+        }
+        // Admit and launch into the next free group.
+        Group* free_g = nullptr;
+        for (int k = 0; k < kStreams; ++k) {
+            Group& g = s->groups[(s->next_group + k) % kStreams];
+            if (!g.busy) {
+                free_g = &g;
+                s->next_group = (s->next_group + k + 1) % kStreams;
+                break;
+            }
+        }
+        batch.clear();
+        offs.clear();
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->stop) return;
+            if (free_g) {
+                uint32_t nv = 0;
+                while (!s->queue.empty() && s->inflight + batch.size() < s->cfg.max_inflight) {
+                    Pending& p = s->queue.front();
+                    if (nv + p.req.n_videos > s->cfg.max_inflight * s->cfg.max_videos) break;
+                    uint64_t off;
+                    if (!s->arena.alloc(p.cap, off)) break;  // wait for memory to free up
+                    offs.push_back(off);
+                    nv += p.req.n_videos;
+                    batch.push_back(std::move(p));
+                    s->queue.pop_front();
+                }
+            }
+        }
+        if (!batch.empty()) {
+            s->inflight += (uint32_t)batch.size();
+            const int r = launch_group(s, *free_g, batch, offs);
+            if (r != PCS_OK) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->fatal = r;
+                s->stop = true;
+                s->cv.notify_all();
+                return;
+            }
+            idle = false;
+        }
+        if (idle) {
+            bool any_busy = false;
+            for (Group& g : s->groups) any_busy |= g.busy;
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->stop) return;
+            if (!any_busy && s->queue.empty())
+                s->cv.wait_for(lk, std::chrono::milliseconds(2));
+            else
+                lk.unlock(), std::this_thread::yield();
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcs_server_create(int device, const pcs_server_config* cfg, pcs_server** out)
+{
+    if (!out) return set_error(PCS_EINVAL, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PCS_ENODEV, "no GPU visible");
+    if (device < 0 || device >= ndev) return set_error(PCS_EINVAL, "device %d outside [0, %d)", device, ndev);
+    pcs_server* s = new (std::nothrow) pcs_server();
+    if (!s) return set_error(PCS_ENOMEM, "server");
+    s->device = device;
+    if (cfg) s->cfg = *cfg;
+    if (!s->cfg.arena_bytes) s->cfg.arena_bytes = 1ull << 30;
+    if (!s->cfg.max_inflight) s->cfg.max_inflight = 4096;
+    if (!s->cfg.max_videos) s->cfg.max_videos = 64;
+    s->max_queue = 4 * s->cfg.max_inflight;
+    s->systems.reserve(kMaxSystems);  // the scheduler reads entries without the lock
+    s->arena_records = s->cfg.arena_bytes / 8;  // records + sort scratch
+    s->t0 = std::chrono::steady_clock::now();
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = dmalloc(&s->d_sys, kMaxSystems);
+    if (e == hipSuccess) e = dmalloc(&s->d_zero_seed, 1);
+    if (e == hipSuccess) e = hipMemset(s->d_zero_seed, 0, 8);
+    if (e == hipSuccess) e = dmalloc(&s->d_rec, s->arena_records);
+    if (e == hipSuccess) e = dmalloc(&s->d_srt, s->arena_records);
+    for (int k = 0; k < kStreams && e == hipSuccess; ++k)
+        e = group_alloc(s->groups[k], s->cfg.max_inflight, s->cfg.max_inflight * s->cfg.max_videos);
+    if (e != hipSuccess) {
+        const int r = hip_error(e, "pcs_server_create");
+        pcs_server_destroy(s);
+        return r;
+    }
+    s->arena.init(s->arena_records);
+    s->worker = std::thread(scheduler, s);
+    *out = s;
+    return PCS_OK;
+}
+
+int pcs_server_add_system(pcs_server* s, const pcs_system* sys, const uint32_t* obs_hist, uint32_t* sys_id)
+{
+    if (!s || !sys_id) return set_error(PCS_EINVAL, "NULL argument");
+    const int r = validate_system(sys);
+    if (r != PCS_OK) return r;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->systems.size() >= kMaxSystems) return set_error(PCS_EINVAL, "more than %u systems", kMaxSystems);
+    std::vector<uint32_t> sat;
+    uint32_t maxrow = 0;
+    build_sat(obs_hist, sys->n_k, sys->n_de, sat, maxrow);
+    SysEntry ent;
+    ent.sys = *sys;
+    hipError_t e = hipSetDevice(s->device);
+    if (e == hipSuccess) e = dmalloc(&ent.d_sat, sat.size());
+    if (e == hipSuccess) e = hipMemcpy(ent.d_sat, sat.data(), sat.size() * 4, hipMemcpyHostToDevice);
+    const DevSys d = make_devsys(*sys, maxrow, ent.d_sat);
+    const uint32_t id = (uint32_t)s->systems.size();
+    if (e == hipSuccess) e = hipMemcpy(s->d_sys + id, &d, sizeof(DevSys), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (ent.d_sat) hipFree(ent.d_sat);
+        return hip_error(e, "pcs_server_add_system");
+    }
+    s->systems.push_back(ent);
+    *sys_id = id;
+    return PCS_OK;
+}
+
+int pcs_server_submit(pcs_server* s, const pcs_request* req)
+{
+    if (!s || !req) return set_error(PCS_EINVAL, "NULL argument");
+    if (req->n_videos == 0 || req->n_videos > s->cfg.max_videos)
+        return set_error(PCS_EINVAL, "n_videos %u outside [1, %u]", req->n_videos, s->cfg.max_videos);
+    if (!req->frames || !req->particles) return set_error(PCS_EINVAL, "NULL video arrays");
+    Pending p;
+    p.req = *req;
+    p.frames.assign(req->frames, req->frames + req->n_videos);
+    p.particles.assign(req->particles, req->particles + req->n_videos);
+    p.req.frames = nullptr;
+    p.req.particles = nullptr;
+    p.req.prop.first_video = 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->fatal != PCS_OK) return set_error(s->fatal, "server stopped after an error");
+    if (req->sys_id >= s->systems.size()) return set_error(PCS_EINVAL, "unknown sys_id %u", req->sys_id);
+    const pcs_system& sy = s->systems[req->sys_id].sys;
+    if (req->prop.reserved != 0) return set_error(PCS_EINVAL, "reserved field must be 0");
+    int r = validate_proposal(sy, req->prop);
+    if (r != PCS_OK) return r;
+    p.cap = record_capacity(sy, req->prop, p.frames.data(), p.particles.data(), 1.0);
+    if (p.cap > s->arena_records)
+        return set_error(PCS_EINVAL, "request needs %u records, arena holds %llu", p.cap,
+                         (unsigned long long)s->arena_records);
+    if (s->queue.size() >= s->max_queue) return PCS_EAGAIN;
+    p.submit_us = s->now_us();
+    s->queue.push_back(std::move(p));
+    ++s->outstanding;
+    s->cv.notify_all();
+    return PCS_OK;
+}
+
+int pcs_server_poll(pcs_server* s, pcs_completion* out, uint32_t max, uint32_t* n_out, uint32_t wait_us)
+{
+    if (!s || !n_out || (max && !out)) return set_error(PCS_EINVAL, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_us && s->fatal == PCS_OK)
+        s->cv.wait_for(lk, std::chrono::microseconds(wait_us), [s] { return !s->done.empty() || s->stop; });
+    uint32_t n = 0;
+    while (n < max && !s->done.empty()) {
+        out[n++] = s->done.front();
+        s->done.pop_front();
+    }
+    s->outstanding -= n;
+    *n_out = n;
+    if (n == 0 && s->fatal != PCS_OK) return set_error(s->fatal, "server stopped after an error");
+    return PCS_OK;
+}
+
+int pcs_server_pending(pcs_server* s, uint32_t* n)
+{
+    if (!s || !n) return set_error(PCS_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *n = s->outstanding;
+    return PCS_OK;
+}
+
+int pcs_server_destroy(pcs_server* s)
+{
+    if (!s) return PCS_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    hipSetDevice(s->device);
+    for (Group& g : s->groups) group_free(g);
+    for (SysEntry& e : s->systems)
+        if (e.d_sat) hipFree(e.d_sat);
+    void* ptrs[] = {s->d_sys, s->d_zero_seed, s->d_rec, s->d_srt};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete s;
+    return PCS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Integer outputs (histograms, track counts, exact distance sums) must be
+bit-exact; the device FP32 math must equal the oracle's bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib as O
+from part_conc_spt_abc_amd import _abi
+from part_conc_spt_abc_amd.gpu import Context, selftest_math
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_hists(model, props, frames, particles, seed):
+    osys = O.system_from_abi(model.to_abi())
+    out = []
+    for p, (f, n) in zip(props, cases.split_videos(props, frames, particles)):
+        H, total, dropped = O.simulate_f32(osys, O.OracleProposal(p, f, n), seed)
+        out.append((H, total - dropped))
+    return out
+
+
+def run_gpu(ctx, props, frames, particles, seed, mode):
+    b = ctx.batch()
+    b.load(props, frames, particles)
+    b.run(seed, mode)
+    dist, tracks, H = b.read(hist=True)
+    exact = b.read_exact()
+    reruns = b.overflow_reruns()
+    b.close()
+    return dist, tracks, H, exact, reruns
+
+
+def check_against_oracle(model, obs, props, frames, particles, seed, dist, tracks, H, exact):
+    ref = oracle_hists(model, props, frames, particles, seed)
+    for i, (Ho, n_ok) in enumerate(ref):
+        assert np.array_equal(H[i], Ho), f"proposal {i}: histogram differs ({np.abs(H[i].astype(np.int64) - Ho).sum()} counts)"
+        assert tracks[i] == n_ok, f"proposal {i}: tracks {tracks[i]} != {n_ok}"
+        ex, lg = O.distance(Ho, obs, model.dist_kind)
+        assert int(exact[i]) == ex, f"proposal {i}: exact distance {int(exact[i])} != {ex}"
+        if ex > 0:
+            assert dist[i] == pytest.approx(math.log10(ex), rel=0, abs=1e-12)
+
+
+def test_device_math_bit_exact_all_inputs(gpu_lib):
+    # The walk consumes 24 bits of each word: every possible input.
+    w = (np.arange(1 << 24, dtype=np.uint64) << 8).astype(np.uint32)
+    r, s, c = selftest_math(w)
+    ro = O.bm_radius(w)
+    so, co = O.sincos_turn(w)
+    assert np.array_equal(r.view(np.uint32), ro.view(np.uint32))
+    assert np.array_equal(s.view(np.uint32), so.view(np.uint32))
+    assert np.array_equal(c.view(np.uint32), co.view(np.uint32))
+
+
+FIVE_CLASSES = [cases.stokes_einstein(d) for d in (100, 200, 300, 450, 600)]
+
+
+def observed(model, seed=0, particles=1200, frames=None):
+    """Data histogram from the oracle at the true parameters (seed 0)."""
+    frames = frames or [model.n_k]
+    m = FIVE_CLASSES[: model.n_species] if model.n_species != 2 else None
+    p, f, n = cases.fixed_proposal(model, [particles] * len(frames), frames, m=m, sim_id=0xFFFFFFF0)
+    H, _, _ = O.simulate_f32(O.system_from_abi(model.to_abi()), O.OracleProposal(p, f, n), seed)
+    return H
+
+
+@pytest.mark.parametrize("mode", [_abi.PCS_MODE_PLAIN, _abi.PCS_MODE_BATCHED, _abi.PCS_MODE_BATCHED_DIRECT])
+def test_headline_single_simulation(gpu_lib, mode):
+    model = cases.cpu_model()  # 500 frames, H 500 x 200
+    obs = observed(model)
+    p, f, n = cases.fixed_proposal(model, [1200], [500], sim_id=7)
+    with Context(model, obs) as ctx:
+        dist, tracks, H, exact, _ = run_gpu(ctx, [p], f, n, 1, mode)
+    print(f"mode {mode}: tracks {tracks[0]} dist {dist[0]:.6f}", flush=True)
+    check_against_oracle(model, obs, [p], f, n, 1, dist, tracks, H, exact)
+
+
+def test_batched_mixed_sizes(gpu_lib):
+    model = cases.headline_model()  # H 1000 x 500
+    obs = observed(model)
+    rng = np.random.default_rng(3)
+    props, f, n = cases.draw_proposals(model, 24, rng, frames_choices=(200, 500, 1000, 2000), c_range=(1e8, 2e9))
+    print("particles", n.tolist(), "frames", f.tolist(), flush=True)
+    with Context(model, obs) as ctx:
+        for mode in (_abi.PCS_MODE_BATCHED, _abi.PCS_MODE_PLAIN):
+            dist, tracks, H, exact, _ = run_gpu(ctx, props, f, n, 11, mode)
+            check_against_oracle(model, obs, props, f, n, 11, dist, tracks, H, exact)
+
+
+@pytest.mark.parametrize("tune", [dict(particles_per_lane=2), dict(waves_per_group=1, stage_records=128),
+                                  dict(waves_per_group=2, groups_per_cu=2, stage_records=512)])
+def test_tuning_does_not_change_results(gpu_lib, tune):
+    model = cases.headline_model(n_k=600, n_de=300)
+    obs = observed(model, frames=[600])
+    rng = np.random.default_rng(5)
+    props, f, n = cases.draw_proposals(model, 10, rng, frames_choices=(300, 600), videos=3, c_range=(1e8, 1e9))
+    with Context(model, obs) as ctx:
+        ctx.set_tuning(**tune)
+        dist, tracks, H, exact, _ = run_gpu(ctx, props, f, n, 21, _abi.PCS_MODE_BATCHED)
+    check_against_oracle(model, obs, props, f, n, 21, dist, tracks, H, exact)
+
+
+@pytest.mark.parametrize("variant", ["reflect", "gauss", "five_species_l1", "variable_depth"])
+def test_model_variants(gpu_lib, variant):
+    kw = {}
+    m = az = None
+    if variant == "reflect":
+        kw = dict(z_boundary=_abi.PCS_Z_REFLECT)
+    elif variant == "gauss":
+        kw = dict(detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0)
+    elif variant == "five_species_l1":
+        kw = dict(n_species=5, dist_kind=_abi.PCS_DIST_L1)
+        m = FIVE_CLASSES
+    model = cases.headline_model(n_k=800, n_de=400, **kw)
+    obs = observed(model, frames=[800])
+    props = []
+    fr, pa = [], []
+    rng = np.random.default_rng(9)
+    for i in range(6):
+        C = model.n_species
+        mm = m or [cases.D175, cases.D500]
+        c = list(rng.dirichlet(np.ones(C)) * 1e9)
+        az = [float(x) for x in rng.uniform(0.5, 5.0, C)] if variant == "variable_depth" else [2.0]
+        props.append(cases.proposal_record(model, mm, c, az, 100 + i, len(fr), 2))
+        fr += [800, 400]
+        pa += [int(x) for x in rng.integers(50, 2000, 2)]
+    fr = np.array(fr, np.uint32)
+    pa = np.array(pa, np.uint32)
+    with Context(model, obs) as ctx:
+        dist, tracks, H, exact, _ = run_gpu(ctx, props, fr, pa, 5, _abi.PCS_MODE_BATCHED)
+    check_against_oracle(model, obs, props, fr, pa, 5, dist, tracks, H, exact)
+
+
+def test_overflow_reruns_are_exact(gpu_lib):
+    model = cases.cpu_model()
+    obs = observed(model)
+    rng = np.random.default_rng(1)
+    props, f, n = cases.draw_proposals(model, 8, rng, frames_choices=(500,), c_range=(5e8, 2e9))
+    with Context(model, obs) as ctx:
+        ctx.set_record_capacity_factor(0.01)
+        dist, tracks, H, exact, reruns = run_gpu(ctx, props, f, n, 2, _abi.PCS_MODE_BATCHED)
+    assert reruns > 0
+    check_against_oracle(model, obs, props, f, n, 2, dist, tracks, H, exact)
+
+
+def test_graph_replays_with_eager_work_between(gpu_lib):
+    model = cases.cpu_model()
+    obs = observed(model)
+    rng = np.random.default_rng(4)
+    props, f, n = cases.draw_proposals(model, 6, rng, frames_choices=(500,), c_range=(1e8, 1e9))
+    with Context(model, obs) as ctx:
+        b = ctx.batch()
+        b.load(props, f, n)
+        for seed in (3, 4, 3):
+            b.run(seed, _abi.PCS_MODE_BATCHED)
+            dist, tracks, H = b.read(hist=True)
+            exact = b.read_exact()
+            # eager work on the same context between replays
+            dist2, _, _, _, _ = run_gpu(ctx, props[:2], f[:2], n[:2], seed + 100, _abi.PCS_MODE_PLAIN)
+            check_against_oracle(model, obs, props, f, n, seed, dist, tracks, H, exact)
+        b.close()
+
+
+def test_empty_and_tiny_inputs(gpu_lib):
+    model = cases.cpu_model()
+    obs = observed(model)
+    # zero particles, one frame, kmin-length videos
+    props, fr, pa = [], [], []
+    for i, (np_, nf) in enumerate([(0, 500), (5, 1), (64, 3), (65, 2), (1, 500)]):
+        p, f, n = cases.fixed_proposal(model, [np_], [nf], sim_id=i)
+        p.first_video = len(fr)
+        props.append(p)
+        fr += list(f)
+        pa += list(n)
+    fr = np.array(fr, np.uint32)
+    pa = np.array(pa, np.uint32)
+    with Context(model, obs) as ctx:
+        for mode in (_abi.PCS_MODE_PLAIN, _abi.PCS_MODE_BATCHED):
+            dist, tracks, H, exact, _ = run_gpu(ctx, props, fr, pa, 8, mode)
+            check_against_oracle(model, obs, props, fr, pa, 8, dist, tracks, H, exact)
+
+
+def test_invalid_arguments_fail_loudly(gpu_lib):
+    model = cases.cpu_model()
+    bad = model.to_abi()
+    bad.kmin = 1
+    import ctypes as C
+
+    h = C.c_void_p()
+    assert gpu_lib.pcs_open(C.byref(bad), None, 0, C.byref(h)) == _abi.PCS_EINVAL
+    with Context(model) as ctx:
+        p, f, n = cases.fixed_proposal(model, [10], [10])
+        p.cum_prob[1] = 0.5  # last cumulative weight < 1
+        with pytest.raises(_abi.PcsError):
+            b = ctx.batch()
+            b.load([p], f, n)
