@@ -1,0 +1,318 @@
+#!/usr/bin/env python3
+"""Benchmark of the ABC-PMC simulation hot path on one MI355X.
+
+Metric: ABC simulations/s and particle-steps/s, batched vs plain, with p50/p99
+simulation latency. One timed step = one ABC generation of the batched
+workload (100,000 proposals, concentration log-uniform on 1e8..1e10 per mL
+-> 120..12,000 particles, frames uniform on {500, 1000, 2000}, H 1000 x 500
+uint32) run as one captured hipGraph launch (reset, persistent walk,
+distance) plus the read-back of the distances. Inputs are resident in HBM
+before timing starts. Beside it:
+  - plain form: 10,000 proposals of 1,000 frames, one walk + one distance
+    launch per simulation;
+  - serving form: Poisson arrivals of single-simulation requests from 32
+    chains with their own models, p50/p99 latency;
+  - roofline of the walk kernel from HIP events inside the timed replays;
+  - CPU baseline: the oracle's FP64 restatement of the reference's CPU
+    simulate + distance on a bounded sample of the same workload.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+from part_conc_spt_abc_amd import _abi  # noqa: E402
+from part_conc_spt_abc_amd.gpu import Context, Server  # noqa: E402
+from part_conc_spt_abc_amd.model import Model, proposal_record, stokes_einstein  # noqa: E402
+
+# Algorithmic VALU lane-operations per particle-step of the walk (Philox
+# 0.75 blocks x 104, Box-Muller 12 counting log/sqrt/sin/cos as one each,
+# 6 uniform conversions, 27 walk/wrap/detect/track) and gfx950's non-packed
+# VALU issue peak: 256 CU x 64 lanes x 2.4 GHz.
+OPS_PER_STEP = 123
+VALU_PEAK = 256 * 64 * 2.4e9
+
+D175 = stokes_einstein(175.0)
+D500 = stokes_einstein(500.0)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def headline_model() -> Model:
+    return Model(Lx=200.0, Ly=200.0, Lz=30.0, ax=160.0, ay=120.0, deltat=0.05, kmin=3, n_k=1000, n_de=500,
+                 ub_de=20.0, n_species=2)
+
+
+def workload(model, n, rng, frames_choices):
+    props, frames, parts = [], [], []
+    lam_f = model.volume_factor
+    for i in range(n):
+        ct = 10.0 ** rng.uniform(8.0, 10.0)
+        props.append(proposal_record(model, [D175, D500], [ct / 2, ct / 2], [2.0], i, i, 1))
+        frames.append(int(rng.choice(frames_choices)))
+        parts.append(int(rng.poisson(ct * lam_f)))
+    return props, np.array(frames, np.uint32), np.array(parts, np.uint32)
+
+
+def particle_steps(frames, parts):
+    return float(np.sum(parts.astype(np.float64) * (frames.astype(np.float64) - 1.0)))
+
+
+def observed_hist(ctx, model):
+    """Data histogram at the truth (c = 1e9/mL, 50/50), seed 0, 1000 frames."""
+    rng = np.random.default_rng(0)
+    n = int(rng.poisson(1e9 * model.volume_factor))
+    p = proposal_record(model, [D175, D500], [5e8, 5e8], [2.0], 0xFFFFFFF0, 0, 1)
+    b = ctx.batch()
+    b.load([p], np.array([1000], np.uint32), np.array([n], np.uint32))
+    b.run(0, _abi.PCS_MODE_BATCHED)
+    _, _, H = b.read(hist=True)
+    b.close()
+    return H[0]
+
+
+def run_timed(batch, steps, warmup, mode, seed0):
+    """Wall time per step (run + read of the distances) and the device times
+    of each step from HIP events on the batch stream (whole run, walk
+    kernel; the walk time only in PCS_MODE_BATCHED_DIRECT)."""
+    for i in range(warmup):
+        batch.run(seed0 + 1000 + i, mode)
+        batch.read()
+    walk_ms, total_ms, walls = [], [], []
+    for i in range(steps):
+        t0 = time.perf_counter()
+        batch.run(seed0 + i, mode)
+        batch.read()
+        walls.append(time.perf_counter() - t0)
+        tot, walk, _ = batch.timing()
+        total_ms.append(tot)
+        walk_ms.append(walk)
+    return walls, total_ms, walk_ms
+
+
+def serving_latency(n_requests, rate, seed=7):
+    """Poisson arrivals at `rate` req/s from 32 chains, each with its own box
+    (lateral 100-400 um, depth 10-50 um) and 2-4 species (100-600 nm)."""
+    rng = np.random.default_rng(seed)
+    srv = Server(device=0, arena_bytes=4 << 30, max_inflight=4096)
+    chains = []
+    for c in range(32):
+        L = float(rng.uniform(100, 400))
+        Lz = float(rng.uniform(10, 50))
+        C = int(rng.integers(2, 5))
+        m = Model(Lx=L, Ly=L, Lz=Lz, ax=0.8 * L, ay=0.6 * L, deltat=0.05, kmin=3, n_k=1000, n_de=500, ub_de=20.0,
+                  n_species=C)
+        sid = srv.add_system(m)
+        D = sorted(stokes_einstein(float(d)) for d in rng.uniform(100, 600, C))
+        chains.append((m, sid, D))
+    reqs = []
+    for i in range(n_requests):
+        m, sid, D = chains[i % 32]
+        ct = 10.0 ** rng.uniform(8.0, 10.0)
+        w = rng.dirichlet(np.ones(m.n_species))
+        p = proposal_record(m, D, list(w * ct), [2.0], i, 0, 1)
+        n = int(rng.poisson(ct * m.volume_factor))
+        reqs.append((sid, p, np.array([1000], np.uint32), np.array([min(n, 60000)], np.uint32)))
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    lat = []
+    done = 0
+    t0 = time.perf_counter()
+    next_t = t0
+    i = 0
+    while done < n_requests:
+        now = time.perf_counter()
+        while i < n_requests and now >= next_t:
+            sid, p, f, n = reqs[i]
+            if not srv.submit(i, sid, p, f, n, seed=12345):
+                break
+            i += 1
+            next_t += gaps[i - 1] if i < n_requests else 0.0
+        for c in srv.poll(4096, wait_us=200):
+            lat.append(c.done_us - c.submit_us)
+            done += 1
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = np.array(lat)
+    return {"rate_req_s": rate, "requests": n_requests, "throughput_req_s": n_requests / wall,
+            "p50_ms": float(np.percentile(lat, 50) / 1e3), "p99_ms": float(np.percentile(lat, 99) / 1e3)}
+
+
+def cpu_baseline(model, props, frames, parts, obs, n_sample):
+    """Oracle FP64 restatement of the reference CPU path (simulate.jl +
+    distance.jl) on the first n_sample proposals, one simulation per thread."""
+    import ctypes as C
+
+    import oracle_lib as O
+
+    osys = O.system_from_abi(model.to_abi())
+    jobs = []
+    for i in range(n_sample):
+        p = props[i]
+        op = O.OracleProposal(p, frames[i:i + 1], parts[i:i + 1])
+        pd = O.orc_proposal_f64()
+        for s in range(model.n_species):
+            pd.sigma[s] = math.sqrt(2.0 * [D175, D500][s] * model.deltat)
+            pd.half_az[s] = 1.0
+            pd.cum_prob[s] = [0.5, 2.0][s]
+        jobs.append((op, pd))
+    lib = O.lib()
+
+    def one(job):
+        op, pd = job
+        H = np.zeros((model.n_k, model.n_de), np.uint32)
+        dr = np.zeros(1, np.uint64)
+        lib.orc_simulate_f64(C.byref(osys), C.byref(op.s), C.byref(pd), model.ub_de, model.deltat, 1,
+                             H.ctypes.data_as(C.c_void_p), dr.ctypes.data_as(C.c_void_p))
+        return O.distance(H, obs, 0)[1]
+
+    cores = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        list(ex.map(one, jobs))
+    dt = time.perf_counter() - t0
+    steps = particle_steps(frames[:n_sample], parts[:n_sample])
+    return {"value": n_sample / dt, "unit": "sims/s", "cores": cores, "kind": "port",
+            "particle_steps_per_s": steps / dt,
+            "sample": f"first {n_sample} proposals of the batched workload, FP64 oracle simulate + dense distance, "
+                      f"{cores} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batched-sims", type=int, default=100_000)
+    ap.add_argument("--plain-sims", type=int, default=10_000)
+    ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--serve-rate", type=float, default=5000.0)
+    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is sharded by proposal "
+              f"ranges across GPUs by design but nothing here runs on more than one.", file=sys.stderr)
+        sys.exit(2)
+
+    model = headline_model()
+    ctx = Context(model)  # scratch context to make the data histogram
+    obs = observed_hist(ctx, model)
+    ctx.close()
+    ctx = Context(model, obs)
+    log(f"observed histogram: {int(obs.sum())} tracks")
+
+    rng = np.random.default_rng(1)
+    props, frames, parts = workload(model, args.batched_sims, rng, (500, 1000, 2000))
+    steps_per_gen = particle_steps(frames, parts)
+    b = ctx.batch()
+    b.load(props, frames, parts)
+    log(f"batched: {args.batched_sims} proposals, {steps_per_gen:.3e} particle-steps per generation")
+    walls, total_ms, _ = run_timed(b, args.steps, args.warmup, _abi.PCS_MODE_BATCHED, 100)
+    reruns = b.overflow_reruns()
+    # The same three kernels launched without the graph, with HIP events
+    # between them on the batch stream: the walk kernel's duration.
+    dwalls, _, walk_ms = run_timed(b, args.steps, 0, _abi.PCS_MODE_BATCHED_DIRECT, 100)
+    dist, tracks, _ = b.read()
+    b.close()
+    wall = sum(walls)
+    sims_s = args.batched_sims * args.steps / wall
+    psteps_s = steps_per_gen * args.steps / wall
+    walk_s = float(np.mean(walk_ms)) / 1e3 if min(walk_ms) > 0 else float("nan")
+    log(f"batched: {sims_s:.0f} sims/s, {psteps_s:.3e} particle-steps/s, walk kernel {walk_s*1e3:.1f} ms, "
+        f"graph {np.mean(total_ms):.1f} ms, wall {np.mean(walls)*1e3:.1f} ms, reruns {reruns}")
+
+    plain = None
+    if not args.no_plain:
+        rng2 = np.random.default_rng(2)
+        pprops, pframes, pparts = workload(model, args.plain_sims, rng2, (1000,))
+        pb = ctx.batch()
+        pb.load(pprops, pframes, pparts)
+        pw, ptot, _ = run_timed(pb, 1, 1, _abi.PCS_MODE_PLAIN, 500)
+        pb.close()
+        psims = args.plain_sims / pw[0]
+        plain = {"sims_per_s": psims, "particle_steps_per_s": particle_steps(pframes, pparts) / pw[0],
+                 "sims": args.plain_sims, "wall_s": pw[0]}
+        log(f"plain: {psims:.0f} sims/s over {args.plain_sims} simulations ({pw[0]:.2f} s)")
+
+    serve = None
+    if not args.no_serve:
+        serve = serving_latency(args.serve_requests, args.serve_rate)
+        log(f"serving: {serve}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(model, props, frames, parts, obs, args.cpu_sample)
+        log(f"cpu baseline: {cpu}")
+
+    ach = steps_per_gen * OPS_PER_STEP / walk_s if walk_s == walk_s else None
+    traffic = None
+    pmc = ROOT / "profiles" / "walk_kernel_pmc.json"
+    if pmc.exists():
+        try:
+            traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    line = {
+        "metric": "ABC sims/sec & particle-steps/s, batched vs plain; p50/p99 sim latency",
+        "value": sims_s,
+        "unit": "sims/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched ABC generation: {args.batched_sims} proposals, c log-uniform 1e8-1e10/mL "
+                        f"(120-12,000 particles), frames U{{500,1000,2000}}, box 200x200x30 um periodic, "
+                        f"2 species 175/500 nm, H 1000x500 uint32, one hipGraph launch per generation",
+            "particle_steps_per_generation": steps_per_gen,
+            "particle_steps_per_s": psteps_s,
+            "plain": plain,
+            "batched_vs_plain": (sims_s / plain["sims_per_s"]) if plain else None,
+            "serving": serve,
+            "p50_ms": serve["p50_ms"] if serve else None,
+            "p99_ms": serve["p99_ms"] if serve else None,
+            "overflow_reruns": reruns,
+            "ms_per_step_graph_device": float(np.mean(total_ms)),
+            "ms_per_step_direct": float(np.mean(dwalls)) * 1e3,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": ach / 1e12 if ach else None,
+            "peak": VALU_PEAK / 1e12,
+            "unit": "Tlane-op/s",
+            "frac": (ach / VALU_PEAK) if ach else None,
+            "traffic": traffic,
+            "kernel": "walk_kernel",
+            "kernel_ms": walk_s * 1e3,
+            "ops_per_particle_step": OPS_PER_STEP,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -171,7 +171,7 @@ typedef struct pcs_tuning {
 } pcs_tuning;
 int pcs_set_tuning(pcs_ctx* ctx, const pcs_tuning* t);
 int pcs_get_tuning(pcs_ctx* ctx, pcs_tuning* t);
-/* Scale of the per-simulation record slice estimate (default 1.0; the bound
+/* Scale of the per-simulation record slice estimate (default 0.5; the bound
  * is tracks <= detected particle-frames / kmin). Slices that overflow are
  * rerun exactly, so this only trades memory for reruns. Takes effect at the
  * next pcs_batch_load. */

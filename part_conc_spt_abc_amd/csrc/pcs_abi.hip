@@ -153,7 +153,7 @@ struct pcs_ctx {
     uint32_t* d_sat = nullptr;
     hipStream_t stream = nullptr;
     pcs_tuning tune{1, 4, 8, 256};
-    double cap_factor = 1.0;
+    double cap_factor = 0.5;
 };
 
 struct pcs_batch {
@@ -182,6 +182,7 @@ struct pcs_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     pcs_tuning graph_tune{};
+    bool graph_events = false;
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int last_mode = -1;
@@ -333,11 +334,15 @@ int enqueue_plain(pcs_batch* b, const std::vector<uint32_t>* subset)
     return PCS_OK;
 }
 
-int capture_graph(pcs_batch* b)
+// Capture reset + walk + distance into one graph. Events recorded during
+// capture do not yield elapsed times on this runtime, so the graph is
+// captured without them; PCS_MODE_BATCHED_DIRECT launches the same kernels
+// with events between them for kernel timing.
+int capture_graph_once(pcs_batch* b, bool events)
 {
     destroy_graph(b);
     PCS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int r = enqueue_batched(b, false);
+    const int r = enqueue_batched(b, events);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (r != PCS_OK) {
@@ -348,8 +353,11 @@ int capture_graph(pcs_batch* b)
     b->graph = g;
     PCS_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     b->graph_tune = b->ctx->tune;
+    b->graph_events = events;
     return PCS_OK;
 }
+
+int capture_graph(pcs_batch* b) { return capture_graph_once(b, false); }
 
 bool same_tune(const pcs_tuning& a, const pcs_tuning& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
 
@@ -679,9 +687,9 @@ int pcs_batch_run(pcs_batch* b, uint64_t seed, int mode)
     b->reruns = 0;
     if (mode == PCS_MODE_BATCHED) {
         if (!b->gexec || !same_tune(b->graph_tune, c->tune)) PCS_TRY(capture_graph(b));
-        PCS_HIP(hipEventRecord(b->ev[0], b->stream));
+        if (!b->graph_events) PCS_HIP(hipEventRecord(b->ev[0], b->stream));
         PCS_HIP(hipGraphLaunch(b->gexec, b->stream));
-        PCS_HIP(hipEventRecord(b->ev[3], b->stream));
+        if (!b->graph_events) PCS_HIP(hipEventRecord(b->ev[3], b->stream));
     } else if (mode == PCS_MODE_BATCHED_DIRECT) {
         PCS_TRY(enqueue_batched(b, true));
     } else {
@@ -710,7 +718,7 @@ int pcs_batch_timing(pcs_batch* b, float out[3])
     PCS_HIP(hipEventSynchronize(b->ev[3]));
     out[0] = out[1] = out[2] = -1.0f;
     PCS_HIP(hipEventElapsedTime(&out[0], b->ev[0], b->ev[3]));
-    if (b->last_mode == PCS_MODE_BATCHED_DIRECT) {
+    if (b->last_mode == PCS_MODE_BATCHED_DIRECT || (b->last_mode == PCS_MODE_BATCHED && b->graph_events)) {
         PCS_HIP(hipEventElapsedTime(&out[1], b->ev[1], b->ev[2]));
         PCS_HIP(hipEventElapsedTime(&out[2], b->ev[2], b->ev[3]));
     }

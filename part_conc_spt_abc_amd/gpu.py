@@ -193,7 +193,9 @@ class Server:
         return True
 
     def poll(self, max_n: int = 1024, wait_us: int = 0):
-        buf = (_abi.pcs_completion * max_n)()
+        buf = getattr(self, "_buf", None)
+        if buf is None or len(buf) < max_n:
+            buf = self._buf = (_abi.pcs_completion * max_n)()
         n = C.c_uint32(0)
         _abi.check(self.lib.pcs_server_poll(self.h, buf, max_n, C.byref(n), wait_us), "pcs_server_poll")
         return [buf[i] for i in range(n.value)]
