@@ -40,10 +40,12 @@ from part_conc_spt_abc_amd.model import Model, proposal_record, stokes_einstein 
 
 # Algorithmic VALU lane-operations per particle-step of the walk (Philox
 # 0.75 blocks x 104, Box-Muller 12 counting log/sqrt/sin/cos as one each,
-# 6 uniform conversions, 27 walk/wrap/detect/track) and gfx950's non-packed
-# VALU issue peak: 256 CU x 64 lanes x 2.4 GHz.
+# 6 uniform conversions, 27 walk/wrap/detect/track) and MI355X's FP32 vector
+# peak in lane-operations: 157.3 TFLOP/s / 2 flops per FMA = 78.6e12/s
+# (256 CU x 128 lanes/cycle x 2.4 GHz; measured issue rates of the walk
+# exceed the 64 lanes/cycle/CU of a 4-cycle wave64 issue, see profiles/).
 OPS_PER_STEP = 123
-VALU_PEAK = 256 * 64 * 2.4e9
+VALU_PEAK = 157.3e12 / 2
 
 D175 = stokes_einstein(175.0)
 D500 = stokes_einstein(500.0)
@@ -192,6 +194,33 @@ def cpu_baseline(model, props, frames, parts, obs, n_sample):
                       f"{cores} threads, {dt:.1f} s"}
 
 
+def profile_run(args):
+    """Workload for rocprofv3: the batched kernels launched directly (one
+    warmup, `steps` profiled replays) on the first `batched_sims` proposals of
+    the batched workload, then a device-to-device copy of 256 MiB."""
+    model = headline_model()
+    ctx = Context(model)
+    obs = observed_hist(ctx, model)
+    ctx.close()
+    ctx = Context(model, obs)
+    rng = np.random.default_rng(1)
+    n = min(args.batched_sims, 4000)
+    props, frames, parts = workload(model, n, rng, (500, 1000, 2000))
+    b = ctx.batch()
+    b.load(props, frames, parts)
+    walls, _, walk_ms = run_timed(b, args.steps, 1, _abi.PCS_MODE_BATCHED_DIRECT, 100)
+    b.close()
+    steps = particle_steps(frames, parts)
+    log(f"profile: {n} proposals, {steps:.3e} particle-steps, walk kernel {np.mean(walk_ms):.2f} ms")
+    import ctypes as C
+
+    ms = C.c_float(0.0)
+    _abi.check(_abi.load().pcs_selftest_copy(0, 256 << 20, C.byref(ms)), "pcs_selftest_copy")
+    log(f"profile: copied 268435456 bytes device to device in {ms.value:.3f} ms")
+    print(json.dumps({"proposals": n, "particle_steps": steps, "walk_ms": float(np.mean(walk_ms)),
+                      "copy_bytes": 256 << 20}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,7 +234,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: batched-direct replays of a reduced workload, then a 256 MiB "
+                         "device copy whose known traffic calibrates the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.profile:
+        return profile_run(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is sharded by proposal "
               f"ranges across GPUs by design but nothing here runs on more than one.", file=sys.stderr)
@@ -262,11 +296,13 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     ach = steps_per_gen * OPS_PER_STEP / walk_s if walk_s == walk_s else None
+    # HBM bytes per launch: FETCH_SIZE/WRITE_SIZE per particle-step from the
+    # committed rocprofv3 runs (copy-calibrated), times this launch's steps.
     traffic = None
     pmc = ROOT / "profiles" / "walk_kernel_pmc.json"
     if pmc.exists():
         try:
-            traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
+            traffic = json.loads(pmc.read_text())["hbm_bytes_per_particle_step"] * steps_per_gen
         except Exception:
             traffic = None
     line = {

@@ -182,6 +182,12 @@ int pcs_set_record_capacity_factor(pcs_ctx* ctx, double factor);
  * Test hook for bit-exactness of the device math; no reference counterpart. */
 int pcs_selftest_math(int device, const uint32_t* w, float* radius, float* s, float* c, uint64_t n);
 
+/* Copies `bytes` (a multiple of 16) between two fresh device buffers with a
+ * plain streaming kernel and reports its time in *ms (nullable). Calibration
+ * hook for hardware byte counters (known traffic: bytes read + bytes
+ * written); no reference counterpart. */
+int pcs_selftest_copy(int device, uint64_t bytes, float* ms);
+
 /* ---------------------------------------------------------------- serving */
 
 typedef struct pcs_server_config {

@@ -133,6 +133,7 @@ PROTOTYPES = {
     "pcs_get_tuning": (C.c_int, [P, C.POINTER(pcs_tuning)]),
     "pcs_set_record_capacity_factor": (C.c_int, [P, C.c_double]),
     "pcs_selftest_math": (C.c_int, [C.c_int, P, P, P, P, C.c_uint64]),
+    "pcs_selftest_copy": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_float)]),
     "pcs_server_create": (C.c_int, [C.c_int, C.POINTER(pcs_server_config), C.POINTER(P)]),
     "pcs_server_add_system": (C.c_int, [P, C.POINTER(pcs_system), P, U32P]),
     "pcs_server_submit": (C.c_int, [P, C.POINTER(pcs_request)]),

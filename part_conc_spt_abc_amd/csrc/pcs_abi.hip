@@ -820,6 +820,38 @@ int pcs_selftest_math(int device, const uint32_t* w, float* radius, float* s, fl
     return PCS_OK;
 }
 
+int pcs_selftest_copy(int device, uint64_t bytes, float* ms)
+{
+    if (bytes == 0 || bytes % 16u) return set_error(PCS_EINVAL, "bytes must be a positive multiple of 16");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PCS_ENODEV, "no GPU visible");
+    if (device < 0 || device >= ndev) return set_error(PCS_EINVAL, "device %d outside [0, %d)", device, ndev);
+    PCS_HIP(hipSetDevice(device));
+    uint8_t *src = nullptr, *dst = nullptr;
+    PCS_TRY(dev_alloc(&src, bytes));
+    if (dev_alloc(&dst, bytes) != PCS_OK) {
+        hipFree(src);
+        return PCS_ENOMEM;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemset(src, 1, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = launch_selftest_copy(src, dst, bytes, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(src);
+    hipFree(dst);
+    if (e != hipSuccess) return hip_error(e, "pcs_selftest_copy");
+    return PCS_OK;
+}
+
 int pcs_simulate(pcs_ctx* c, const pcs_proposal* props, uint32_t n, const uint32_t* vframes,
                  const uint32_t* vparticles, uint32_t n_videos, uint64_t seed, int mode, double* dist_out,
                  uint32_t* hist_out, uint32_t* tracks_out)

@@ -80,6 +80,7 @@ hipError_t launch_walk(const WalkArgs& a, int ppl, int waves_per_group, bool per
                        uint32_t grid_groups, bool gauss, bool zreflect, hipStream_t s);
 hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s);
 hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, float* c, uint64_t n, hipStream_t st);
+hipError_t launch_selftest_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st);
 hipError_t launch_dense_hist(const DevProp* props, uint32_t prop_lo, uint32_t n_props,
                              const uint32_t* rec_count, uint32_t max_count,
                              uint32_t n_k, uint32_t n_de, uint32_t* hist, hipStream_t s);

@@ -455,7 +455,21 @@ __global__ void selftest_math_kernel(const uint32_t* w, float* radius, float* s,
     c[i] = cs;
 }
 
+__global__ void selftest_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------- launchers
+
+hipError_t launch_selftest_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st)
+{
+    const uint64_t n = bytes / 16u;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(selftest_copy_kernel, dim3(4096), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, float* c, uint64_t n, hipStream_t st)
 {
