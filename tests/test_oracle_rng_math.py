@@ -1,0 +1,100 @@
+"""Pins the oracle's Philox4x32-10 and FP32 math.
+
+- Philox: the Random123 known-answer vectors (Salmon et al., SC'11,
+  kat_vectors for philox4x32_10) and 64 vectors from rocRAND's independent
+  implementation (tests/golden/philox_rocrand.json).
+- log / sqrt / sincos: every one of the 2^24 inputs the walk can feed them,
+  against libm FP64 through numpy, with ulp bounds; plus identities a wrong
+  quadrant, sign or swapped pair would break.
+"""
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GOLDEN = pathlib.Path(__file__).parent / "golden"
+
+# Random123 kat_vectors, philox4x32_10 (counter, key, expected output).
+R123_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", R123_KAT)
+def test_philox_random123_kat(ctr, key, out):
+    assert tuple(int(x) for x in O.philox(ctr, key)) == out
+
+
+def test_philox_matches_rocrand():
+    vecs = json.loads((GOLDEN / "philox_rocrand.json").read_text())
+    assert len(vecs) == 64
+    for v in vecs:
+        assert [int(x) for x in O.philox(v["ctr"], v["key"])] == v["out"], v
+
+
+@pytest.fixture(scope="module")
+def all_words():
+    return (np.arange(1 << 24, dtype=np.uint64) << 8).astype(np.uint32)
+
+
+def ulps(got, want):
+    want32 = want.astype(np.float32)
+    sp = np.spacing(np.abs(want32)).astype(np.float64)
+    return np.abs(got.astype(np.float64) - want) / sp
+
+
+def test_radius_all_inputs(all_words):
+    r = O.bm_radius(all_words)
+    j = np.arange(1 << 24, dtype=np.float64)
+    want = np.sqrt(-2.0 * np.log((j + 1.0) * 2.0 ** -24))
+    assert r[-1] == 0.0 and not np.signbit(r[-1])  # u = 1 -> radius +0
+    e = ulps(r[:-1], want[:-1])
+    assert e.max() <= 1.5, e.max()
+    # radius is non-increasing in u
+    assert np.all(np.diff(r.astype(np.float64)) <= 0)
+
+
+def test_sincos_all_inputs(all_words):
+    s, c = O.sincos_turn(all_words)
+    j = np.arange(1 << 24, dtype=np.float64)
+    th = 2.0 * np.pi * j * 2.0 ** -24
+    ws, wc = np.sin(th), np.cos(th)
+    assert np.abs(s - ws).max() < 1.2e-7
+    assert np.abs(c - wc).max() < 1.2e-7
+    big = np.abs(ws) > 1e-3
+    assert ulps(s[big], ws[big]).max() <= 2.0
+    big = np.abs(wc) > 1e-3
+    assert ulps(c[big], wc[big]).max() <= 2.0
+    # exact values at the quarter turns and the quadrant symmetries
+    q = 1 << 22
+    assert (s[0], c[0]) == (0.0, 1.0)
+    assert (s[q], c[q]) == (1.0, 0.0)
+    assert (s[2 * q], c[2 * q]) == (0.0, -1.0)
+    assert (s[3 * q], c[3 * q]) == (-1.0, 0.0)
+    k = np.arange(1, q)
+    assert np.array_equal(s[k + q], c[k])       # sin(t + 1/4) = cos t
+    assert np.array_equal(c[k + q], -s[k])      # cos(t + 1/4) = -sin t
+    assert np.array_equal(s[k + 2 * q], -s[k])  # half-turn symmetry
+    norm = s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2
+    assert np.abs(norm - 1.0).max() < 4e-7
+
+
+def test_box_muller_normals_are_standard():
+    """Pairs drawn from Philox words through radius/sincos are N(0,1) and
+    uncorrelated (moments over 2^20 pairs)."""
+    rng = np.random.default_rng(0)
+    w = rng.integers(0, 2 ** 32, size=(1 << 20, 2), dtype=np.uint64).astype(np.uint32)
+    r = O.bm_radius(w[:, 0]).astype(np.float64)
+    s, c = O.sincos_turn(w[:, 1])
+    n1, n2 = r * s, r * c
+    for n in (n1, n2):
+        assert abs(n.mean()) < 5e-3
+        assert abs(n.var() - 1.0) < 5e-3
+        assert abs((n ** 4).mean() - 3.0) < 3e-2
+    assert abs(np.mean(n1 * n2)) < 5e-3
