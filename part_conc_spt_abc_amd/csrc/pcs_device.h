@@ -25,19 +25,50 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
-// The counter words that are wave-uniform (block, video, simulation) and the
-// key stay in SGPRs; only the particle word varies per lane, so the first
-// rounds run partly on the scalar unit.
+// Three-way XOR in one gfx950 VALU instruction (v_bitop3_b32, truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+__device__ __forceinline__ uint32_t mulhi(uint32_t m, uint32_t x) { return (uint32_t)(((uint64_t)m * x) >> 32); }
+
+// Counter words block (c0), video (c2) and simulation (c3) and the key are
+// wave-uniform; only the particle word c1 varies per lane. Rounds 1-3 are
+// written out so that the uniform halves stay scalar (plain ^, s_mul) and
+// only lane-varying words use the fused v_bitop3 XOR; from round 4 on every
+// word varies per lane. Same function as the textbook round loop.
 __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                        uint32_t k0, uint32_t k1)
 {
+    // round 1: c1 varies
+    uint32_t x0 = xor3(mulhi(kPhiloxM1, c2), c1, k0);  // varying
+    uint32_t x1 = kPhiloxM1 * c2;                       // uniform
+    uint32_t x2 = mulhi(kPhiloxM0, c0) ^ c3 ^ k1;       // uniform
+    uint32_t x3 = kPhiloxM0 * c0;                       // uniform
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+    // round 2: x0 varies
+    uint32_t y0 = mulhi(kPhiloxM1, x2) ^ x1 ^ k0;       // uniform
+    uint32_t y1 = kPhiloxM1 * x2;                       // uniform
+    uint32_t y2 = xor3(mulhi(kPhiloxM0, x0), x3, k1);   // varying
+    uint32_t y3 = kPhiloxM0 * x0;                       // varying
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+    // round 3: y2, y3 vary
+    c0 = xor3(mulhi(kPhiloxM1, y2), y1, k0);
+    c1 = kPhiloxM1 * y2;
+    c2 = xor3(mulhi(kPhiloxM0, y0), y3, k1);
+    c3 = kPhiloxM0 * y0;  // uniform
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
 #pragma unroll
-    for (int r = 0; r < 10; ++r) {
+    for (int r = 3; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
         const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
         const uint32_t n1 = (uint32_t)p1;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
         const uint32_t n3 = (uint32_t)p0;
         c0 = n0;
         c1 = n1;
@@ -91,8 +122,26 @@ __device__ __forceinline__ float neg2log_u24(uint32_t w)
     return __builtin_fmaf(-2.0f, L, 0.0f);
 }
 
-// Box-Muller radius sqrt(-2 ln u), correctly rounded sqrt.
-__device__ __forceinline__ float bm_radius(uint32_t w) { return __builtin_sqrtf(neg2log_u24(w)); }
+// Correctly rounded sqrt for y = 0 or 2^-96 <= y < inf: the hardware
+// square root (v_sqrt_f32, within 1 ulp) followed by the exact test of its
+// two neighbours, y - s'*s against zero with one fused multiply-add each,
+// which selects the correctly rounded result. The general lowering also
+// rescales tiny inputs and passes infinities/NaNs through; -2 ln u is either
+// 0 or at least 1.19e-7 and finite, so those steps are not needed here.
+__device__ __forceinline__ float sqrt_rn_domain(float y)
+{
+    const float s = __builtin_amdgcn_sqrtf(y);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rd = __builtin_fmaf(-sd, s, y);
+    const float ru = __builtin_fmaf(-su, s, y);
+    float r = (rd <= 0.0f) ? sd : s;
+    r = (ru > 0.0f) ? su : r;
+    return r;
+}
+
+// Box-Muller radius sqrt(-2 ln u), correctly rounded.
+__device__ __forceinline__ float bm_radius(uint32_t w) { return sqrt_rn_domain(neg2log_u24(w)); }
 
 // sin and cos of 2 pi t, t = (w >> 8) 2^-24: exact reduction to the nearest
 // quarter turn in integers, polynomial on |t| <= 1/8, quadrant rotation by
