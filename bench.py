@@ -234,6 +234,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: batched-direct replays of a reduced workload, then a 256 MiB "
                          "device copy whose known traffic calibrates the FETCH_SIZE/WRITE_SIZE counters")
@@ -255,6 +256,12 @@ def main():
     rng = np.random.default_rng(1)
     props, frames, parts = workload(model, args.batched_sims, rng, (500, 1000, 2000))
     steps_per_gen = particle_steps(frames, parts)
+    tuning = None
+    if not args.no_autotune:
+        # launch shape chosen by the autotuner on a calibration slice of the workload (untimed)
+        k = min(len(props), 4000)
+        tuning, tms = ctx.autotune(props[:k], frames[:k], parts[:k])
+        log(f"autotune: {tuning} ({tms:.2f} ms on {k} proposals)")
     b = ctx.batch()
     b.load(props, frames, parts)
     log(f"batched: {args.batched_sims} proposals, {steps_per_gen:.3e} particle-steps per generation")
@@ -330,6 +337,7 @@ def main():
             "p50_ms": serve["p50_ms"] if serve else None,
             "p99_ms": serve["p99_ms"] if serve else None,
             "overflow_reruns": reruns,
+            "tuning": tuning,
             "ms_per_step_graph_device": float(np.mean(total_ms)),
             "ms_per_step_direct": float(np.mean(dwalls)) * 1e3,
             "parallelism": "single GPU",

@@ -171,6 +171,15 @@ typedef struct pcs_tuning {
 } pcs_tuning;
 int pcs_set_tuning(pcs_ctx* ctx, const pcs_tuning* t);
 int pcs_get_tuning(pcs_ctx* ctx, pcs_tuning* t);
+/* Autotuner: times the batched walk kernel (HIP events)
+ * on the given calibration proposals for every candidate
+ * launch shape - particles per lane {1,2} x waves per group {2,4} x groups
+ * per CU {2,4,6,8,12,16} (those the hardware can co-schedule) - and keeps
+ * the fastest in the context. best/ms (nullable) receive the winner and its
+ * walk time (best of 2 runs after a warmup). Results never depend on the shape. */
+int pcs_autotune(pcs_ctx* ctx, const pcs_proposal* props, uint32_t n, const uint32_t* video_frames,
+                 const uint32_t* video_particles, uint32_t n_videos, pcs_tuning* best, float* ms);
+
 /* Scale of the per-simulation record slice estimate (default 0.5; the bound
  * is tracks <= detected particle-frames / kmin). Slices that overflow are
  * rerun exactly, so this only trades memory for reruns. Takes effect at the

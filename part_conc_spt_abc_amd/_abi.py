@@ -132,6 +132,7 @@ PROTOTYPES = {
     "pcs_set_tuning": (C.c_int, [P, C.POINTER(pcs_tuning)]),
     "pcs_get_tuning": (C.c_int, [P, C.POINTER(pcs_tuning)]),
     "pcs_set_record_capacity_factor": (C.c_int, [P, C.c_double]),
+    "pcs_autotune": (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, C.POINTER(pcs_tuning), C.POINTER(C.c_float)]),
     "pcs_selftest_math": (C.c_int, [C.c_int, P, P, P, P, C.c_uint64]),
     "pcs_selftest_copy": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_float)]),
     "pcs_server_create": (C.c_int, [C.c_int, C.POINTER(pcs_server_config), C.POINTER(P)]),

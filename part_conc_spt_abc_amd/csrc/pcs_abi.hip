@@ -852,6 +852,47 @@ int pcs_selftest_copy(int device, uint64_t bytes, float* ms)
     return PCS_OK;
 }
 
+int pcs_autotune(pcs_ctx* c, const pcs_proposal* props, uint32_t n, const uint32_t* vframes,
+                 const uint32_t* vparticles, uint32_t n_videos, pcs_tuning* best_out, float* ms_out)
+{
+    if (!c) return set_error(PCS_EINVAL, "context is NULL");
+    pcs_batch* b = nullptr;
+    PCS_TRY(pcs_batch_create(c, &b));
+    const pcs_tuning saved = c->tune;
+    pcs_tuning best = saved;
+    float best_ms = -1.0f;
+    int r = pcs_batch_load(b, props, n, vframes, vparticles, n_videos);
+    for (uint32_t ppl : {1u, 2u}) {
+        for (uint32_t wpg : {2u, 4u}) {
+            for (uint32_t gpc : {2u, 4u, 6u, 8u, 12u, 16u}) {
+                if (r != PCS_OK) break;
+                if (wpg * gpc > 32u) continue;  // more waves than a CU holds (8 per SIMD)
+                pcs_tuning t{ppl, wpg, gpc, saved.stage_records};
+                c->tune = t;
+                float times[3];
+                for (int k = 0; k < 3 && r == PCS_OK; ++k) {
+                    r = pcs_batch_run(b, 0x5eedull + (uint64_t)k, PCS_MODE_BATCHED_DIRECT);
+                    float tm[3];
+                    if (r == PCS_OK) r = pcs_batch_timing(b, tm);
+                    times[k] = tm[1];
+                }
+                if (r != PCS_OK) break;
+                const float t_ms = std::min(times[1], times[2]);  // run 0 is the warmup
+                if (best_ms < 0.0f || t_ms < best_ms) {
+                    best_ms = t_ms;
+                    best = t;
+                }
+            }
+        }
+    }
+    pcs_batch_destroy(b);
+    c->tune = (r == PCS_OK) ? best : saved;
+    if (r != PCS_OK) return r;
+    if (best_out) *best_out = best;
+    if (ms_out) *ms_out = best_ms;
+    return PCS_OK;
+}
+
 int pcs_simulate(pcs_ctx* c, const pcs_proposal* props, uint32_t n, const uint32_t* vframes,
                  const uint32_t* vparticles, uint32_t n_videos, uint64_t seed, int mode, double* dist_out,
                  uint32_t* hist_out, uint32_t* tracks_out)

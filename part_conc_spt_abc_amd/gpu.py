@@ -75,6 +75,18 @@ class Context:
         _abi.check(self.lib.pcs_get_tuning(self.h, C.byref(t)), "pcs_get_tuning")
         return {f: int(getattr(t, f)) for f, _ in t._fields_}
 
+    def autotune(self, props, frames, particles):
+        """Pick the fastest launch shape on a calibration batch; returns
+        (tuning dict, walk ms)."""
+        arr = proposals_array(props)
+        fr = np.ascontiguousarray(frames, dtype=np.uint32)
+        pa = np.ascontiguousarray(particles, dtype=np.uint32)
+        t = _abi.pcs_tuning()
+        ms = C.c_float(0.0)
+        _abi.check(self.lib.pcs_autotune(self.h, C.cast(arr, C.c_void_p), len(props), _ptr(fr), _ptr(pa), len(fr),
+                                         C.byref(t), C.byref(ms)), "pcs_autotune")
+        return {f: int(getattr(t, f)) for f, _ in t._fields_}, float(ms.value)
+
     def set_record_capacity_factor(self, f: float):
         _abi.check(self.lib.pcs_set_record_capacity_factor(self.h, f), "pcs_set_record_capacity_factor")
 

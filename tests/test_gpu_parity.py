@@ -200,3 +200,17 @@ def test_invalid_arguments_fail_loudly(gpu_lib):
         with pytest.raises(_abi.PcsError):
             b = ctx.batch()
             b.load([p], f, n)
+
+
+def test_autotune_picks_a_valid_shape_and_keeps_results(gpu_lib):
+    model = cases.headline_model(n_k=600, n_de=300)
+    obs = observed(model, frames=[600])
+    rng = np.random.default_rng(12)
+    props, f, n = cases.draw_proposals(model, 40, rng, frames_choices=(300, 600), c_range=(1e8, 2e9))
+    with Context(model, obs) as ctx:
+        best, ms = ctx.autotune(props, f, n)
+        assert best["particles_per_lane"] in (1, 2) and best["waves_per_group"] in (2, 4)
+        assert best["waves_per_group"] * best["groups_per_cu"] <= 32 and ms > 0
+        assert ctx.get_tuning() == best
+        dist, tracks, H, exact, _ = run_gpu(ctx, props, f, n, 33, _abi.PCS_MODE_BATCHED)
+    check_against_oracle(model, obs, props, f, n, 33, dist, tracks, H, exact)
