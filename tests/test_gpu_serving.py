@@ -1,0 +1,121 @@
+"""Serving loop (continuous batching): every request's result is the same
+when it runs alone, when it runs mixed with requests of other chains/models
+arriving while others are on the GPU, and as part of a resident batch."""
+import time
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib as O
+from part_conc_spt_abc_amd import _abi
+from part_conc_spt_abc_amd.gpu import Context, Server
+from part_conc_spt_abc_amd.model import Model, proposal_record, stokes_einstein
+
+pytestmark = pytest.mark.gpu
+
+
+def chains(rng, n):
+    out = []
+    for c in range(n):
+        L = float(rng.uniform(100, 400))
+        Lz = float(rng.uniform(10, 50))
+        C = int(rng.integers(2, 5))
+        kw = {}
+        if c % 3 == 1:
+            kw = dict(z_boundary=_abi.PCS_Z_REFLECT)
+        elif c % 3 == 2:
+            kw = dict(detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0)
+        m = Model(Lx=L, Ly=L, Lz=Lz, ax=0.8 * L, ay=0.6 * L, deltat=0.05, kmin=3, n_k=500, n_de=200, ub_de=20.0,
+                  n_species=C, **kw)
+        D = sorted(stokes_einstein(float(d)) for d in rng.uniform(100, 600, C))
+        obs = np.zeros((m.n_k, m.n_de), np.uint32)
+        obs[2:40, :50] = rng.integers(0, 3, (38, 50))
+        out.append((m, D, obs))
+    return out
+
+
+def requests(rng, ch, n):
+    reqs = []
+    for i in range(n):
+        m, D, obs = ch[i % len(ch)]
+        ct = 10.0 ** rng.uniform(8.0, 9.0)
+        w = rng.dirichlet(np.ones(m.n_species))
+        p = proposal_record(m, D, list(w * ct), [2.0], i, 0, 2)
+        parts = rng.poisson(ct * m.volume_factor, 2).clip(0, 20000).astype(np.uint32)
+        frames = np.array([500, 250], np.uint32)
+        reqs.append((i % len(ch), p, frames, parts, 1000 + i))
+    return reqs
+
+
+def collect(srv, n, timeout=120.0):
+    got = {}
+    t0 = time.time()
+    while len(got) < n:
+        for c in srv.poll(1024, wait_us=2000):
+            assert c.status == 0
+            got[c.tag] = (c.dist, c.tracks)
+        assert time.time() - t0 < timeout, f"only {len(got)} of {n} completions"
+    return got
+
+
+def test_alone_vs_mixed_vs_batch():
+    rng = np.random.default_rng(21)
+    ch = chains(rng, 6)
+    reqs = requests(rng, ch, 48)
+    srv = Server(device=0, arena_bytes=1 << 30, max_inflight=256)
+    sids = [srv.add_system(m, obs) for m, _, obs in ch]
+    # alone: one request at a time
+    alone = {}
+    for tag, (ci, p, f, n, seed) in enumerate(reqs):
+        assert srv.submit(tag, sids[ci], p, f, n, seed)
+        alone.update(collect(srv, 1))
+    # mixed: all at once, arriving while earlier ones run
+    for tag, (ci, p, f, n, seed) in enumerate(reqs):
+        assert srv.submit(tag, sids[ci], p, f, n, seed)
+        if tag % 7 == 0:
+            time.sleep(0.0005)
+    mixed = collect(srv, len(reqs))
+    assert srv.pending() == 0
+    srv.close()
+    assert alone == mixed
+    # and equal to the same simulations as a resident batch of their chain
+    for ci, (m, D, obs) in enumerate(ch):
+        mine = [(tag, r) for tag, r in enumerate(reqs) if r[0] == ci]
+        with Context(m, obs) as ctx:
+            for tag, (_, p, f, n, seed) in mine:
+                b = ctx.batch()
+                b.load([p], f, n)
+                b.run(seed, _abi.PCS_MODE_BATCHED)
+                dist, tracks, _ = b.read()
+                b.close()
+                assert (float(dist[0]), int(tracks[0])) == alone[tag], (ci, tag)
+
+
+def test_admission_by_footprint_and_backpressure():
+    rng = np.random.default_rng(5)
+    ch = chains(rng, 2)
+    reqs = requests(rng, ch, 64)
+    # an arena that holds only a few simulations at a time
+    srv = Server(device=0, arena_bytes=8 << 20, max_inflight=8)
+    sids = [srv.add_system(m, obs) for m, _, obs in ch]
+    submitted = 0
+    done = {}
+    t0 = time.time()
+    while len(done) < len(reqs):
+        while submitted < len(reqs):
+            ci, p, f, n, seed = reqs[submitted]
+            if not srv.submit(submitted, sids[ci], p, f, n, seed):
+                break
+            submitted += 1
+        for c in srv.poll(64, wait_us=1000):
+            done[c.tag] = c
+        assert time.time() - t0 < 120
+    lat = [c.done_us - c.submit_us for c in done.values()]
+    assert min(lat) > 0
+    # a request larger than the whole arena is refused at submit
+    m, D, obs = ch[0]
+    big = proposal_record(m, D, [1e10] * m.n_species, [50.0], 999, 0, 1)
+    with pytest.raises(_abi.PcsError):
+        srv.submit(999, sids[0], big, np.array([2000], np.uint32), np.array([4_000_000], np.uint32), 1)
+    srv.close()
