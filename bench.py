@@ -229,7 +229,7 @@ def main():
     ap.add_argument("--batched-sims", type=int, default=100_000)
     ap.add_argument("--plain-sims", type=int, default=10_000)
     ap.add_argument("--serve-requests", type=int, default=4000)
-    ap.add_argument("--serve-rate", type=float, default=5000.0)
+    ap.add_argument("--serve-rates", default="2000,5000,10000,20000,50000")
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -294,8 +294,10 @@ def main():
 
     serve = None
     if not args.no_serve:
-        serve = serving_latency(args.serve_requests, args.serve_rate)
-        log(f"serving: {serve}")
+        serve = []
+        for rate in [float(x) for x in args.serve_rates.split(",") if x]:
+            serve.append(serving_latency(args.serve_requests, rate))
+            log(f"serving: {serve[-1]}")
 
     cpu = None
     if not args.no_cpu:
@@ -334,8 +336,8 @@ def main():
             "plain": plain,
             "batched_vs_plain": (sims_s / plain["sims_per_s"]) if plain else None,
             "serving": serve,
-            "p50_ms": serve["p50_ms"] if serve else None,
-            "p99_ms": serve["p99_ms"] if serve else None,
+            "p50_ms": serve[1]["p50_ms"] if serve and len(serve) > 1 else (serve[0]["p50_ms"] if serve else None),
+            "p99_ms": serve[1]["p99_ms"] if serve and len(serve) > 1 else (serve[0]["p99_ms"] if serve else None),
             "overflow_reruns": reruns,
             "tuning": tuning,
             "ms_per_step_graph_device": float(np.mean(total_ms)),

@@ -1,0 +1,56 @@
+"""ABC-PMC posterior of the C++ estimator on the GPU against the CPU
+restatement of the reference's Julia estimator (tests/abc_cpu.py: estimate.jl
++ simulate.jl + distance.jl semantics on the FP64 oracle), on the same
+synthetic data and the same fixed tolerance schedule (gamma 9.95 - 0.05 g,
+6 generations, pmc-standard weights, 128 members). Julia itself is not
+available, so this CPU restatement stands in for the reference run. Means
+must agree within Monte-Carlo error (4 standard errors from the weights'
+effective sample sizes) and the 10-90 % credible intervals must overlap.
+"""
+import math
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+from abc_cpu import CpuAbc
+from abc_data import make_data, read_output
+
+pytestmark = pytest.mark.gpu
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+PCS_ABC = ROOT / "part_conc_spt_abc_amd" / "lib" / "pcs_abc"
+
+
+def wstats(x, w):
+    mu = float((x * w).sum())
+    var = float((w * (x - mu) ** 2).sum())
+    ess = 1.0 / float((w ** 2).sum())
+    o = np.argsort(x)
+    cw = np.cumsum(w[o])
+    q = lambda p: float(x[o][min(np.searchsorted(cw, p), len(x) - 1)])  # noqa: E731
+    return mu, math.sqrt(var / ess), (q(0.1), q(0.9))
+
+
+def test_gpu_posterior_matches_cpu_restatement(tmp_path):
+    inp, ntracks, (model, frames, K, DE) = make_data(tmp_path, weighting="pmc-standard", trials=100000,
+                                                     samples=128, adaptive=False, gamma0=9.95, delta_gamma=0.05)
+    out = tmp_path / "gpu.xml"
+    r = subprocess.run([str(PCS_ABC), "-i", str(inp), "-o", str(out), "-F", "11", "-S", "6", "-q"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    _, vec, scal = read_output(out)
+    g_m, g_c, g_w = vec("m").reshape(-1, 2).T, vec("c").reshape(-1, 2).T, vec("w")
+    assert float(scal("gamma")) == pytest.approx(9.65, abs=1e-9)
+
+    cpu = CpuAbc(model, frames, K, DE, 0.25, 12.0, 2.5e7, 4e8, 0.5, 8.0, 128, seed=5, threads=16).run(9.95, 0.05, 6)
+    assert cpu["gamma"] == pytest.approx(9.65, abs=1e-9)
+
+    for name, a, b in [("log10 c_total", np.log10(g_c.sum(0)), np.log10(cpu["c"].sum(0))),
+                       ("m_1", g_m[0], cpu["m"][0]), ("m_2", g_m[1], cpu["m"][1])]:
+        ma, sa, ia = wstats(a, g_w)
+        mb, sb, ib = wstats(b, cpu["w"])
+        print(f"{name}: gpu {ma:.4f} +- {sa:.4f} {ia}  cpu {mb:.4f} +- {sb:.4f} {ib}", flush=True)
+        assert abs(ma - mb) < 4.0 * math.hypot(sa, sb), name
+        assert ia[0] < ib[1] and ib[0] < ia[1], name
