@@ -154,6 +154,45 @@ def serving_latency(n_requests, rate, seed=7):
             "p50_ms": float(np.percentile(lat, 50) / 1e3), "p99_ms": float(np.percentile(lat, 99) / 1e3)}
 
 
+def large_model(n_sims, seed=9):
+    """Large polydisperse model run batched under the serving loop: 400x400x60
+    um box, 5 classes (100/200/300/450/600 nm) with Dirichlet(1,...,1)
+    abundances, Gaussian z detection (sigma 1 um), c = 1e10/mL (~96,000
+    particles), 2000 frames, H 2000 x 1000; all requests submitted at once."""
+    rng = np.random.default_rng(seed)
+    m = Model(Lx=400.0, Ly=400.0, Lz=60.0, ax=160.0, ay=120.0, deltat=0.05, kmin=3, n_k=2000, n_de=1000,
+              ub_de=20.0, n_species=5, detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0)
+    D = [stokes_einstein(d) for d in (100.0, 200.0, 300.0, 450.0, 600.0)]
+    srv = Server(device=0, arena_bytes=8 << 30, max_inflight=1024)
+    sid = srv.add_system(m)
+    reqs, steps = [], 0.0
+    for i in range(n_sims):
+        w = rng.dirichlet(np.ones(5))
+        p = proposal_record(m, D, list(w * 1e10), [2.0], i, 0, 1)
+        n = int(rng.poisson(1e10 * m.volume_factor))
+        reqs.append((p, np.array([2000], np.uint32), np.array([n], np.uint32)))
+        steps += n * 1999.0
+    # warm the kernels once
+    srv.submit(10 ** 6, sid, reqs[0][0], reqs[0][1], reqs[0][2], seed=1)
+    while not srv.poll(4, wait_us=10000):
+        pass
+    t0 = time.perf_counter()
+    for i, (p, f, n) in enumerate(reqs):
+        while not srv.submit(i, sid, p, f, n, seed=2):
+            srv.poll(0)
+    lat, done = [], 0
+    while done < n_sims:
+        for c in srv.poll(256, wait_us=2000):
+            lat.append(c.done_us - c.submit_us)
+            done += 1
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = np.array(lat)
+    return {"sims": n_sims, "particles_per_sim": steps / n_sims / 1999.0, "sims_per_s": n_sims / wall,
+            "particle_steps_per_s": steps / wall, "p50_ms": float(np.percentile(lat, 50) / 1e3),
+            "p99_ms": float(np.percentile(lat, 99) / 1e3)}
+
+
 def cpu_baseline(model, props, frames, parts, obs, n_sample):
     """Oracle FP64 restatement of the reference CPU path (simulate.jl +
     distance.jl) on the first n_sample proposals, one simulation per thread."""
@@ -235,6 +274,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--large-sims", type=int, default=64)
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: batched-direct replays of a reduced workload, then a 256 MiB "
                          "device copy whose known traffic calibrates the FETCH_SIZE/WRITE_SIZE counters")
@@ -299,6 +339,11 @@ def main():
             serve.append(serving_latency(args.serve_requests, rate))
             log(f"serving: {serve[-1]}")
 
+    large = None
+    if args.large_sims > 0:
+        large = large_model(args.large_sims)
+        log(f"large polydisperse model (serving loop): {large}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(model, props, frames, parts, obs, args.cpu_sample)
@@ -340,6 +385,7 @@ def main():
             "p99_ms": serve[1]["p99_ms"] if serve and len(serve) > 1 else (serve[0]["p99_ms"] if serve else None),
             "overflow_reruns": reruns,
             "tuning": tuning,
+            "large_polydisperse": large,
             "ms_per_step_graph_device": float(np.mean(total_ms)),
             "ms_per_step_direct": float(np.mean(dwalls)) * 1e3,
             "parallelism": "single GPU",

@@ -214,3 +214,37 @@ def test_autotune_picks_a_valid_shape_and_keeps_results(gpu_lib):
         assert ctx.get_tuning() == best
         dist, tracks, H, exact, _ = run_gpu(ctx, props, f, n, 33, _abi.PCS_MODE_BATCHED)
     check_against_oracle(model, obs, props, f, n, 33, dist, tracks, H, exact)
+
+
+def test_large_polydisperse_model(gpu_lib):
+    """The large polydisperse configuration (400x400x60 um box, 5 classes
+    100-600 nm with Dirichlet abundances, Gaussian z detection sigma 1 um,
+    2000 frames, H 2000 x 1000) with 12,000 particles so the oracle finishes
+    in seconds; FP32 drift against the FP64 twin is bounded."""
+    model = cases.headline_model(n_k=2000, n_de=1000, Lx=400.0, Ly=400.0, Lz=60.0, n_species=5,
+                                 detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0)
+    rng = np.random.default_rng(50)
+    w = rng.dirichlet(np.ones(5))
+    p = cases.proposal_record(model, FIVE_CLASSES, list(w * 1e10), [2.0], 77, 0, 1)
+    f, n = np.array([2000], np.uint32), np.array([12000], np.uint32)
+    obs = observed(model, particles=3000, frames=[2000])
+    with Context(model, obs) as ctx:
+        dist, tracks, H, exact, _ = run_gpu(ctx, [p], f, n, 6, _abi.PCS_MODE_BATCHED)
+    check_against_oracle(model, obs, [p], f, n, 6, dist, tracks, H, exact)
+    osys = O.system_from_abi(model.to_abi())
+    op = O.OracleProposal(p, f, n)
+    pd = O.orc_proposal_f64()
+    acc = 0.0
+    for i, D in enumerate(FIVE_CLASSES):
+        pd.sigma[i] = math.sqrt(2 * D * 0.05)
+        pd.half_az[i] = 1.0
+        acc += w[i]
+        pd.cum_prob[i] = acc if i < 4 else 2.0
+    worst = 0.0
+    for part in range(0, 12000, 1500):
+        x32, x64 = O.trace_particle(osys, op, pd, 6, 0, part)
+        d = np.abs(x32.astype(np.float64) - x64)
+        L = np.array([400.0, 400.0, 60.0])
+        worst = max(worst, np.minimum(d, L - d).max())
+    print(f"FP32 vs FP64 position drift over 2000 frames: {worst:.2e} um", flush=True)
+    assert worst < 1e-2
