@@ -197,6 +197,26 @@ int pcs_selftest_math(int device, const uint32_t* w, float* radius, float* s, fl
  * written); no reference counterpart. */
 int pcs_selftest_copy(int device, uint64_t bytes, float* ms);
 
+/* ------------------------------------------------------- beyond the walk */
+
+/* Track lists of one simulation: K and DE of every track with k >= kmin, in
+ * no particular order, with the walk of pcs_simulate (same stream, same FP32
+ * arithmetic), without binning - the synthetic-experiment generator of
+ * src/cpu/generate_experiment.jl:1-115 on the GPU. Writes up to cap pairs;
+ * *n_out receives the number of tracks (may exceed cap). */
+int pcs_generate_tracks(pcs_ctx* ctx, const pcs_proposal* prop, const uint32_t* video_frames,
+                        const uint32_t* video_particles, uint32_t n_videos, uint64_t seed, uint32_t* K, float* DE,
+                        uint64_t cap, uint64_t* n_out);
+
+/* pmc-standard importance weights on the GPU
+ * (src/gpu/detail/simulation_impl.inl:832-871, src/cpu/estimate.jl:193-208):
+ * w[j] proportional to 1 / sum_i w_old[i] prod_p N(theta_new[p][j] - theta_old[p][i]; 0, tau[p]),
+ * normalised to sum 1 (the Gaussian normalisation constants cancel).
+ * theta_* are n_params rows (row-major, row p = parameter p) of n_new /
+ * n_old members; n_params <= 24. FP64 throughout. */
+int pcs_pmc_weights(int device, const double* theta_new, uint32_t n_new, const double* theta_old,
+                    const double* w_old, uint32_t n_old, const double* tau, uint32_t n_params, double* w_out);
+
 /* ---------------------------------------------------------------- serving */
 
 typedef struct pcs_server_config {

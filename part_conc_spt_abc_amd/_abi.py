@@ -135,6 +135,9 @@ PROTOTYPES = {
     "pcs_autotune": (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, C.POINTER(pcs_tuning), C.POINTER(C.c_float)]),
     "pcs_selftest_math": (C.c_int, [C.c_int, P, P, P, P, C.c_uint64]),
     "pcs_selftest_copy": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_float)]),
+    "pcs_generate_tracks": (C.c_int, [P, C.POINTER(pcs_proposal), P, P, C.c_uint32, C.c_uint64, P, P, C.c_uint64,
+                                       C.POINTER(C.c_uint64)]),
+    "pcs_pmc_weights": (C.c_int, [C.c_int, P, C.c_uint32, P, P, C.c_uint32, P, C.c_uint32, P]),
     "pcs_server_create": (C.c_int, [C.c_int, C.POINTER(pcs_server_config), C.POINTER(P)]),
     "pcs_server_add_system": (C.c_int, [P, C.POINTER(pcs_system), P, U32P]),
     "pcs_server_submit": (C.c_int, [P, C.POINTER(pcs_request)]),

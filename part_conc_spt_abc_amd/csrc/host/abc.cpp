@@ -341,7 +341,21 @@ void Estimator::run()
                 to.push_back(az_[z]);
                 tau.push_back(tauAz_[z]);
             }
-            w_ = pmc_weights(tn, to, w_, tau);
+            if (N_ >= cfg_.device_weights_min) {
+                // O(N^2) kernel sums on the GPU
+                std::vector<double> xn, xo;
+                for (std::size_t p = 0; p < tn.size(); ++p) {
+                    xn.insert(xn.end(), tn[p].begin(), tn[p].end());
+                    xo.insert(xo.end(), to[p].begin(), to[p].end());
+                }
+                std::vector<double> wn(N_);
+                check(pcs_pmc_weights(cfg_.device, xn.data(), (uint32_t)N_, xo.data(), w_.data(), (uint32_t)N_,
+                                      tau.data(), (uint32_t)tau.size(), wn.data()),
+                      "pcs_pmc_weights");
+                w_.swap(wn);
+            } else {
+                w_ = pmc_weights(tn, to, w_, tau);
+            }
         } else {
             w_ = inverse_square_weights(distBis);
         }

@@ -41,6 +41,9 @@ struct Config {
     std::size_t maxIter = 0;  // 0 = until converged
     int verbosity = 0;
     uint64_t seed = 0;
+    // pmc-standard weights move to the GPU (pcs_pmc_weights) from this
+    // population size on; below it the host loop is cheaper than the copies.
+    std::size_t device_weights_min = 2048;
 };
 
 class Estimator {

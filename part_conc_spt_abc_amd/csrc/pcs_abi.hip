@@ -893,6 +893,137 @@ int pcs_autotune(pcs_ctx* c, const pcs_proposal* props, uint32_t n, const uint32
     return PCS_OK;
 }
 
+int pcs_generate_tracks(pcs_ctx* c, const pcs_proposal* prop, const uint32_t* vframes, const uint32_t* vparticles,
+                        uint32_t n_videos, uint64_t seed, uint32_t* K, float* DE, uint64_t cap, uint64_t* n_out)
+{
+    if (!c || !prop || !n_out || (n_videos && (!vframes || !vparticles)) || (cap && (!K || !DE)))
+        return set_error(PCS_EINVAL, "NULL argument");
+    if (prop->reserved != 0) return set_error(PCS_EINVAL, "reserved field must be 0");
+    PCS_TRY(validate_proposal(c->sys, *prop));
+    PCS_HIP(hipSetDevice(c->device));
+    // Exact bound: a track needs kmin detected frames and one undetected
+    // frame (or the end), so a video holds at most (F+1)/(kmin+1) per particle.
+    uint64_t words = 0;
+    std::vector<uint32_t> vtile(n_videos + 1), vprop(n_videos, 0u);
+    uint32_t tiles = 0;
+    for (uint32_t v = 0; v < n_videos; ++v) {
+        words += 2ull * vparticles[v] * ((vframes[v] + 1u) / (c->sys.kmin + 1u));
+        vtile[v] = tiles;
+        tiles += (vparticles[v] + 63u) / 64u;
+    }
+    vtile[n_videos] = tiles;
+    if (words > 0xFFFFFFFFull) return set_error(PCS_EINVAL, "simulation too large for one track list");
+    DevProp dp{};
+    dp.sim_id = prop->sim_id;
+    dp.first_video = 0;
+    dp.n_videos = n_videos;
+    dp.rec_cap = (uint32_t)words;
+    for (int s = 0; s < PCS_MAX_SPECIES; ++s) {
+        dp.half_az[s] = prop->half_az[s];
+        dp.cum_prob[s] = prop->cum_prob[s];
+        dp.sigma[s] = prop->sigma[s];
+    }
+    uint32_t *d_rec = nullptr, *d_v = nullptr, *d_count = nullptr;
+    DevProp* d_prop = nullptr;
+    uint64_t* d_seed = nullptr;
+    int r = dev_alloc(&d_rec, words);
+    if (r == PCS_OK) r = dev_alloc(&d_v, 4ull * n_videos + 1);
+    if (r == PCS_OK) r = dev_alloc(&d_count, 1);
+    if (r == PCS_OK) r = dev_alloc(&d_prop, 1);
+    if (r == PCS_OK) r = dev_alloc(&d_seed, 1);
+    uint32_t count = 0;
+    if (r == PCS_OK) {
+        dp.rec = d_rec;
+        dp.srt = nullptr;
+        hipStream_t st = c->stream;
+        hipError_t e = hipMemcpyAsync(d_prop, &dp, sizeof(dp), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && n_videos) e = hipMemcpyAsync(d_v, vframes, n_videos * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && n_videos)
+            e = hipMemcpyAsync(d_v + n_videos, vparticles, n_videos * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && n_videos)
+            e = hipMemcpyAsync(d_v + 2 * n_videos, vprop.data(), n_videos * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_v + 3 * n_videos, vtile.data(), (n_videos + 1) * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_seed, &seed, 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = launch_reset(d_count, 1, nullptr, st);
+        if (e == hipSuccess) {
+            WalkArgs a{};
+            a.sys = c->d_sys;
+            a.props = d_prop;
+            a.vframes = d_v;
+            a.vparticles = d_v + n_videos;
+            a.vprop = d_v + 2 * n_videos;
+            a.vtile = d_v + 3 * n_videos;
+            a.n_videos = n_videos;
+            a.tile_lo = 0;
+            a.tile_hi = tiles;
+            a.stage_cap = 256;
+            a.rec_count = d_count;
+            a.seed = d_seed;
+            e = launch_tracks(a, c->sys.detect_model == PCS_DETECT_GAUSS_Z, c->sys.z_boundary == PCS_Z_REFLECT, st);
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&count, d_count, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        std::vector<uint32_t> words_h;
+        if (e == hipSuccess && count) {
+            words_h.resize(count);
+            e = hipMemcpy(words_h.data(), d_rec, (size_t)count * 4, hipMemcpyDeviceToHost);
+        }
+        if (e != hipSuccess) r = hip_error(e, "pcs_generate_tracks");
+        const uint64_t n = count / 2u;
+        for (uint64_t i = 0; i < n && i < cap && r == PCS_OK; ++i) {
+            K[i] = words_h[2 * i];
+            std::memcpy(&DE[i], &words_h[2 * i + 1], 4);
+        }
+        *n_out = n;
+    }
+    void* ptrs[] = {d_rec, d_v, d_count, d_prop, d_seed};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    return r;
+}
+
+int pcs_pmc_weights(int device, const double* theta_new, uint32_t n_new, const double* theta_old, const double* w_old,
+                    uint32_t n_old, const double* tau, uint32_t n_params, double* w_out)
+{
+    if (n_params < 1 || n_params > 3 * PCS_MAX_SPECIES) return set_error(PCS_EINVAL, "n_params outside [1, 24]");
+    if ((n_new && (!theta_new || !w_out)) || (n_old && (!theta_old || !w_old)) || !tau)
+        return set_error(PCS_EINVAL, "NULL argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PCS_ENODEV, "no GPU visible");
+    if (device < 0 || device >= ndev) return set_error(PCS_EINVAL, "device %d outside [0, %d)", device, ndev);
+    PCS_HIP(hipSetDevice(device));
+    if (n_new == 0) return PCS_OK;
+    // scale by 1/tau on the host so the kernel works in standard units
+    std::vector<double> xn((size_t)n_params * n_new), xo((size_t)n_params * n_old);
+    for (uint32_t p = 0; p < n_params; ++p) {
+        for (uint32_t j = 0; j < n_new; ++j) xn[(size_t)p * n_new + j] = theta_new[(size_t)p * n_new + j] / tau[p];
+        for (uint32_t i = 0; i < n_old; ++i) xo[(size_t)p * n_old + i] = theta_old[(size_t)p * n_old + i] / tau[p];
+    }
+    double *d_xn = nullptr, *d_xo = nullptr, *d_w = nullptr, *d_s = nullptr;
+    int r = dev_alloc(&d_xn, xn.size());
+    if (r == PCS_OK) r = dev_alloc(&d_xo, xo.size() + 1);
+    if (r == PCS_OK) r = dev_alloc(&d_w, (size_t)n_old + 1);
+    if (r == PCS_OK) r = dev_alloc(&d_s, n_new);
+    if (r == PCS_OK) {
+        hipError_t e = hipMemcpy(d_xn, xn.data(), xn.size() * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess && n_old) e = hipMemcpy(d_xo, xo.data(), xo.size() * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess && n_old) e = hipMemcpy(d_w, w_old, (size_t)n_old * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = launch_pmc_sums(d_xn, n_new, d_xo, d_w, n_old, n_params, d_s, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(w_out, d_s, (size_t)n_new * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) r = hip_error(e, "pcs_pmc_weights");
+    }
+    void* ptrs[] = {d_xn, d_xo, d_w, d_s};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (r != PCS_OK) return r;
+    double sum = 0.0;
+    for (uint32_t j = 0; j < n_new; ++j) sum += (w_out[j] = 1.0 / w_out[j]);
+    for (uint32_t j = 0; j < n_new; ++j) w_out[j] /= sum;
+    return PCS_OK;
+}
+
 int pcs_simulate(pcs_ctx* c, const pcs_proposal* props, uint32_t n, const uint32_t* vframes,
                  const uint32_t* vparticles, uint32_t n_videos, uint64_t seed, int mode, double* dist_out,
                  uint32_t* hist_out, uint32_t* tracks_out)

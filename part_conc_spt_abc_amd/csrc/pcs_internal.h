@@ -79,6 +79,10 @@ hipError_t launch_reset(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter,
 hipError_t launch_walk(const WalkArgs& a, int ppl, int waves_per_group, bool persistent,
                        uint32_t grid_groups, bool gauss, bool zreflect, hipStream_t s);
 hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s);
+// Track-list form (k, DE) of the walk, one wave per tile, tiles of 64 particles.
+hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s);
+hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, const double* w_old, uint32_t n_old,
+                           uint32_t P, double* wsum, hipStream_t s);
 hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, float* c, uint64_t n, hipStream_t st);
 hipError_t launch_selftest_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st);
 hipError_t launch_dense_hist(const DevProp* props, uint32_t prop_lo, uint32_t n_props,

@@ -79,6 +79,20 @@ __device__ __forceinline__ void stage_push(Stage& st, bool ok, uint32_t v)
     st.cnt += (uint32_t)__builtin_popcountll(m);
 }
 
+// Two-word records (k, DE bits) of the track-list form.
+__device__ __forceinline__ void stage_push2(Stage& st, bool ok, uint32_t a, uint32_t b)
+{
+    const uint64_t m = ballot(ok);
+    if (m == 0ull) return;
+    if (st.cnt + (uint32_t)(2 * kWave) > st.cap) stage_flush(st);
+    const uint32_t pos = st.cnt + 2u * mbcnt64(m);
+    if (ok) {
+        st.buf[pos] = a;
+        st.buf[pos + 1u] = b;
+    }
+    st.cnt += 2u * (uint32_t)__builtin_popcountll(m);
+}
+
 // Track record: de / ((k-1) 4 dt) then bin = trunc(value / dDE), dropped when
 // the bin or the length is outside the matrix
 // (src/gpu/cusim/kernel-simulate-unroll.inl:187-190, src/gpu/cusim/histogram2d.inl:98-106).
@@ -102,7 +116,10 @@ struct Particle {
     bool active;
 };
 
-template <int PPL, bool GAUSS, bool ZREF>
+// TRACKS = true: the track-list form of src/cpu/generate_experiment.jl -
+// every track with k >= kmin is emitted as (k, de / ((k-1) 4 dt)) without
+// binning; otherwise tracks are binned records (row << 16 | bin).
+template <int PPL, bool GAUSS, bool ZREF, bool TRACKS>
 __device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stage& st, uint32_t s0, uint32_t s1)
 {
     // Video owning tile t: largest v with vtile[v] <= t (wave-uniform search).
@@ -215,10 +232,16 @@ __device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stag
                           __builtin_fabsf(q.y) <= hay;
                 const bool closing = q.active && !det && q.k >= kmin;
                 if (ballot(closing) != 0ull) {
-                    uint32_t rv = 0u;
-                    bool ok = false;
-                    if (closing) ok = track_record(q.k, q.de, four_dt, dde, n_k, n_de, rv);
-                    stage_push(st, ok, rv);
+                    if (TRACKS) {
+                        float v = 0.0f;
+                        if (closing) v = q.de / ((float)(q.k - 1u) * four_dt);
+                        stage_push2(st, closing, q.k, __float_as_uint(v));
+                    } else {
+                        uint32_t rv = 0u;
+                        bool ok = false;
+                        if (closing) ok = track_record(q.k, q.de, four_dt, dde, n_k, n_de, rv);
+                        stage_push(st, ok, rv);
+                    }
                 }
                 const float de_n = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, q.de));
                 q.de = (det && q.k >= 1u) ? de_n : 0.0f;
@@ -231,16 +254,22 @@ __device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stag
     for (int i = 0; i < PPL; ++i) {
         const bool closing = pt[i].active && pt[i].k >= kmin;
         if (ballot(closing) != 0ull) {
-            uint32_t rv = 0u;
-            bool ok = false;
-            if (closing) ok = track_record(pt[i].k, pt[i].de, four_dt, dde, n_k, n_de, rv);
-            stage_push(st, ok, rv);
+            if (TRACKS) {
+                float v = 0.0f;
+                if (closing) v = pt[i].de / ((float)(pt[i].k - 1u) * four_dt);
+                stage_push2(st, closing, pt[i].k, __float_as_uint(v));
+            } else {
+                uint32_t rv = 0u;
+                bool ok = false;
+                if (closing) ok = track_record(pt[i].k, pt[i].de, four_dt, dde, n_k, n_de, rv);
+                stage_push(st, ok, rv);
+            }
         }
     }
     stage_flush(st);
 }
 
-template <int PPL, bool PERSIST, bool GAUSS, bool ZREF>
+template <int PPL, bool PERSIST, bool GAUSS, bool ZREF, bool TRACKS = false>
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
 {
     extern __shared__ uint32_t stage_lds[];
@@ -258,11 +287,11 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
             if (lane_id() == 0u) t = atomicAdd(a.tile_counter, 1u);
             t = uniform(t) + a.tile_lo;
             if (t >= a.tile_hi) break;
-            process_tile<PPL, GAUSS, ZREF>(a, t, st, k0, k1);
+            process_tile<PPL, GAUSS, ZREF, TRACKS>(a, t, st, k0, k1);
         }
     } else {
         const uint32_t t = a.tile_lo + blockIdx.x * wpg + wave;
-        if (t < a.tile_hi) process_tile<PPL, GAUSS, ZREF>(a, t, st, k0, k1);
+        if (t < a.tile_hi) process_tile<PPL, GAUSS, ZREF, TRACKS>(a, t, st, k0, k1);
     }
 }
 
@@ -461,7 +490,54 @@ __global__ void selftest_copy_kernel(const uint4* __restrict__ src, uint4* __res
         dst[i] = src[i];
 }
 
+// pmc-standard importance kernel sums: wsum[j] = sum_i w_old[i] exp(-1/2
+// sum_p ((x_new[p][j] - x_old[p][i]) / tau_p)^2), inputs pre-divided by tau.
+// One thread per new member, old members staged through LDS in tiles.
+constexpr int kPmcMaxP = 3 * PCS_MAX_SPECIES;
+
+__global__ __launch_bounds__(256) void pmc_sums_kernel(const double* xn, uint32_t n_new, const double* xo,
+                                                       const double* w_old, uint32_t n_old, uint32_t P, double* wsum)
+{
+    __shared__ double sh[kPmcMaxP][256];
+    __shared__ double shw[256];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t j = blockIdx.x * 256u + tid;
+    double x[kPmcMaxP];
+#pragma unroll
+    for (int p = 0; p < kPmcMaxP; ++p) x[p] = ((uint32_t)p < P && j < n_new) ? xn[(uint64_t)p * n_new + j] : 0.0;
+    double acc = 0.0;
+    for (uint32_t t0 = 0; t0 < n_old; t0 += 256u) {
+        const uint32_t i = t0 + tid;
+        for (uint32_t p = 0; p < P; ++p) sh[p][tid] = i < n_old ? xo[(uint64_t)p * n_old + i] : 0.0;
+        shw[tid] = i < n_old ? w_old[i] : 0.0;
+        __syncthreads();
+        const uint32_t m = min(256u, n_old - t0);
+        for (uint32_t ii = 0; ii < m; ++ii) {
+            double s2 = 0.0;
+#pragma unroll
+            for (int p = 0; p < kPmcMaxP; ++p) {
+                if ((uint32_t)p < P) {
+                    const double d = x[p] - sh[p][ii];
+                    s2 = fma(d, d, s2);
+                }
+            }
+            acc = fma(shw[ii], exp(-0.5 * s2), acc);
+        }
+        __syncthreads();
+    }
+    if (j < n_new) wsum[j] = acc;
+}
+
 // ---------------------------------------------------------------- launchers
+
+hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, const double* w_old, uint32_t n_old,
+                           uint32_t P, double* wsum, hipStream_t s)
+{
+    if (n_new == 0) return hipSuccess;
+    hipLaunchKernelGGL(pmc_sums_kernel, dim3((n_new + 255u) / 256u), dim3(256), 0, s, xn, n_new, xo, w_old, n_old, P,
+                       wsum);
+    return hipGetLastError();
+}
 
 hipError_t launch_selftest_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st)
 {
@@ -517,6 +593,22 @@ hipError_t launch_walk(const WalkArgs& a, int ppl, int wpg, bool persistent, uin
                           : launch_walk_m<2, false>(a, wpg, grid, gauss, zref, s);
     return persistent ? launch_walk_m<1, true>(a, wpg, grid, gauss, zref, s)
                       : launch_walk_m<1, false>(a, wpg, grid, gauss, zref, s);
+}
+
+hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s)
+{
+    if (a.tile_hi <= a.tile_lo) return hipSuccess;
+    const uint32_t grid = (a.tile_hi - a.tile_lo + 3u) / 4u;
+    const size_t shmem = (size_t)4 * a.stage_cap * sizeof(uint32_t);
+    if (gauss && zref)
+        hipLaunchKernelGGL((walk_kernel<1, false, true, true, true>), dim3(grid), dim3(256), shmem, s, a);
+    else if (gauss)
+        hipLaunchKernelGGL((walk_kernel<1, false, true, false, true>), dim3(grid), dim3(256), shmem, s, a);
+    else if (zref)
+        hipLaunchKernelGGL((walk_kernel<1, false, false, true, true>), dim3(grid), dim3(256), shmem, s, a);
+    else
+        hipLaunchKernelGGL((walk_kernel<1, false, false, false, true>), dim3(grid), dim3(256), shmem, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s)

@@ -248,3 +248,51 @@ def test_large_polydisperse_model(gpu_lib):
         worst = max(worst, np.minimum(d, L - d).max())
     print(f"FP32 vs FP64 position drift over 2000 frames: {worst:.2e} um", flush=True)
     assert worst < 1e-2
+
+
+def test_generate_tracks_matches_oracle(gpu_lib):
+    """Track-list form (generate_experiment.jl) on the GPU: the same (K, DE)
+    multiset as the oracle, DE bit for bit."""
+    import ctypes as C
+
+    for variant in ("slab", "gauss"):
+        kw = dict(detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0) if variant == "gauss" else {}
+        model = cases.headline_model(**kw)
+        p, f, n = cases.fixed_proposal(model, [900, 0, 130], [1000, 500, 2000], sim_id=41)
+        with Context(model) as ctx:
+            cap = 1 << 20
+            K = np.zeros(cap, np.uint32)
+            DE = np.zeros(cap, np.float32)
+            cnt = C.c_uint64(0)
+            _abi.check(gpu_lib.pcs_generate_tracks(ctx.h, C.byref(p), f.ctypes.data, n.ctypes.data, len(f), 19,
+                                                   K.ctypes.data, DE.ctypes.data, cap, C.byref(cnt)),
+                       "pcs_generate_tracks")
+        m = int(cnt.value)
+        Ko, DEo = O.tracks_f32(O.system_from_abi(model.to_abi()), O.OracleProposal(p, f, n), 19)
+        assert m == len(Ko) > 100
+        got = sorted(zip(K[:m].tolist(), DE[:m].view(np.uint32).tolist()))
+        want = sorted(zip(Ko.tolist(), DEo.view(np.uint32).tolist()))
+        assert got == want, variant
+
+
+def test_pmc_weights_on_device(gpu_lib):
+    """Device pmc-standard weights vs the formula in FP64 numpy
+    (normpdf products with their constants, then normalisation)."""
+    import ctypes as C
+
+    rng = np.random.default_rng(8)
+    P, n_old, n_new = 5, 700, 900
+    old = rng.normal(0.0, 1.0, (P, n_old)) * np.array([[0.5], [2.0], [1.0], [0.3], [3.0]])
+    new = old[:, rng.integers(0, n_old, n_new)] + rng.normal(0.0, 0.2, (P, n_new))
+    w_old = rng.random(n_old)
+    w_old /= w_old.sum()
+    tau = np.sqrt(2.0 * old.var(axis=1))
+    out = np.zeros(n_new)
+    _abi.check(gpu_lib.pcs_pmc_weights(0, new.ctypes.data, n_new, old.ctypes.data, w_old.ctypes.data, n_old,
+                                       tau.ctypes.data, P, out.ctypes.data), "pcs_pmc_weights")
+    d = (new[:, :, None] - old[:, None, :]) / tau[:, None, None]
+    k = np.prod(np.exp(-0.5 * d ** 2) / (np.sqrt(2 * np.pi) * tau[:, None, None]), axis=0)
+    ws = 1.0 / (k * w_old[None, :]).sum(axis=1)
+    ws /= ws.sum()
+    assert np.allclose(out, ws, rtol=1e-11, atol=0)
+    assert abs(out.sum() - 1.0) < 1e-12
