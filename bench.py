@@ -294,7 +294,9 @@ def main():
     log(f"observed histogram: {int(obs.sum())} tracks")
 
     rng = np.random.default_rng(1)
+    t_pack = time.perf_counter()
     props, frames, parts = workload(model, args.batched_sims, rng, (500, 1000, 2000))
+    t_pack = time.perf_counter() - t_pack
     steps_per_gen = particle_steps(frames, parts)
     tuning = None
     if not args.no_autotune:
@@ -303,8 +305,12 @@ def main():
         tuning, tms = ctx.autotune(props[:k], frames[:k], parts[:k])
         log(f"autotune: {tuning} ({tms:.2f} ms on {k} proposals)")
     b = ctx.batch()
+    t_load = time.perf_counter()
     b.load(props, frames, parts)
-    log(f"batched: {args.batched_sims} proposals, {steps_per_gen:.3e} particle-steps per generation")
+    t_load = time.perf_counter() - t_load
+    arena_cap, arena_bytes = b.arena()
+    log(f"batched: {args.batched_sims} proposals, {steps_per_gen:.3e} particle-steps per generation, "
+        f"host packing {t_pack:.2f} s, load {t_load * 1e3:.1f} ms, arena {arena_bytes / 2**30:.2f} GiB")
     walls, total_ms, _ = run_timed(b, args.steps, args.warmup, _abi.PCS_MODE_BATCHED, 100)
     reruns = b.overflow_reruns()
     # The same three kernels launched without the graph, with HIP events
@@ -312,6 +318,7 @@ def main():
     dwalls, _, walk_ms = run_timed(b, args.steps, 0, _abi.PCS_MODE_BATCHED_DIRECT, 100)
     dist, tracks, _ = b.read()
     b.close()
+    arena_use = float(tracks.astype(np.float64).sum()) / max(arena_cap, 1)
     wall = sum(walls)
     sims_s = args.batched_sims * args.steps / wall
     psteps_s = steps_per_gen * args.steps / wall
@@ -385,6 +392,10 @@ def main():
             "p99_ms": serve[1]["p99_ms"] if serve and len(serve) > 1 else (serve[0]["p99_ms"] if serve else None),
             "overflow_reruns": reruns,
             "tuning": tuning,
+            "host_packing_s": t_pack,
+            "load_ms": t_load * 1e3,
+            "arena_bytes": arena_bytes,
+            "arena_use": arena_use,
             "large_polydisperse": large,
             "ms_per_step_graph_device": float(np.mean(total_ms)),
             "ms_per_step_direct": float(np.mean(dwalls)) * 1e3,

@@ -159,6 +159,9 @@ int pcs_batch_timing(pcs_batch* b, float out[3]);
 /* Number of proposals whose record arena overflowed in the last run and
  * were rerun (informational). */
 int pcs_batch_overflow_reruns(pcs_batch* b, uint32_t* n);
+/* Record arena of the loaded batch: total slice capacity in records and the
+ * device bytes held by the record and sort-scratch arenas. */
+int pcs_batch_arena(pcs_batch* b, uint64_t* capacity_records, uint64_t* bytes);
 int pcs_batch_destroy(pcs_batch* b);
 
 /* Launch shape of the batched walk kernel. Zero fields mean "default".

@@ -128,6 +128,7 @@ PROTOTYPES = {
     "pcs_batch_read_exact": (C.c_int, [P, P]),
     "pcs_batch_timing": (C.c_int, [P, C.POINTER(C.c_float)]),
     "pcs_batch_overflow_reruns": (C.c_int, [P, U32P]),
+    "pcs_batch_arena": (C.c_int, [P, U64P, U64P]),
     "pcs_batch_destroy": (C.c_int, [P]),
     "pcs_set_tuning": (C.c_int, [P, C.POINTER(pcs_tuning)]),
     "pcs_get_tuning": (C.c_int, [P, C.POINTER(pcs_tuning)]),

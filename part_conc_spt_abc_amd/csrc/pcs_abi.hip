@@ -725,6 +725,14 @@ int pcs_batch_timing(pcs_batch* b, float out[3])
     return PCS_OK;
 }
 
+int pcs_batch_arena(pcs_batch* b, uint64_t* capacity_records, uint64_t* bytes)
+{
+    if (!b) return set_error(PCS_EINVAL, "batch is NULL");
+    if (capacity_records) *capacity_records = b->arena_records;
+    if (bytes) *bytes = 2ull * 4ull * b->cap_arena;
+    return PCS_OK;
+}
+
 int pcs_batch_overflow_reruns(pcs_batch* b, uint32_t* n)
 {
     if (!b || !n) return set_error(PCS_EINVAL, "NULL argument");

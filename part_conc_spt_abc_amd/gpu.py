@@ -139,6 +139,12 @@ class Batch:
         _abi.check(self.lib.pcs_batch_timing(self.h, t), "pcs_batch_timing")
         return float(t[0]), float(t[1]), float(t[2])
 
+    def arena(self):
+        """(slice capacity in records, device bytes of the record arenas)."""
+        cap, nb = C.c_uint64(0), C.c_uint64(0)
+        _abi.check(self.lib.pcs_batch_arena(self.h, C.byref(cap), C.byref(nb)), "pcs_batch_arena")
+        return int(cap.value), int(nb.value)
+
     def overflow_reruns(self) -> int:
         n = C.c_uint32(0)
         _abi.check(self.lib.pcs_batch_overflow_reruns(self.h, C.byref(n)), "pcs_batch_overflow_reruns")
