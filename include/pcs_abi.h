@@ -128,6 +128,8 @@ int pcs_reference_hist(const uint64_t* K, const double* DE, uint64_t n, double u
  * (plain counts, n_k*n_de; the cumulative table is built internally, replacing
  * the host SAT + upload of src/gpu/detail/simulation_impl.inl:617-629). */
 int pcs_open(const pcs_system* sys, const uint32_t* obs_hist, int device, pcs_ctx** out);
+/* Releases the caller's handle. Batches created from the context keep it
+ * alive until they are destroyed; no other call may use ctx afterwards. */
 int pcs_close(pcs_ctx* ctx);
 
 /* One-shot: simulate n proposals and write log10 distances (n doubles).
@@ -222,11 +224,27 @@ int pcs_pmc_weights(int device, const double* theta_new, uint32_t n_new, const d
 
 /* ---------------------------------------------------------------- serving */
 
+/* Serving forms (pcs_server_config.flags):
+ *   0                      host-driven: a scheduler thread packs the requests
+ *                          admitted in one pass into a launch group (walk +
+ *                          distance kernels) on the next of 4 streams.
+ *   PCS_SERVE_DEVICE_LOOP  device-resident: one persistent kernel takes
+ *                          requests from a ring in pinned host memory, claims
+ *                          their tiles in arrival order, computes each distance
+ *                          in the group that finishes the last tile and writes
+ *                          the result back to host memory; the host thread
+ *                          only admits (record arena, ring slot) and collects.
+ *                          The kernel leaves after loop_idle_us without work
+ *                          and is launched again when requests arrive.
+ * Both give the same result for the same request. */
+#define PCS_SERVE_DEVICE_LOOP 1u
+
 typedef struct pcs_server_config {
     uint64_t arena_bytes;    /* HBM budget for track records (admission), 0 = 1 GiB */
     uint32_t max_inflight;   /* simulations on the GPU at once, 0 = 4096 */
     uint32_t max_videos;     /* videos per request, 0 = 64 */
-    uint32_t flags;          /* reserved, 0 */
+    uint32_t flags;          /* 0 or PCS_SERVE_DEVICE_LOOP */
+    uint32_t loop_idle_us;   /* device loop: idle time before it leaves, 0 = 20000 */
 } pcs_server_config;
 
 typedef struct pcs_request {

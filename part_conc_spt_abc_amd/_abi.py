@@ -31,6 +31,7 @@ PCS_DIST_L1 = 1
 PCS_MODE_PLAIN = 0
 PCS_MODE_BATCHED = 1
 PCS_MODE_BATCHED_DIRECT = 2
+PCS_SERVE_DEVICE_LOOP = 1
 
 F8 = C.c_float * PCS_MAX_SPECIES
 
@@ -79,6 +80,7 @@ class pcs_server_config(C.Structure):
         ("max_inflight", C.c_uint32),
         ("max_videos", C.c_uint32),
         ("flags", C.c_uint32),
+        ("loop_idle_us", C.c_uint32),
     ]
 
 

@@ -1,6 +1,7 @@
 // pcs_abi.hip -- C ABI of the hot path: contexts, resident batches (plain and
 // batched forms), overflow reruns. The serving loop lives in pcs_server.hip.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -154,6 +155,9 @@ struct pcs_ctx {
     hipStream_t stream = nullptr;
     pcs_tuning tune{1, 4, 8, 256};
     double cap_factor = 0.5;
+    // The caller's handle plus one per live batch: pcs_close drops the
+    // caller's, and the context is freed when its last batch is destroyed.
+    std::atomic<int> refs{1};
 };
 
 struct pcs_batch {
@@ -501,14 +505,24 @@ int pcs_open(const pcs_system* sys, const uint32_t* obs_hist, int device, pcs_ct
     return PCS_OK;
 }
 
-int pcs_close(pcs_ctx* c)
+namespace {
+
+void ctx_release(pcs_ctx* c)
 {
-    if (!c) return PCS_OK;
+    if (c->refs.fetch_sub(1) != 1) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->d_sat) hipFree(c->d_sat);
     if (c->d_sys) hipFree(c->d_sys);
     delete c;
+}
+
+}  // namespace
+
+int pcs_close(pcs_ctx* c)
+{
+    if (!c) return PCS_OK;
+    ctx_release(c);
     return PCS_OK;
 }
 
@@ -540,6 +554,7 @@ int pcs_batch_create(pcs_ctx* c, pcs_batch** out)
     pcs_batch* b = new (std::nothrow) pcs_batch();
     if (!b) return set_error(PCS_ENOMEM, "batch");
     b->ctx = c;
+    c->refs.fetch_add(1);
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&b->ev[i]);
     if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_seed, sizeof(uint64_t), hipHostMallocDefault);
@@ -568,7 +583,9 @@ int pcs_batch_destroy(pcs_batch* b)
     for (hipEvent_t& e : b->ev)
         if (e) hipEventDestroy(e);
     if (b->stream) hipStreamDestroy(b->stream);
+    pcs_ctx* c = b->ctx;
     delete b;
+    ctx_release(c);
     return PCS_OK;
 }
 

@@ -74,6 +74,74 @@ struct DistArgs {
     int32_t* status;            // PCS_OK or PCS_EOVERFLOW per proposal
 };
 
+// ------------------------------------------------ device-resident request loop
+//
+// Requests are numbered r = 0, 1, 2, ... in the order the host publishes them
+// and live in ring slot r % R. The host writes a LoopReq (+ the slot's video
+// arrays) into pinned, coherent host memory and publishes it by storing
+// seq = r + 1 last. Inside the loop kernel a slot of HBM state moves through
+//   free (0, or 4(r-R)+3)  ->  4r+1 (being copied in)  ->  4r+2 (tiles open)
+//   ->  4r+3 (result written; slot free for r + R)
+// and one 64-bit cursor (request << 24 | next tile) hands out tiles in
+// request order. The group that finishes a request's last tile computes its
+// distance and writes a LoopDone into host memory, seq = r + 1 last.
+constexpr uint32_t kLoopTileBits = 24;
+constexpr uint32_t kLoopMaxTiles = (1u << kLoopTileBits) - 64u;
+constexpr int kLoopWaves = 4;  // waves per group; a group claims up to 4 tiles at a time
+
+struct LoopReq {
+    uint64_t seq;
+    uint32_t n_videos;
+    uint32_t tiles;  // >= 1 (a request without particles has one empty tile)
+    DevProp prop;    // first_video = 0; rec/srt point at the admitted arena range
+};
+static_assert(sizeof(LoopReq) == 160, "LoopReq layout");
+
+struct LoopDone {
+    uint64_t seq;
+    double dist;
+    uint64_t exact;
+    uint32_t tracks;
+    int32_t status;
+};
+static_assert(sizeof(LoopDone) == 32, "LoopDone layout");
+
+struct LoopSlot {
+    uint64_t state;
+    uint32_t tiles, tiles_done, rec_count, n_videos;
+    uint64_t pad;
+    DevProp prop;
+};
+static_assert(sizeof(LoopSlot) == 176, "LoopSlot layout");
+
+struct LoopCtl {
+    uint64_t cursor;
+    uint64_t last_claim;  // wall clock of the latest tile claim, any group
+    uint32_t quit;        // set when the loop should end: every group leaves
+    uint32_t pad;
+};
+
+// Per-slot video table, (maxv + 1) entries of 3 words at slot * 3 * (maxv + 1)
+// in host memory (ring_vid) and HBM (slot_vid): entry v = {frames, particles,
+// first tile} of video v, entry n_videos = {0, 0, total tiles}.
+constexpr uint32_t kLoopPollers = 8;  // groups that read the host ring and stop flag
+
+struct LoopArgs {
+    const DevSys* sys;
+    LoopSlot* slots;
+    uint32_t* slot_vid;
+    const LoopReq* ring;
+    const uint32_t* ring_vid;
+    LoopDone* done;
+    const uint32_t* stop;   // host memory: non-zero = leave now
+    LoopCtl* ctl;
+    uint32_t R, maxv;
+    uint32_t stage_cap;     // LDS records per wave
+    uint32_t nk_lds, nde_lds;
+    uint32_t pad;
+    uint64_t idle_ticks;    // wall-clock ticks without work before the loop ends
+};
+
 // Launchers (pcs_kernels.hip). waves_per_group in {1,2,4}; ppl in {1,2}.
 hipError_t launch_reset(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter, hipStream_t s);
 hipError_t launch_walk(const WalkArgs& a, int ppl, int waves_per_group, bool persistent,
@@ -85,6 +153,11 @@ hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, c
                            uint32_t P, double* wsum, hipStream_t s);
 hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, float* c, uint64_t n, hipStream_t st);
 hipError_t launch_selftest_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st);
+// Request loop: `grid` groups of kLoopWaves waves; n_de_max selects the
+// distance segment width. Returns once launched.
+hipError_t launch_serve_loop(const LoopArgs& a, uint32_t grid, uint32_t n_de_max, hipStream_t s);
+size_t serve_loop_lds(const LoopArgs& a);
+hipError_t serve_loop_occupancy(const LoopArgs& a, uint32_t n_de_max, int* groups_per_cu);
 hipError_t launch_dense_hist(const DevProp* props, uint32_t prop_lo, uint32_t n_props,
                              const uint32_t* rec_count, uint32_t max_count,
                              uint32_t n_k, uint32_t n_de, uint32_t* hist, hipStream_t s);

@@ -119,26 +119,16 @@ struct Particle {
 // TRACKS = true: the track-list form of src/cpu/generate_experiment.jl -
 // every track with k >= kmin is emitted as (k, de / ((k-1) 4 dt)) without
 // binning; otherwise tracks are binned records (row << 16 | bin).
+// Walk of one tile: particles [p0, p0 + 64*PPL) of video `video` (F frames,
+// N particles) of simulation pr; the caller has pointed the stage at pr's
+// record slice and counter.
 template <int PPL, bool GAUSS, bool ZREF, bool TRACKS>
-__device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stage& st, uint32_t s0, uint32_t s1)
+__device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, uint32_t F, uint32_t N,
+                                          uint32_t video, uint32_t p0, Stage& st, uint32_t s0, uint32_t s1)
 {
-    // Video owning tile t: largest v with vtile[v] <= t (wave-uniform search).
-    uint32_t lo = 0u, hi = a.n_videos;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a.vtile[mid] <= t) lo = mid; else hi = mid;
-    }
-    const uint32_t v = lo;
-    const uint32_t P = a.vprop[v];
-    const DevProp& pr = a.props[P];
-    const DevSys& sy = a.sys[pr.sys_id];
-
-    const uint32_t F = a.vframes[v];
-    const uint32_t N = a.vparticles[v];
-    const uint32_t video = v - pr.first_video;
+    if (p0 >= N) return;  // empty tile (a request without particles)
     const uint32_t sim = pr.sim_id;
     const uint32_t k0 = s0 ^ pr.key0, k1 = s1 ^ pr.key1;
-    const uint32_t p0 = (t - a.vtile[v]) * (uint32_t)(kWave * PPL);
 
     const float hx = sy.half_l[0], hy = sy.half_l[1], hz = sy.half_l[2];
     const float Lx = 2.0f * hx, Ly = 2.0f * hy, Lz = 2.0f * hz;
@@ -147,10 +137,6 @@ __device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stag
     const uint32_t kmin = sy.kmin, n_k = sy.n_k, n_de = sy.n_de, C = sy.n_species;
     const float four_dt = sy.four_dt, dde = sy.de_bin_width;
     constexpr uint32_t B = GAUSS ? 4u : 3u;
-
-    st.rec = pr.rec;
-    st.slice_cap = pr.rec_cap;
-    st.counter = a.rec_count + P;
 
     Particle pt[PPL];
 #pragma unroll
@@ -164,7 +150,8 @@ __device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stag
         q.z = hz * (2.0f * u24(w.z) - 1.0f);
         const float us = u24(w.w);
         uint32_t s = 0u;
-        for (uint32_t c = 0; c + 1u < C; ++c) s += (us >= pr.cum_prob[c]) ? 1u : 0u;
+#pragma unroll
+        for (uint32_t c = 0; c + 1u < PCS_MAX_SPECIES; ++c) s += (c + 1u < C && us >= pr.cum_prob[c]) ? 1u : 0u;
         float sg = pr.sigma[0], az = pr.half_az[0];
 #pragma unroll
         for (uint32_t c = 1; c < PCS_MAX_SPECIES; ++c) {
@@ -269,6 +256,26 @@ __device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stag
     stage_flush(st);
 }
 
+// Tile t of a batch plan: the video owning it is the largest v with
+// vtile[v] <= t (wave-uniform binary search).
+template <int PPL, bool GAUSS, bool ZREF, bool TRACKS>
+__device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stage& st, uint32_t s0, uint32_t s1)
+{
+    uint32_t lo = 0u, hi = a.n_videos;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.vtile[mid] <= t) lo = mid; else hi = mid;
+    }
+    const uint32_t v = lo;
+    const uint32_t P = a.vprop[v];
+    const DevProp& pr = a.props[P];
+    st.rec = pr.rec;
+    st.slice_cap = pr.rec_cap;
+    st.counter = a.rec_count + P;
+    walk_tile<PPL, GAUSS, ZREF, TRACKS>(pr, a.sys[pr.sys_id], a.vframes[v], a.vparticles[v], v - pr.first_video,
+                                        (t - a.vtile[v]) * (uint32_t)(kWave * PPL), st, s0, s1);
+}
+
 template <int PPL, bool PERSIST, bool GAUSS, bool ZREF, bool TRACKS = false>
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
 {
@@ -322,38 +329,45 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, 
     return base + x - v;
 }
 
-template <int SEG>
-__global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
-{
-    __shared__ uint32_t rowc[PCS_MAX_NK];
-    __shared__ uint32_t colh[PCS_MAX_NDE];
-    __shared__ uint32_t wsum[kDistThreads / kWave];
-    __shared__ uint32_t smax;
-    __shared__ uint64_t red[kDistThreads];
+// LDS working set of the distance of one simulation: row counts / row
+// offsets (n_k), one row's column histogram (n_de), the block-scan partials,
+// the deepest occupied row and the final reduction.
+struct DistLds {
+    uint32_t* rowc;
+    uint32_t* colh;
+    uint32_t* wsum;
+    uint32_t* smax;
+    uint64_t* red;
+};
 
-    const uint32_t P = a.prop_lo + blockIdx.x;
-    const DevProp& pr = a.props[P];
-    const DevSys& sy = a.sys[pr.sys_id];
+// Result of distance_block, valid in thread 0.
+struct DistOut {
+    uint64_t exact;
+    uint32_t tracks;
+    int32_t status;
+};
+
+// Exact distance of one simulation's record slice against the observed
+// summed-area table, by a whole 256-thread block. SEG * 256 >= n_de.
+template <int SEG>
+__device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSys& sy, uint32_t count, const DistLds& L)
+{
+    uint32_t* rowc = L.rowc;
+    uint32_t* colh = L.colh;
+    uint32_t* wsum = L.wsum;
+    uint64_t* red = L.red;
     const uint32_t tid = threadIdx.x;
     const uint32_t n_k = sy.n_k, n_de = sy.n_de;
-    const uint32_t count = a.rec_count[P];
     const uint32_t* rec = pr.rec;
     uint32_t* srt = pr.srt;
 
-    if (count > pr.rec_cap) {  // arena slice overflowed: the host reruns it
-        if (tid == 0) {
-            a.dist[P] = __builtin_nan("");
-            a.exact[P] = 0ull;
-            a.tracks[P] = count;
-            a.status[P] = PCS_EOVERFLOW;
-        }
-        return;
-    }
+    if (count > pr.rec_cap)  // record slice overflowed: counted, not stored
+        return DistOut{0ull, count, PCS_EOVERFLOW};
     const uint32_t T = count;
 
     for (uint32_t i = tid; i < n_k; i += kDistThreads) rowc[i] = 0u;
     for (uint32_t i = tid; i < n_de; i += kDistThreads) colh[i] = 0u;
-    if (tid == 0) smax = 0u;
+    if (tid == 0) *L.smax = 0u;
     __syncthreads();
 
     // Row histogram of the records and the deepest occupied row.
@@ -363,7 +377,7 @@ __global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
         atomicAdd(&rowc[r], 1u);
         mr = max(mr, r);
     }
-    atomicMax(&smax, mr);
+    atomicMax(L.smax, mr);
     __syncthreads();
 
     // Exclusive scan of the row counts: rowc[r] becomes the first slot of row r.
@@ -391,7 +405,7 @@ __global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
     __threadfence_block();
     __syncthreads();
 
-    const uint32_t last = max(smax, sy.maxrow_obs);
+    const uint32_t last = max(*L.smax, sy.maxrow_obs);
     const uint32_t j0 = tid * SEG;
     uint32_t Ccol[SEG];
 #pragma unroll
@@ -442,12 +456,298 @@ __global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
         if (tid < s) red[tid] += red[tid + s];
         __syncthreads();
     }
-    if (tid == 0) {
-        const uint64_t tot = red[0];
-        a.exact[P] = tot;
-        a.dist[P] = log10((double)tot);
-        a.tracks[P] = T;
-        a.status[P] = PCS_OK;
+    return DistOut{red[0], T, PCS_OK};
+}
+
+template <int SEG>
+__global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
+{
+    __shared__ uint32_t rowc[PCS_MAX_NK];
+    __shared__ uint32_t colh[PCS_MAX_NDE];
+    __shared__ uint32_t wsum[kDistThreads / kWave];
+    __shared__ uint32_t smax;
+    __shared__ uint64_t red[kDistThreads];
+
+    const uint32_t P = a.prop_lo + blockIdx.x;
+    const DevProp& pr = a.props[P];
+    const DistOut o = distance_block<SEG>(pr, a.sys[pr.sys_id], a.rec_count[P], DistLds{rowc, colh, wsum, &smax, red});
+    if (threadIdx.x == 0) {  // an overflowed slice is rerun by the host
+        a.exact[P] = o.exact;
+        a.dist[P] = o.status == PCS_OK ? log10((double)o.exact) : __builtin_nan("");
+        a.tracks[P] = o.tracks;
+        a.status[P] = o.status;
+    }
+}
+
+// ------------------------------------------------------------ request loop
+//
+// serve_loop_kernel is the device-resident form of the serving loop: a
+// persistent grid of 4-wave groups takes requests straight from a ring in
+// pinned host memory. Wave 0 of a group decides (admit the next published
+// request, claim up to 4 tiles from the cursor, or idle); each wave walks one
+// claimed tile; the group that completes a request's last tile computes its
+// distance with the block algorithm above and writes the result to host
+// memory. Every wait is bounded: a group leaves on the host's stop flag, on
+// another group's quit flag, or after idle_ticks without work (setting quit),
+// and no group ever waits for another one to make progress.
+
+enum : uint32_t { kLoopIdle = 0u, kLoopWork = 1u, kLoopExit = 2u };
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t v)
+{
+    return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | (uint64_t)uniform((uint32_t)v);
+}
+
+__device__ __forceinline__ uint64_t ld_agent64(const uint64_t* p)
+{
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t ld_agent32(const uint32_t* p)
+{
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t ld_system64(const uint64_t* p)
+{
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ uint32_t ld_system32(const uint32_t* p)
+{
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ bool cas_agent64(uint64_t* p, uint64_t expect, uint64_t want)
+{
+    return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-uniform copy of *p: vector loads (slot memory is written inside the
+// kernel, so it may not go through the scalar cache) broadcast from lane 0.
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T* p)
+{
+    static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+    uint32_t w[sizeof(T) / 4];
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 4; ++i) w[i] = uniform(src[i]);
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
+// Wave-wide: true once request q owns its slot with its tiles open (or
+// finished). A poller group also admits q: if the host has published q and
+// the slot is free, it copies q in (exactly one wave wins the slot) and opens
+// it. Other groups only watch the slot state, so host memory is read by at
+// most kLoopPollers groups.
+__device__ bool loop_admit(const LoopArgs& a, uint64_t q, bool poller)
+{
+    const uint32_t slot = (uint32_t)(q % a.R);
+    LoopSlot& S = a.slots[slot];
+    const bool l0 = lane_id() == 0u;
+    const uint64_t st = bcast64(l0 ? ld_agent64(&S.state) : 0ull);
+    if (st == 4ull * q + 2ull || st == 4ull * q + 3ull) return true;
+    if (!poller) return false;
+    const uint64_t expect = q < a.R ? 0ull : 4ull * (q - a.R) + 3ull;
+    if (st != expect) return false;
+    const LoopReq& Q = a.ring[slot];
+    const uint64_t seq = bcast64(l0 ? ld_system64(&Q.seq) : 0ull);
+    if (seq != q + 1ull) return false;
+    uint32_t won = 0u;
+    if (l0) won = cas_agent64(&S.state, expect, 4ull * q + 1ull) ? 1u : 0u;
+    if (uniform(won) == 0u) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the ring entry was written before its seq
+    // One round of host reads: the proposal (36 words) and n_videos, tiles.
+    constexpr uint32_t kPropWords = sizeof(DevProp) / 4u;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&Q.prop);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&S.prop);
+    const uint32_t lane = lane_id();
+    uint32_t w = 0u;
+    if (lane < kPropWords) w = src[lane];
+    else if (lane == kPropWords) w = Q.n_videos;
+    else if (lane == kPropWords + 1u) w = Q.tiles;
+    if (lane < kPropWords) dst[lane] = w;
+    const uint32_t nv = min(__shfl(w, (int)kPropWords, kWave), a.maxv);
+    const uint32_t tiles = __shfl(w, (int)kPropWords + 1, kWave);
+    const uint32_t vw = 3u * (nv + 1u);
+    const uint32_t* vs = a.ring_vid + (uint64_t)slot * 3u * (a.maxv + 1u);
+    uint32_t* vd = a.slot_vid + (uint64_t)slot * 3u * (a.maxv + 1u);
+    for (uint32_t i = lane; i < vw; i += kWave) vd[i] = vs[i];
+    if (l0) {
+        S.tiles = tiles;
+        S.n_videos = nv;
+        S.tiles_done = 0u;
+        S.rec_count = 0u;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (l0) __hip_atomic_store(&S.state, 4ull * q + 2ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Wave-wide decision of wave 0: kLoopWork with (r, t0, nt) claimed, kLoopIdle
+// (nothing claimable now) or kLoopExit (stop or quit).
+__device__ uint32_t loop_decide(const LoopArgs& a, bool poller, uint64_t& r_out, uint32_t& t0, uint32_t& nt)
+{
+    const bool l0 = lane_id() == 0u;
+    constexpr uint32_t kTileMask = (1u << kLoopTileBits) - 1u;
+    for (int attempt = 0; attempt < 16; ++attempt) {
+        uint32_t leave = 0u;
+        if (l0) {
+            leave = ld_agent32(&a.ctl->quit);
+            if (poller && leave == 0u && ld_system32(a.stop) != 0u) {
+                __hip_atomic_store(&a.ctl->quit, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                leave = 1u;
+            }
+        }
+        if (uniform(leave)) return kLoopExit;
+        const uint64_t c = bcast64(l0 ? ld_agent64(&a.ctl->cursor) : 0ull);
+        const uint64_t r = c >> kLoopTileBits;
+        const uint32_t t = (uint32_t)c & kTileMask;
+        if (!loop_admit(a, r, poller)) return kLoopIdle;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const uint32_t T = uniform(l0 ? ld_agent32(&a.slots[r % a.R].tiles) : 0u);
+        if (t < T) {
+            const uint32_t n = min((uint32_t)kLoopWaves, T - t);
+            uint32_t won = 0u;
+            if (l0) won = cas_agent64(&a.ctl->cursor, c, c + n) ? 1u : 0u;
+            if (uniform(won)) {
+                r_out = r;
+                t0 = t;
+                nt = n;
+                return kLoopWork;
+            }
+            continue;  // another group moved the cursor
+        }
+        // Every tile of r is claimed: move on once r + 1 is admitted.
+        if (!loop_admit(a, r + 1ull, poller)) return kLoopIdle;
+        if (l0) cas_agent64(&a.ctl->cursor, c, (r + 1ull) << kLoopTileBits);
+    }
+    return kLoopIdle;
+}
+
+template <int SEG>
+__global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per_eu(6, 6))) void serve_loop_kernel(LoopArgs a)
+{
+    extern __shared__ uint32_t loop_lds[];
+    __shared__ uint32_t wsum[kDistThreads / kWave];
+    __shared__ uint32_t smax;
+    __shared__ uint64_t red[kDistThreads];
+    __shared__ uint64_t cmd_r;
+    __shared__ uint32_t cmd[5];  // kind, t0, nt, last, sleep units
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid / kWave;
+    const bool poller = blockIdx.x < kLoopPollers;
+    Stage st;
+    st.buf = loop_lds + wave * a.stage_cap;
+    st.cap = a.stage_cap;
+    st.cnt = 0u;
+    uint32_t* rowc = loop_lds + kLoopWaves * a.stage_cap;
+    const DistLds L{rowc, rowc + a.nk_lds, wsum, &smax, red};
+    const uint32_t vw = 3u * (a.maxv + 1u);
+    uint64_t last_work = wall_clock64();
+    uint32_t idle_run = 0u;
+
+    for (;;) {
+        if (wave == 0u) {
+            uint64_t r = 0ull;
+            uint32_t t0 = 0u, nt = 0u;
+            uint32_t kind = loop_decide(a, poller, r, t0, nt);
+            const uint64_t now = wall_clock64();
+            if (kind == kLoopWork) {
+                last_work = now;
+                idle_run = 0u;
+                if (lane_id() == 0u)
+                    __hip_atomic_store(&a.ctl->last_claim, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (kind == kLoopIdle) {
+                idle_run = min(idle_run + 1u, 4u);
+                if (now - last_work > a.idle_ticks) {
+                    // idle here; has any group claimed work lately?
+                    const uint64_t g = bcast64(lane_id() == 0u ? ld_agent64(&a.ctl->last_claim) : 0ull);
+                    if (g > last_work) last_work = g;
+                    if (now - last_work > a.idle_ticks) {
+                        if (lane_id() == 0u)
+                            __hip_atomic_store(&a.ctl->quit, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        kind = kLoopExit;
+                    }
+                }
+            }
+            if (lane_id() == 0u) {
+                cmd[0] = kind;
+                cmd[1] = t0;
+                cmd[2] = nt;
+                // pollers wake every ~3 us; the others back off to ~27 us
+                cmd[4] = (poller || idle_run == 0u) ? 1u : (1u << (idle_run - 1u));
+                cmd_r = r;
+            }
+        }
+        __syncthreads();
+        const uint32_t kind = uniform(cmd[0]);
+        if (kind == kLoopExit) break;
+        if (kind == kLoopIdle) {
+            const uint32_t units = uniform(cmd[4]);
+            for (uint32_t i = 0; i < units; ++i) __builtin_amdgcn_s_sleep(127);
+            __syncthreads();
+            continue;
+        }
+        const uint64_t r = bcast64(cmd_r);
+        const uint32_t t0 = uniform(cmd[1]), nt = uniform(cmd[2]);
+        const uint32_t slot = (uint32_t)(r % a.R);
+        LoopSlot& S = a.slots[slot];
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // slot contents of the admitting group
+        if (wave < nt) {
+            const uint32_t t = t0 + wave;
+            const uint32_t* vid = a.slot_vid + (uint64_t)slot * vw;
+            uint32_t lo = 0u, hi = uniform(S.n_videos);
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (uniform(vid[3u * mid + 2u]) <= t) lo = mid; else hi = mid;
+            }
+            const DevProp pr = load_uniform(&S.prop);
+            const DevSys sy = load_uniform(&a.sys[pr.sys_id]);
+            st.rec = pr.rec;
+            st.slice_cap = pr.rec_cap;
+            st.counter = &S.rec_count;
+            const uint32_t F = uniform(vid[3u * lo]), N = uniform(vid[3u * lo + 1u]);
+            const uint32_t p0 = (t - uniform(vid[3u * lo + 2u])) * (uint32_t)kWave;
+            const bool gauss = sy.detect_model == PCS_DETECT_GAUSS_Z;
+            const bool zref = sy.z_boundary == PCS_Z_REFLECT;
+            if (gauss) {
+                if (zref) walk_tile<1, true, true, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
+                else walk_tile<1, true, false, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
+            } else {
+                if (zref) walk_tile<1, false, true, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
+                else walk_tile<1, false, false, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this wave's records
+        }
+        __syncthreads();
+        if (tid == 0u) {
+            const uint32_t prev = __hip_atomic_fetch_add(&S.tiles_done, nt, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            cmd[3] = (prev + nt == S.tiles) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (uniform(cmd[3]) != 0u) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // records of every group
+            const DevProp pr = load_uniform(&S.prop);
+            const DevSys sy = load_uniform(&a.sys[pr.sys_id]);
+            const uint32_t count = uniform(ld_agent32(&S.rec_count));
+            const DistOut o = distance_block<SEG>(pr, sy, count, L);
+            if (tid == 0u) {
+                LoopDone& d = a.done[slot];
+                d.exact = o.exact;
+                d.dist = o.status == PCS_OK ? log10((double)o.exact) : __builtin_nan("");
+                d.tracks = o.tracks;
+                d.status = o.status;
+                __hip_atomic_store(&S.state, 4ull * r + 3ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&d.seq, r + 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -533,6 +833,7 @@ __global__ __launch_bounds__(256) void pmc_sums_kernel(const double* xn, uint32_
 hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, const double* w_old, uint32_t n_old,
                            uint32_t P, double* wsum, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launch's error
     if (n_new == 0) return hipSuccess;
     hipLaunchKernelGGL(pmc_sums_kernel, dim3((n_new + 255u) / 256u), dim3(256), 0, s, xn, n_new, xo, w_old, n_old, P,
                        wsum);
@@ -541,6 +842,7 @@ hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, c
 
 hipError_t launch_selftest_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st)
 {
+    (void)hipGetLastError();  // report only this launch's error
     const uint64_t n = bytes / 16u;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(selftest_copy_kernel, dim3(4096), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n);
@@ -549,6 +851,7 @@ hipError_t launch_selftest_copy(const void* src, void* dst, uint64_t bytes, hipS
 
 hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, float* c, uint64_t n, hipStream_t st)
 {
+    (void)hipGetLastError();  // report only this launch's error
     if (n == 0) return hipSuccess;
     const uint64_t nb = (n + 255u) / 256u;
     hipLaunchKernelGGL(selftest_math_kernel, dim3((uint32_t)nb), dim3(256), 0, st, w, radius, s, c, n);
@@ -557,6 +860,7 @@ hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, floa
 
 hipError_t launch_reset(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launch's error
     const uint32_t nb = (n + 255u) / 256u;
     hipLaunchKernelGGL(reset_kernel, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, rec_count, n, tile_counter);
     return hipGetLastError();
@@ -565,6 +869,7 @@ hipError_t launch_reset(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter,
 template <int PPL, bool PERSIST, bool GAUSS, bool ZREF>
 static hipError_t launch_walk_t(const WalkArgs& a, int wpg, uint32_t grid, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launch's error
     const size_t shmem = (size_t)wpg * a.stage_cap * sizeof(uint32_t);
     hipLaunchKernelGGL((walk_kernel<PPL, PERSIST, GAUSS, ZREF>), dim3(grid), dim3(wpg * kWave), shmem, s, a);
     return hipGetLastError();
@@ -597,6 +902,7 @@ hipError_t launch_walk(const WalkArgs& a, int ppl, int wpg, bool persistent, uin
 
 hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launch's error
     if (a.tile_hi <= a.tile_lo) return hipSuccess;
     const uint32_t grid = (a.tile_hi - a.tile_lo + 3u) / 4u;
     const size_t shmem = (size_t)4 * a.stage_cap * sizeof(uint32_t);
@@ -613,6 +919,7 @@ hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s
 
 hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launch's error
     if (a.n_props == 0) return hipSuccess;
     const dim3 g(a.n_props), b(kDistThreads);
     if (n_de_max <= 256) hipLaunchKernelGGL(distance_kernel<1>, g, b, 0, s, a);
@@ -623,9 +930,45 @@ hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s)
     return hipGetLastError();
 }
 
+size_t serve_loop_lds(const LoopArgs& a)
+{
+    return ((size_t)kLoopWaves * a.stage_cap + a.nk_lds + a.nde_lds) * sizeof(uint32_t);
+}
+
+template <int SEG>
+static hipError_t loop_occ_t(const LoopArgs& a, int* n)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, serve_loop_kernel<SEG>, kLoopWaves * kWave,
+                                                        serve_loop_lds(a));
+}
+
+hipError_t serve_loop_occupancy(const LoopArgs& a, uint32_t n_de_max, int* groups_per_cu)
+{
+    if (n_de_max <= 256) return loop_occ_t<1>(a, groups_per_cu);
+    if (n_de_max <= 512) return loop_occ_t<2>(a, groups_per_cu);
+    if (n_de_max <= 1024) return loop_occ_t<4>(a, groups_per_cu);
+    if (n_de_max <= 2048) return loop_occ_t<8>(a, groups_per_cu);
+    return loop_occ_t<16>(a, groups_per_cu);
+}
+
+hipError_t launch_serve_loop(const LoopArgs& a, uint32_t grid, uint32_t n_de_max, hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launch's error
+    if (grid == 0 || n_de_max > a.nde_lds) return hipErrorInvalidValue;
+    const dim3 g(grid), b(kLoopWaves * kWave);
+    const size_t sh = serve_loop_lds(a);
+    if (n_de_max <= 256) hipLaunchKernelGGL(serve_loop_kernel<1>, g, b, sh, s, a);
+    else if (n_de_max <= 512) hipLaunchKernelGGL(serve_loop_kernel<2>, g, b, sh, s, a);
+    else if (n_de_max <= 1024) hipLaunchKernelGGL(serve_loop_kernel<4>, g, b, sh, s, a);
+    else if (n_de_max <= 2048) hipLaunchKernelGGL(serve_loop_kernel<8>, g, b, sh, s, a);
+    else hipLaunchKernelGGL(serve_loop_kernel<16>, g, b, sh, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_dense_hist(const DevProp* props, uint32_t prop_lo, uint32_t n_props, const uint32_t* rec_count, uint32_t max_count, uint32_t n_k, uint32_t n_de,
                              uint32_t* hist, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launch's error
     if (n_props == 0) return hipSuccess;
     uint32_t gx = (max_count + 255u) / 256u;
     gx = gx < 1u ? 1u : (gx > 1024u ? 1024u : gx);

@@ -16,6 +16,11 @@
 //   - the distance kernel writes each result straight into pinned host
 //     memory; the scheduler retires groups whose completion event fired,
 //     frees their arena ranges and hands completions to poll().
+// With PCS_SERVE_DEVICE_LOOP the launch groups are replaced by one persistent
+// kernel (serve_loop_kernel) that takes requests from a ring in pinned host
+// memory; the scheduler thread then only admits requests into the ring,
+// collects results from the completion ring and relaunches the kernel when it
+// has left for lack of work.
 // The random stream is counter based, keyed per request, so a request's
 // result is the same whatever it shares the GPU with.
 #include <algorithm>
@@ -112,6 +117,35 @@ struct Group {
     uint32_t n = 0;
 };
 
+// Host side of the device-resident loop.
+struct LoopHost {
+    bool on = false;
+    uint32_t R = 0, maxv = 0;
+    LoopReq* h_ring = nullptr;      // pinned, coherent: requests (host writes)
+    uint32_t* h_ring_vid = nullptr; // pinned, coherent: video arrays per slot
+    LoopDone* h_done = nullptr;     // pinned, coherent: results (device writes)
+    uint32_t* h_stop = nullptr;     // pinned, coherent: 1 = leave now
+    LoopSlot* d_slots = nullptr;
+    uint32_t* d_slot_vid = nullptr;
+    LoopCtl* d_ctl = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    bool running = false, stopping = false;
+    uint32_t run_nsys = 0;          // systems known to the running launch
+    uint64_t next_r = 0;            // number of the next request to publish
+    uint64_t idle_ticks = 0;
+    uint64_t launches = 0;
+    int n_cu = 0;
+    struct Book {
+        bool busy = false;
+        uint64_t r = 0, tag = 0, off = 0;
+        uint32_t n = 0;
+        double submit_us = 0.0;
+    };
+    std::vector<Book> book;         // per slot
+    std::vector<uint32_t> live;     // slots in flight
+};
+
 }  // namespace
 
 struct pcs_server {
@@ -125,6 +159,7 @@ struct pcs_server {
     Arena arena;
     Group groups[kStreams];
     int next_group = 0;
+    LoopHost loop;
     uint32_t inflight = 0;
     uint32_t max_queue = 0;
 
@@ -187,6 +222,29 @@ void group_free(Group& g)
     g = Group{};
 }
 
+// Device record of an admitted request: its videos start at first_video of
+// the launch, its records at arena offset off; Philox key = request seed.
+DevProp make_prop(const pcs_server* s, const Pending& p, uint32_t first_video, uint64_t off)
+{
+    DevProp d;
+    std::memset(&d, 0, sizeof(d));
+    d.sim_id = p.req.prop.sim_id;
+    d.first_video = first_video;
+    d.n_videos = p.req.n_videos;
+    d.sys_id = p.req.sys_id;
+    d.rec = s->d_rec + off;
+    d.srt = s->d_srt + off;
+    d.rec_cap = p.cap;
+    d.key0 = (uint32_t)p.req.seed;
+    d.key1 = (uint32_t)(p.req.seed >> 32);
+    for (int c = 0; c < PCS_MAX_SPECIES; ++c) {
+        d.half_az[c] = p.req.prop.half_az[c];
+        d.cum_prob[c] = p.req.prop.cum_prob[c];
+        d.sigma[c] = p.req.prop.sigma[c];
+    }
+    return d;
+}
+
 // Issue one launch group on g from the admitted requests.
 int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vector<uint64_t>& offs)
 {
@@ -201,22 +259,7 @@ int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vect
     for (uint32_t i = 0; i < n; ++i) {
         const Pending& p = batch[i];
         const pcs_system& sy = s->systems[p.req.sys_id].sys;
-        DevProp& d = g.h_props[i];
-        std::memset(&d, 0, sizeof(d));
-        d.sim_id = p.req.prop.sim_id;
-        d.first_video = nv;
-        d.n_videos = p.req.n_videos;
-        d.sys_id = p.req.sys_id;
-        d.rec = s->d_rec + offs[i];
-        d.srt = s->d_srt + offs[i];
-        d.rec_cap = p.cap;
-        d.key0 = (uint32_t)p.req.seed;
-        d.key1 = (uint32_t)(p.req.seed >> 32);
-        for (int c = 0; c < PCS_MAX_SPECIES; ++c) {
-            d.half_az[c] = p.req.prop.half_az[c];
-            d.cum_prob[c] = p.req.prop.cum_prob[c];
-            d.sigma[c] = p.req.prop.sigma[c];
-        }
+        g.h_props[i] = make_prop(s, p, nv, offs[i]);
         for (uint32_t v = 0; v < p.req.n_videos; ++v, ++nv) {
             g.h_vframes[nv] = p.frames[v];
             g.h_vparticles[nv] = p.particles[v];
@@ -384,6 +427,244 @@ void scheduler(pcs_server* s)
     }
 }
 
+// ------------------------------------------------------ device-resident loop
+
+uint32_t request_tiles(const Pending& p)
+{
+    uint64_t t = 0;
+    for (uint32_t v = 0; v < p.req.n_videos; ++v) t += (p.particles[v] + 63u) / 64u;
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(t, 1), 0xffffffffu);
+}
+
+// Write request r into its ring slot and publish it (seq last).
+void loop_publish(pcs_server* s, uint32_t slot, uint64_t r, const Pending& p, uint64_t off)
+{
+    LoopHost& L = s->loop;
+    LoopReq& Q = L.h_ring[slot];
+    uint32_t* vid = L.h_ring_vid + (uint64_t)slot * 3u * (L.maxv + 1u);
+    uint32_t acc = 0;
+    for (uint32_t v = 0; v < p.req.n_videos; ++v) {
+        vid[3 * v] = p.frames[v];
+        vid[3 * v + 1] = p.particles[v];
+        vid[3 * v + 2] = acc;
+        acc += (p.particles[v] + 63u) / 64u;
+    }
+    vid[3 * p.req.n_videos] = 0;
+    vid[3 * p.req.n_videos + 1] = 0;
+    vid[3 * p.req.n_videos + 2] = acc;
+    Q.n_videos = p.req.n_videos;
+    Q.tiles = std::max(acc, 1u);
+    Q.prop = make_prop(s, p, 0, off);
+    __atomic_store_n(&Q.seq, r + 1, __ATOMIC_RELEASE);
+}
+
+// Move finished requests to the completion queue; returns how many.
+uint32_t loop_collect(pcs_server* s, std::vector<pcs_completion>& out)
+{
+    LoopHost& L = s->loop;
+    out.clear();
+    for (size_t i = 0; i < L.live.size();) {
+        const uint32_t slot = L.live[i];
+        LoopHost::Book& b = L.book[slot];
+        const LoopDone& d = L.h_done[slot];
+        if (__atomic_load_n(&d.seq, __ATOMIC_ACQUIRE) != b.r + 1) {
+            ++i;
+            continue;
+        }
+        pcs_completion c{};
+        c.tag = b.tag;
+        c.dist = d.dist;
+        c.tracks = d.tracks;
+        c.status = d.status;
+        c.submit_us = b.submit_us;
+        c.done_us = s->now_us();
+        out.push_back(c);
+        s->arena.release(b.off, b.n);
+        b.busy = false;
+        L.live[i] = L.live.back();
+        L.live.pop_back();
+    }
+    if (!out.empty()) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (auto& c : out) s->done.push_back(c);
+        s->cv.notify_all();
+    }
+    return (uint32_t)out.size();
+}
+
+// Launch the loop kernel for the first nsys systems (LDS and distance width
+// sized for the largest of them).
+int loop_launch(pcs_server* s, uint32_t nsys)
+{
+    LoopHost& L = s->loop;
+    uint32_t nk = 1, nde = 1;
+    for (uint32_t i = 0; i < nsys; ++i) {
+        nk = std::max(nk, s->systems[i].sys.n_k);
+        nde = std::max(nde, s->systems[i].sys.n_de);
+    }
+    LoopArgs a{};
+    a.sys = s->d_sys;
+    a.slots = L.d_slots;
+    a.slot_vid = L.d_slot_vid;
+    a.ring = L.h_ring;
+    a.ring_vid = L.h_ring_vid;
+    a.done = L.h_done;
+    a.stop = L.h_stop;
+    a.ctl = L.d_ctl;
+    a.R = L.R;
+    a.maxv = L.maxv;
+    a.stage_cap = 256;
+    a.nk_lds = nk;
+    a.nde_lds = nde;
+    a.idle_ticks = L.idle_ticks;
+    int gpc = 0;
+    hipError_t e = serve_loop_occupancy(a, nde, &gpc);
+    if (e != hipSuccess) return hip_error(e, "serving loop occupancy");
+    if (gpc < 1) return set_error(PCS_EINVAL, "serving loop does not fit a CU (n_k %u, n_de %u)", nk, nde);
+    const uint32_t grid = (uint32_t)L.n_cu * (uint32_t)std::min(gpc, 8);
+    e = hipMemsetAsync(&L.d_ctl->quit, 0, sizeof(uint32_t), L.stream);
+    if (e == hipSuccess) e = launch_serve_loop(a, grid, nde, L.stream);
+    if (e == hipSuccess) e = hipEventRecord(L.ev, L.stream);
+    if (e != hipSuccess) return hip_error(e, "serving loop launch");
+    L.running = true;
+    L.run_nsys = nsys;
+    ++L.launches;
+    return PCS_OK;
+}
+
+void fail(pcs_server* s, int r)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->fatal = r;
+    s->stop = true;
+    s->cv.notify_all();
+}
+
+void scheduler_loop(pcs_server* s)
+{
+    hipSetDevice(s->device);
+    LoopHost& L = s->loop;
+    std::vector<pcs_completion> out;
+    for (;;) {
+        bool progress = loop_collect(s, out) > 0;
+        // Has the kernel left (idle limit, or the stop flag of a restart)?
+        if (L.running) {
+            const hipError_t q = hipEventQuery(L.ev);
+            if (q == hipSuccess) {
+                L.running = false;
+            } else if (q != hipErrorNotReady) {
+                fail(s, hip_error(q, "serving loop"));
+                return;
+            }
+        }
+        if (!L.running && L.stopping) {
+            __atomic_store_n(L.h_stop, 0u, __ATOMIC_RELEASE);
+            L.stopping = false;
+        }
+        // Admit queued requests into free ring slots while the arena has room.
+        bool restart = false;
+        uint32_t nsys = 0;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->stop) return;
+            nsys = (uint32_t)s->systems.size();
+            while (!s->queue.empty() && L.live.size() < L.R) {
+                Pending& p = s->queue.front();
+                if (L.running && p.req.sys_id >= L.run_nsys) {
+                    restart = true;  // the running launch does not know this model
+                    break;
+                }
+                const uint32_t slot = (uint32_t)(L.next_r % L.R);
+                if (L.book[slot].busy) break;  // request next_r - R still running
+                uint64_t off;
+                if (!s->arena.alloc(p.cap, off)) break;  // wait for memory to free up
+                LoopHost::Book& b = L.book[slot];
+                b.busy = true;
+                b.r = L.next_r;
+                b.tag = p.req.tag;
+                b.off = off;
+                b.n = p.cap;
+                b.submit_us = p.submit_us;
+                loop_publish(s, slot, L.next_r, p, off);
+                L.live.push_back(slot);
+                ++L.next_r;
+                s->queue.pop_front();
+                progress = true;
+            }
+        }
+        if (restart && L.running && !L.stopping) {
+            __atomic_store_n(L.h_stop, 1u, __ATOMIC_RELEASE);
+            L.stopping = true;
+        }
+        if (!L.running && !L.stopping && !L.live.empty()) {
+            const int r = loop_launch(s, nsys);
+            if (r != PCS_OK) {
+                fail(s, r);
+                return;
+            }
+            progress = true;
+        }
+        if (!progress) {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->stop) return;
+            if (L.live.empty() && s->queue.empty())
+                s->cv.wait_for(lk, std::chrono::milliseconds(2));
+            else
+                lk.unlock(), std::this_thread::yield();
+        }
+    }
+}
+
+hipError_t loop_alloc(pcs_server* s)
+{
+    LoopHost& L = s->loop;
+    L.on = true;
+    L.R = s->cfg.max_inflight;
+    L.maxv = s->cfg.max_videos;
+    L.book.assign(L.R, LoopHost::Book{});
+    L.live.reserve(L.R);
+    const size_t vid = (size_t)L.R * 3u * (L.maxv + 1u);
+    hipError_t e = hipHostMalloc((void**)&L.h_ring, L.R * sizeof(LoopReq), hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&L.h_ring_vid, vid * 4, hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&L.h_done, L.R * sizeof(LoopDone), hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&L.h_stop, 64, hipHostMallocCoherent);
+    if (e == hipSuccess) e = dmalloc(&L.d_slots, L.R);
+    if (e == hipSuccess) e = dmalloc(&L.d_slot_vid, vid);
+    if (e == hipSuccess) e = dmalloc(&L.d_ctl, 1);
+    if (e == hipSuccess) e = hipMemset(L.d_slots, 0, L.R * sizeof(LoopSlot));
+    if (e == hipSuccess) e = hipMemset(L.d_ctl, 0, sizeof(LoopCtl));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&L.n_cu, hipDeviceAttributeMultiprocessorCount, s->device);
+    int khz = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device);
+    if (e != hipSuccess) return e;
+    std::memset(L.h_ring, 0, L.R * sizeof(LoopReq));
+    std::memset(L.h_ring_vid, 0, vid * 4);
+    std::memset(L.h_done, 0, L.R * sizeof(LoopDone));
+    std::memset(L.h_stop, 0, 64);
+    const uint64_t us = s->cfg.loop_idle_us ? s->cfg.loop_idle_us : 20000u;
+    L.idle_ticks = us * (uint64_t)(khz > 0 ? khz : 100000) / 1000u;
+    return hipSuccess;
+}
+
+// Stop the kernel (it leaves after its current tiles) and free the loop.
+void loop_free(pcs_server* s)
+{
+    LoopHost& L = s->loop;
+    if (L.h_stop) __atomic_store_n(L.h_stop, 1u, __ATOMIC_RELEASE);
+    if (L.stream) hipStreamSynchronize(L.stream);
+    void* dev[] = {L.d_slots, L.d_slot_vid, L.d_ctl};
+    for (void* p : dev)
+        if (p) hipFree(p);
+    void* host[] = {L.h_ring, L.h_ring_vid, L.h_done, L.h_stop};
+    for (void* p : host)
+        if (p) hipHostFree(p);
+    if (L.ev) hipEventDestroy(L.ev);
+    if (L.stream) hipStreamDestroy(L.stream);
+    L = LoopHost{};
+}
+
 }  // namespace
 
 extern "C" {
@@ -402,6 +683,14 @@ int pcs_server_create(int device, const pcs_server_config* cfg, pcs_server** out
     if (!s->cfg.arena_bytes) s->cfg.arena_bytes = 1ull << 30;
     if (!s->cfg.max_inflight) s->cfg.max_inflight = 4096;
     if (!s->cfg.max_videos) s->cfg.max_videos = 64;
+    if (s->cfg.flags & ~PCS_SERVE_DEVICE_LOOP) {
+        delete s;
+        return set_error(PCS_EINVAL, "unknown server flags 0x%x", s->cfg.flags);
+    }
+    if (s->cfg.max_videos > 4096) {
+        delete s;
+        return set_error(PCS_EINVAL, "max_videos %u above 4096", s->cfg.max_videos);
+    }
     s->max_queue = 4 * s->cfg.max_inflight;
     s->systems.reserve(kMaxSystems);  // the scheduler reads entries without the lock
     s->arena_records = s->cfg.arena_bytes / 8;  // records + sort scratch
@@ -412,15 +701,20 @@ int pcs_server_create(int device, const pcs_server_config* cfg, pcs_server** out
     if (e == hipSuccess) e = hipMemset(s->d_zero_seed, 0, 8);
     if (e == hipSuccess) e = dmalloc(&s->d_rec, s->arena_records);
     if (e == hipSuccess) e = dmalloc(&s->d_srt, s->arena_records);
-    for (int k = 0; k < kStreams && e == hipSuccess; ++k)
-        e = group_alloc(s->groups[k], s->cfg.max_inflight, s->cfg.max_inflight * s->cfg.max_videos);
+    const bool device_loop = (s->cfg.flags & PCS_SERVE_DEVICE_LOOP) != 0;
+    if (device_loop) {
+        if (e == hipSuccess) e = loop_alloc(s);
+    } else {
+        for (int k = 0; k < kStreams && e == hipSuccess; ++k)
+            e = group_alloc(s->groups[k], s->cfg.max_inflight, s->cfg.max_inflight * s->cfg.max_videos);
+    }
     if (e != hipSuccess) {
         const int r = hip_error(e, "pcs_server_create");
         pcs_server_destroy(s);
         return r;
     }
     s->arena.init(s->arena_records);
-    s->worker = std::thread(scheduler, s);
+    s->worker = std::thread(device_loop ? scheduler_loop : scheduler, s);
     *out = s;
     return PCS_OK;
 }
@@ -473,6 +767,9 @@ int pcs_server_submit(pcs_server* s, const pcs_request* req)
     int r = validate_proposal(sy, req->prop);
     if (r != PCS_OK) return r;
     p.cap = record_capacity(sy, req->prop, p.frames.data(), p.particles.data(), 1.0);
+    if (s->loop.on && request_tiles(p) > kLoopMaxTiles)
+        return set_error(PCS_EINVAL, "request needs %u tiles of 64 particles, the loop takes %u", request_tiles(p),
+                         kLoopMaxTiles);
     if (p.cap > s->arena_records)
         return set_error(PCS_EINVAL, "request needs %u records, arena holds %llu", p.cap,
                          (unsigned long long)s->arena_records);
@@ -519,6 +816,7 @@ int pcs_server_destroy(pcs_server* s)
     }
     if (s->worker.joinable()) s->worker.join();
     hipSetDevice(s->device);
+    loop_free(s);
     for (Group& g : s->groups) group_free(g);
     for (SysEntry& e : s->systems)
         if (e.d_sat) hipFree(e.d_sat);

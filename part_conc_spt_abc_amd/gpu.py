@@ -173,11 +173,15 @@ def selftest_math(words: np.ndarray, device: int = 0):
 
 
 class Server:
-    """Continuous-batching serving loop (pcs_server_*)."""
+    """Continuous-batching serving loop (pcs_server_*). device_loop=True
+    selects the device-resident form (one persistent kernel fed from a ring in
+    pinned host memory) instead of host-driven launch groups."""
 
-    def __init__(self, device: int = 0, arena_bytes: int = 0, max_inflight: int = 0, max_videos: int = 0):
+    def __init__(self, device: int = 0, arena_bytes: int = 0, max_inflight: int = 0, max_videos: int = 0,
+                 device_loop: bool = False, loop_idle_us: int = 0):
         self.lib = _abi.load()
-        cfg = _abi.pcs_server_config(arena_bytes, max_inflight, max_videos, 0)
+        flags = _abi.PCS_SERVE_DEVICE_LOOP if device_loop else 0
+        cfg = _abi.pcs_server_config(arena_bytes, max_inflight, max_videos, flags, loop_idle_us)
         h = C.c_void_p()
         _abi.check(self.lib.pcs_server_create(device, C.byref(cfg), C.byref(h)), "pcs_server_create")
         self.h = h

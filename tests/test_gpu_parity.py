@@ -202,6 +202,28 @@ def test_invalid_arguments_fail_loudly(gpu_lib):
             b.load([p], f, n)
 
 
+def test_batch_outlives_its_context(gpu_lib):
+    # pcs_close releases the caller's handle only: a live batch keeps the
+    # context, runs after the close, and destroying it frees both.
+    model = cases.headline_model(n_k=300, n_de=200)
+    obs = observed(model, frames=[300])
+    rng = np.random.default_rng(13)
+    props, f, n = cases.draw_proposals(model, 6, rng, frames_choices=(300,), c_range=(1e8, 1e9))
+    ctx = Context(model, obs)
+    b = ctx.batch()
+    b.load(props, f, n)
+    ctx.close()
+    b.run(41, _abi.PCS_MODE_BATCHED)
+    dist, tracks, H = b.read(hist=True)
+    exact = b.read_exact()
+    b.close()
+    check_against_oracle(model, obs, props, f, n, 41, dist, tracks, H, exact)
+    # and later launches on this thread report no stale error
+    with Context(model, obs) as ctx2:
+        dist2, tracks2, _, _, _ = run_gpu(ctx2, props, f, n, 41, _abi.PCS_MODE_BATCHED)
+    assert np.array_equal(dist2, dist) and np.array_equal(tracks2, tracks)
+
+
 def test_autotune_picks_a_valid_shape_and_keeps_results(gpu_lib):
     model = cases.headline_model(n_k=600, n_de=300)
     obs = observed(model, frames=[600])

@@ -119,3 +119,140 @@ def test_admission_by_footprint_and_backpressure():
     with pytest.raises(_abi.PcsError):
         srv.submit(999, sids[0], big, np.array([2000], np.uint32), np.array([4_000_000], np.uint32), 1)
     srv.close()
+
+
+# ------------------------------------------------------------------ device loop
+
+def host_loop_results(ch, reqs):
+    srv = Server(device=0, arena_bytes=1 << 30, max_inflight=256)
+    sids = [srv.add_system(m, obs) for m, _, obs in ch]
+    for tag, (ci, p, f, n, seed) in enumerate(reqs):
+        assert srv.submit(tag, sids[ci], p, f, n, seed)
+    got = collect(srv, len(reqs))
+    srv.close()
+    return got
+
+
+def test_device_loop_alone_mixed_equal_host_loop():
+    rng = np.random.default_rng(21)
+    ch = chains(rng, 6)
+    reqs = requests(rng, ch, 48)
+    ref = host_loop_results(ch, reqs)
+    srv = Server(device=0, arena_bytes=1 << 30, max_inflight=256, device_loop=True)
+    sids = [srv.add_system(m, obs) for m, _, obs in ch]
+    alone = {}
+    for tag, (ci, p, f, n, seed) in enumerate(reqs):
+        print("device loop alone", tag, flush=True)
+        assert srv.submit(tag, sids[ci], p, f, n, seed)
+        alone.update(collect(srv, 1, timeout=30.0))
+    for tag, (ci, p, f, n, seed) in enumerate(reqs):
+        assert srv.submit(tag, sids[ci], p, f, n, seed)
+        if tag % 7 == 0:
+            time.sleep(0.0005)
+    mixed = collect(srv, len(reqs), timeout=60.0)
+    assert srv.pending() == 0
+    srv.close()
+    assert alone == mixed
+    assert mixed == ref
+
+
+def test_device_loop_ring_reuse_and_admission():
+    # 4 ring slots and a small arena: every slot is reused many times and
+    # requests wait for memory; results equal the host-driven loop's.
+    rng = np.random.default_rng(5)
+    ch = chains(rng, 3)
+    reqs = requests(rng, ch, 64)
+    ref = host_loop_results(ch, reqs)
+    srv = Server(device=0, arena_bytes=8 << 20, max_inflight=4, device_loop=True)
+    sids = [srv.add_system(m, obs) for m, _, obs in ch]
+    submitted = 0
+    done = {}
+    t0 = time.time()
+    while len(done) < len(reqs):
+        while submitted < len(reqs):
+            ci, p, f, n, seed = reqs[submitted]
+            if not srv.submit(submitted, sids[ci], p, f, n, seed):
+                break
+            submitted += 1
+        for c in srv.poll(64, wait_us=1000):
+            assert c.status == 0
+            done[c.tag] = (c.dist, c.tracks)
+        assert time.time() - t0 < 60, f"{len(done)} of {len(reqs)} done"
+    srv.close()
+    assert done == ref
+
+
+def test_device_loop_idle_exit_relaunch_and_new_model():
+    rng = np.random.default_rng(8)
+    ch = chains(rng, 3)
+    reqs = requests(rng, ch, 12)
+    ref = host_loop_results(ch, reqs)
+    srv = Server(device=0, arena_bytes=1 << 30, max_inflight=64, device_loop=True, loop_idle_us=2000)
+    s0 = srv.add_system(*ch[0][::2])
+    got = {}
+    first = [t for t, r in enumerate(reqs) if r[0] == 0]
+    for tag in first:
+        ci, p, f, n, seed = reqs[tag]
+        assert srv.submit(tag, s0, p, f, n, seed)
+    got.update(collect(srv, len(first), timeout=30.0))
+    time.sleep(0.05)  # the loop leaves after 2 ms without work; the next submit relaunches it
+    s1 = srv.add_system(*ch[1][::2])
+    s2 = srv.add_system(*ch[2][::2])
+    rest = [t for t in range(len(reqs)) if t not in first]
+    for tag in rest:
+        ci, p, f, n, seed = reqs[tag]
+        assert srv.submit(tag, [s0, s1, s2][ci], p, f, n, seed)
+    got.update(collect(srv, len(rest), timeout=30.0))
+    srv.close()
+    assert got == ref
+
+
+def test_device_loop_model_added_while_running():
+    # A model registered while the loop kernel runs: the loop is stopped and
+    # relaunched with the new model's tables before its requests are published.
+    rng = np.random.default_rng(9)
+    ch = chains(rng, 2)
+    big = Model(Lx=100.0, Ly=100.0, Lz=20.0, ax=80.0, ay=60.0, deltat=0.05, kmin=3, n_k=1500, n_de=1200,
+                ub_de=20.0, n_species=2)
+    ch.append((big, [2.0, 0.9], np.zeros((big.n_k, big.n_de), np.uint32)))
+    reqs = [(i % 2, *r[1:]) for i, r in enumerate(requests(rng, ch[:2], 40))]
+    p = proposal_record(big, [2.0, 0.9], [3e8, 3e8], [2.0], 99, 0, 2)
+    reqs.append((2, p, np.array([500, 300], np.uint32), np.array([700, 0], np.uint32), 77))
+    ref = host_loop_results(ch, reqs)
+    srv = Server(device=0, arena_bytes=1 << 30, max_inflight=64, device_loop=True)
+    sids = [srv.add_system(m, obs) for m, _, obs in ch[:2]]
+    for tag, (ci, p, f, n, seed) in enumerate(reqs[:-1]):
+        assert srv.submit(tag, sids[ci], p, f, n, seed)
+    sids.append(srv.add_system(big, ch[2][2]))
+    ci, p, f, n, seed = reqs[-1]
+    assert srv.submit(len(reqs) - 1, sids[2], p, f, n, seed)
+    got = collect(srv, len(reqs), timeout=60.0)
+    srv.close()
+    assert got == ref
+
+
+def test_device_loop_empty_and_multi_video_requests():
+    rng = np.random.default_rng(10)
+    ch = chains(rng, 3)
+    reqs = []
+    for i in range(12):
+        m, D, obs = ch[i % 3]
+        ct = 10.0 ** rng.uniform(8.0, 9.0)
+        p = proposal_record(m, D, list(rng.dirichlet(np.ones(m.n_species)) * ct), [2.0], i, 0, 5)
+        parts = rng.poisson(ct * m.volume_factor, 5).clip(0, 5000).astype(np.uint32)
+        if i % 4 == 0:
+            parts[:] = 0  # no particles at all: one empty tile, distance of an empty histogram
+        elif i % 4 == 1:
+            parts[1::2] = 0
+        frames = rng.integers(2, 700, 5).astype(np.uint32)
+        reqs.append((i % 3, p, frames, parts, 500 + i))
+    ref = host_loop_results(ch, reqs)
+    srv = Server(device=0, arena_bytes=1 << 30, max_inflight=32, device_loop=True)
+    sids = [srv.add_system(m, obs) for m, _, obs in ch]
+    for tag, (ci, p, f, n, seed) in enumerate(reqs):
+        assert srv.submit(tag, sids[ci], p, f, n, seed)
+    got = collect(srv, len(reqs), timeout=30.0)
+    srv.close()
+    assert got == ref
+    # the empty requests: no tracks
+    assert all(got[t][1] == 0 for t in range(0, 12, 4))
