@@ -11,7 +11,8 @@ before timing starts. Beside it:
   - plain form: 10,000 proposals of 1,000 frames, one walk + one distance
     launch per simulation;
   - serving form: Poisson arrivals of single-simulation requests from 32
-    chains with their own models, p50/p99 latency;
+    chains with their own models, p50/p99 latency, for the host-driven
+    launch groups and for the device-resident request loop;
   - roofline of the walk kernel from HIP events inside the timed replays;
   - CPU baseline: the oracle's FP64 restatement of the reference's CPU
     simulate + distance on a bounded sample of the same workload.
@@ -107,11 +108,12 @@ def run_timed(batch, steps, warmup, mode, seed0):
     return walls, total_ms, walk_ms
 
 
-def serving_latency(n_requests, rate, seed=7):
+def serving_latency(n_requests, rate, seed=7, device_loop=False):
     """Poisson arrivals at `rate` req/s from 32 chains, each with its own box
-    (lateral 100-400 um, depth 10-50 um) and 2-4 species (100-600 nm)."""
+    (lateral 100-400 um, depth 10-50 um) and 2-4 species (100-600 nm);
+    host-driven launch groups or the device-resident loop."""
     rng = np.random.default_rng(seed)
-    srv = Server(device=0, arena_bytes=4 << 30, max_inflight=4096)
+    srv = Server(device=0, arena_bytes=4 << 30, max_inflight=4096, device_loop=device_loop)
     chains = []
     for c in range(32):
         L = float(rng.uniform(100, 400))
@@ -154,7 +156,7 @@ def serving_latency(n_requests, rate, seed=7):
             "p50_ms": float(np.percentile(lat, 50) / 1e3), "p99_ms": float(np.percentile(lat, 99) / 1e3)}
 
 
-def large_model(n_sims, seed=9):
+def large_model(n_sims, seed=9, device_loop=False):
     """Large polydisperse model run batched under the serving loop: 400x400x60
     um box, 5 classes (100/200/300/450/600 nm) with Dirichlet(1,...,1)
     abundances, Gaussian z detection (sigma 1 um), c = 1e10/mL (~96,000
@@ -163,7 +165,7 @@ def large_model(n_sims, seed=9):
     m = Model(Lx=400.0, Ly=400.0, Lz=60.0, ax=160.0, ay=120.0, deltat=0.05, kmin=3, n_k=2000, n_de=1000,
               ub_de=20.0, n_species=5, detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0)
     D = [stokes_einstein(d) for d in (100.0, 200.0, 300.0, 450.0, 600.0)]
-    srv = Server(device=0, arena_bytes=8 << 30, max_inflight=1024)
+    srv = Server(device=0, arena_bytes=8 << 30, max_inflight=1024, device_loop=device_loop)
     sid = srv.add_system(m)
     reqs, steps = [], 0.0
     for i in range(n_sims):
@@ -339,17 +341,21 @@ def main():
                  "sims": args.plain_sims, "wall_s": pw[0]}
         log(f"plain: {psims:.0f} sims/s over {args.plain_sims} simulations ({pw[0]:.2f} s)")
 
-    serve = None
+    serve = serve_dev = None
     if not args.no_serve:
-        serve = []
+        serve, serve_dev = [], []
         for rate in [float(x) for x in args.serve_rates.split(",") if x]:
             serve.append(serving_latency(args.serve_requests, rate))
-            log(f"serving: {serve[-1]}")
+            log(f"serving (host-driven groups): {serve[-1]}")
+            serve_dev.append(serving_latency(args.serve_requests, rate, device_loop=True))
+            log(f"serving (device-resident loop): {serve_dev[-1]}")
 
-    large = None
+    large = large_dev = None
     if args.large_sims > 0:
         large = large_model(args.large_sims)
-        log(f"large polydisperse model (serving loop): {large}")
+        log(f"large polydisperse model (host-driven serving): {large}")
+        large_dev = large_model(args.large_sims, device_loop=True)
+        log(f"large polydisperse model (device-resident loop): {large_dev}")
 
     cpu = None
     if not args.no_cpu:
@@ -388,6 +394,7 @@ def main():
             "plain": plain,
             "batched_vs_plain": (sims_s / plain["sims_per_s"]) if plain else None,
             "serving": serve,
+            "serving_device_loop": serve_dev,
             "p50_ms": serve[1]["p50_ms"] if serve and len(serve) > 1 else (serve[0]["p50_ms"] if serve else None),
             "p99_ms": serve[1]["p99_ms"] if serve and len(serve) > 1 else (serve[0]["p99_ms"] if serve else None),
             "overflow_reruns": reruns,
@@ -397,6 +404,7 @@ def main():
             "arena_bytes": arena_bytes,
             "arena_use": arena_use,
             "large_polydisperse": large,
+            "large_polydisperse_device_loop": large_dev,
             "ms_per_step_graph_device": float(np.mean(total_ms)),
             "ms_per_step_direct": float(np.mean(dwalls)) * 1e3,
             "parallelism": "single GPU",

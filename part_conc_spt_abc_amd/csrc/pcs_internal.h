@@ -78,15 +78,18 @@ struct DistArgs {
 //
 // Requests are numbered r = 0, 1, 2, ... in the order the host publishes them
 // and live in ring slot r % R. The host writes a LoopReq (+ the slot's video
-// arrays) into pinned, coherent host memory and publishes it by storing
-// seq = r + 1 last. Inside the loop kernel a slot of HBM state moves through
+// table) into pinned, coherent host memory and publishes it by storing
+// seq = r + 1 last. Group 0 of the loop kernel is the admitter: in request
+// order it copies each published request into its slot once the slot is free.
+// A slot's state moves through
 //   free (0, or 4(r-R)+3)  ->  4r+1 (being copied in)  ->  4r+2 (tiles open)
-//   ->  4r+3 (result written; slot free for r + R)
-// and one 64-bit cursor (request << 24 | next tile) hands out tiles in
-// request order. The group that finishes a request's last tile computes its
-// distance and writes a LoopDone into host memory, seq = r + 1 last.
-constexpr uint32_t kLoopTileBits = 24;
-constexpr uint32_t kLoopMaxTiles = (1u << kLoopTileBits) - 64u;
+//   ->  4r+3 (result written; slot free for r + R).
+// The other groups are workers: a cursor names the oldest request that may
+// still have unclaimed tiles, and a worker claims up to 4 tiles of it with a
+// fetch-add on the slot's claim word (low 32 bits of r << 32 | next tile).
+// The group that finishes a request's last tile computes its distance and
+// writes a LoopDone into host memory, seq = r + 1 last.
+constexpr uint32_t kLoopMaxTiles = (1u << 30);
 constexpr int kLoopWaves = 4;  // waves per group; a group claims up to 4 tiles at a time
 
 struct LoopReq {
@@ -109,22 +112,22 @@ static_assert(sizeof(LoopDone) == 32, "LoopDone layout");
 struct LoopSlot {
     uint64_t state;
     uint32_t tiles, tiles_done, rec_count, n_videos;
-    uint64_t pad;
+    uint64_t claim;  // (r mod 2^32) << 32 | next unclaimed tile
     DevProp prop;
 };
 static_assert(sizeof(LoopSlot) == 176, "LoopSlot layout");
 
 struct LoopCtl {
-    uint64_t cursor;
+    uint64_t cursor;      // oldest request that may have unclaimed tiles
     uint64_t last_claim;  // wall clock of the latest tile claim, any group
+    uint64_t admit_next;  // next request the admitter copies in (kept across launches)
     uint32_t quit;        // set when the loop should end: every group leaves
-    uint32_t pad;
+    uint32_t error;       // non-zero: a claim could not be resolved (the loop ended)
 };
 
 // Per-slot video table, (maxv + 1) entries of 3 words at slot * 3 * (maxv + 1)
 // in host memory (ring_vid) and HBM (slot_vid): entry v = {frames, particles,
 // first tile} of video v, entry n_videos = {0, 0, total tiles}.
-constexpr uint32_t kLoopPollers = 8;  // groups that read the host ring and stop flag
 
 struct LoopArgs {
     const DevSys* sys;

@@ -482,14 +482,15 @@ __global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
 // ------------------------------------------------------------ request loop
 //
 // serve_loop_kernel is the device-resident form of the serving loop: a
-// persistent grid of 4-wave groups takes requests straight from a ring in
-// pinned host memory. Wave 0 of a group decides (admit the next published
-// request, claim up to 4 tiles from the cursor, or idle); each wave walks one
-// claimed tile; the group that completes a request's last tile computes its
-// distance with the block algorithm above and writes the result to host
-// memory. Every wait is bounded: a group leaves on the host's stop flag, on
-// another group's quit flag, or after idle_ticks without work (setting quit),
-// and no group ever waits for another one to make progress.
+// persistent grid of 4-wave groups fed from a ring in pinned host memory.
+// Group 0 is the admitter: one wave that copies published requests into
+// their HBM slots in request order, ahead of the workers. Every other group is
+// a worker: wave 0 claims up to 4 tiles of the oldest request with unclaimed
+// tiles, each wave walks one of them, and the group that completes a
+// request's last tile computes its distance with the block algorithm above
+// and writes the result to host memory. Every wait is bounded: groups leave
+// on the quit flag (set by the admitter when the host raises stop, or by a
+// worker after idle_ticks in which no group claimed a tile).
 
 enum : uint32_t { kLoopIdle = 0u, kLoopWork = 1u, kLoopExit = 2u };
 
@@ -518,6 +519,16 @@ __device__ __forceinline__ uint32_t ld_system32(const uint32_t* p)
     return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+__device__ __forceinline__ void st_agent32(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_agent64(uint64_t* p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ bool cas_agent64(uint64_t* p, uint64_t expect, uint64_t want)
 {
     return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
@@ -539,28 +550,20 @@ __device__ __forceinline__ T load_uniform(const T* p)
     return out;
 }
 
-// Wave-wide: true once request q owns its slot with its tiles open (or
-// finished). A poller group also admits q: if the host has published q and
-// the slot is free, it copies q in (exactly one wave wins the slot) and opens
-// it. Other groups only watch the slot state, so host memory is read by at
-// most kLoopPollers groups.
-__device__ bool loop_admit(const LoopArgs& a, uint64_t q, bool poller)
+// Admitter, wave-wide: copy request q into its slot if the host has published
+// it and the slot's previous request (q - R) has finished. Only the admitter
+// writes slot contents, state 4q+1 and 4q+2, in request order.
+__device__ bool loop_admit(const LoopArgs& a, uint64_t q)
 {
     const uint32_t slot = (uint32_t)(q % a.R);
     LoopSlot& S = a.slots[slot];
     const bool l0 = lane_id() == 0u;
-    const uint64_t st = bcast64(l0 ? ld_agent64(&S.state) : 0ull);
-    if (st == 4ull * q + 2ull || st == 4ull * q + 3ull) return true;
-    if (!poller) return false;
     const uint64_t expect = q < a.R ? 0ull : 4ull * (q - a.R) + 3ull;
-    if (st != expect) return false;
+    if (bcast64(l0 ? ld_agent64(&S.state) : 0ull) != expect) return false;
     const LoopReq& Q = a.ring[slot];
-    const uint64_t seq = bcast64(l0 ? ld_system64(&Q.seq) : 0ull);
-    if (seq != q + 1ull) return false;
-    uint32_t won = 0u;
-    if (l0) won = cas_agent64(&S.state, expect, 4ull * q + 1ull) ? 1u : 0u;
-    if (uniform(won) == 0u) return false;
+    if (bcast64(l0 ? ld_system64(&Q.seq) : 0ull) != q + 1ull) return false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the ring entry was written before its seq
+    if (l0) st_agent64(&S.state, 4ull * q + 1ull);
     // One round of host reads: the proposal (36 words) and n_videos, tiles.
     constexpr uint32_t kPropWords = sizeof(DevProp) / 4u;
     const uint32_t* src = reinterpret_cast<const uint32_t*>(&Q.prop);
@@ -582,49 +585,108 @@ __device__ bool loop_admit(const LoopArgs& a, uint64_t q, bool poller)
         S.n_videos = nv;
         S.tiles_done = 0u;
         S.rec_count = 0u;
+        st_agent64(&S.claim, (q & 0xffffffffull) << 32);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (l0) __hip_atomic_store(&S.state, 4ull * q + 2ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (l0) st_agent64(&S.state, 4ull * q + 2ull);
     return true;
 }
 
-// Wave-wide decision of wave 0: kLoopWork with (r, t0, nt) claimed, kLoopIdle
-// (nothing claimable now) or kLoopExit (stop or quit).
-__device__ uint32_t loop_decide(const LoopArgs& a, bool poller, uint64_t& r_out, uint32_t& t0, uint32_t& nt)
+__device__ void loop_admitter(const LoopArgs& a)
 {
     const bool l0 = lane_id() == 0u;
-    constexpr uint32_t kTileMask = (1u << kLoopTileBits) - 1u;
-    for (int attempt = 0; attempt < 16; ++attempt) {
+    uint64_t q = bcast64(l0 ? ld_agent64(&a.ctl->admit_next) : 0ull);
+    for (;;) {
         uint32_t leave = 0u;
         if (l0) {
             leave = ld_agent32(&a.ctl->quit);
-            if (poller && leave == 0u && ld_system32(a.stop) != 0u) {
-                __hip_atomic_store(&a.ctl->quit, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (leave == 0u && ld_system32(a.stop) != 0u) {
+                st_agent32(&a.ctl->quit, 1u);
                 leave = 1u;
             }
         }
-        if (uniform(leave)) return kLoopExit;
-        const uint64_t c = bcast64(l0 ? ld_agent64(&a.ctl->cursor) : 0ull);
-        const uint64_t r = c >> kLoopTileBits;
-        const uint32_t t = (uint32_t)c & kTileMask;
-        if (!loop_admit(a, r, poller)) return kLoopIdle;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const uint32_t T = uniform(l0 ? ld_agent32(&a.slots[r % a.R].tiles) : 0u);
-        if (t < T) {
-            const uint32_t n = min((uint32_t)kLoopWaves, T - t);
-            uint32_t won = 0u;
-            if (l0) won = cas_agent64(&a.ctl->cursor, c, c + n) ? 1u : 0u;
-            if (uniform(won)) {
-                r_out = r;
-                t0 = t;
-                nt = n;
-                return kLoopWork;
-            }
-            continue;  // another group moved the cursor
+        if (uniform(leave)) return;
+        uint32_t n = 0u;
+        while (n < 64u && loop_admit(a, q)) {
+            ++q;
+            ++n;
         }
-        // Every tile of r is claimed: move on once r + 1 is admitted.
-        if (!loop_admit(a, r + 1ull, poller)) return kLoopIdle;
-        if (l0) cas_agent64(&a.ctl->cursor, c, (r + 1ull) << kLoopTileBits);
+        if (n != 0u) {
+            if (l0) st_agent64(&a.ctl->admit_next, q);
+        } else {
+            __builtin_amdgcn_s_sleep(16);
+        }
+    }
+}
+
+// Worker, wave-wide: take the fetch-add result `old` on a slot's claim word.
+// If it names request r (the one the worker looked at), it claims tiles
+// [t, t+4) of r when t < T. If the slot has moved on to a newer request q,
+// the increment landed on q's word and claims q's tiles: those are resolved
+// against q once the admitter has opened q, so that no tile is skipped.
+__device__ uint32_t loop_resolve(const LoopArgs& a, LoopSlot& S, uint64_t r, uint32_t T, uint64_t old,
+                                 uint64_t& r_out, uint32_t& t0, uint32_t& nt)
+{
+    const bool l0 = lane_id() == 0u;
+    const uint32_t q32 = (uint32_t)(old >> 32), t = (uint32_t)old;
+    if (q32 == (uint32_t)r) {
+        if (t >= T) return kLoopIdle;
+        r_out = r;
+        t0 = t;
+        nt = min((uint32_t)kLoopWaves, T - t);
+        return kLoopWork;
+    }
+    for (uint32_t spin = 0; spin < (1u << 20); ++spin) {
+        const uint64_t st = bcast64(l0 ? ld_agent64(&S.state) : 0ull);
+        const uint64_t q = st >> 2;
+        if ((uint32_t)q != q32 || (st & 3ull) == 3ull) return kLoopIdle;  // q finished: the claim was surplus
+        if ((st & 3ull) == 2ull) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const uint32_t Tq = uniform(l0 ? ld_agent32(&S.tiles) : 0u);
+            if (t >= Tq) return kLoopIdle;
+            r_out = q;
+            t0 = t;
+            nt = min((uint32_t)kLoopWaves, Tq - t);
+            return kLoopWork;
+        }
+        __builtin_amdgcn_s_sleep(2);  // the admitter is copying q in
+    }
+    if (l0) {
+        st_agent32(&a.ctl->error, 1u);
+        st_agent32(&a.ctl->quit, 1u);
+    }
+    return kLoopExit;
+}
+
+// Worker decision (wave 0): kLoopWork with (r, t0, nt) claimed, kLoopIdle or
+// kLoopExit.
+__device__ uint32_t loop_decide(const LoopArgs& a, uint64_t& r_out, uint32_t& t0, uint32_t& nt)
+{
+    const bool l0 = lane_id() == 0u;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        if (uniform(l0 ? ld_agent32(&a.ctl->quit) : 0u)) return kLoopExit;
+        const uint64_t r = bcast64(l0 ? ld_agent64(&a.ctl->cursor) : 0ull);
+        LoopSlot& S = a.slots[r % a.R];
+        const uint64_t st = bcast64(l0 ? ld_agent64(&S.state) : 0ull);
+        if (st == 4ull * r + 2ull) {
+            const uint32_t T = uniform(l0 ? ld_agent32(&S.tiles) : 0u);
+            const uint64_t w = bcast64(l0 ? ld_agent64(&S.claim) : 0ull);
+            if ((uint32_t)(w >> 32) == (uint32_t)r && (uint32_t)w < T) {
+                uint64_t old = 0ull;
+                if (l0) old = __hip_atomic_fetch_add(&S.claim, (uint64_t)kLoopWaves, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                old = bcast64(old);
+                const uint32_t k = loop_resolve(a, S, r, T, old, r_out, t0, nt);
+                if (k != kLoopIdle) return k;
+                continue;
+            }
+        } else if (st != 4ull * r + 3ull) {
+            return kLoopIdle;  // r not admitted yet (or the cursor was read stale)
+        }
+        // Every tile of r is claimed: advance once r + 1 is open.
+        const uint64_t st1 = bcast64(l0 ? ld_agent64(&a.slots[(r + 1ull) % a.R].state) : 0ull);
+        if (st1 != 4ull * (r + 1ull) + 2ull && st1 != 4ull * (r + 1ull) + 3ull) return kLoopIdle;
+        if (l0) cas_agent64(&a.ctl->cursor, r, r + 1ull);
     }
     return kLoopIdle;
 }
@@ -641,7 +703,10 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid / kWave;
-    const bool poller = blockIdx.x < kLoopPollers;
+    if (blockIdx.x == 0u) {  // the admitter: one wave, no barriers
+        if (wave == 0u) loop_admitter(a);
+        return;
+    }
     Stage st;
     st.buf = loop_lds + wave * a.stage_cap;
     st.cap = a.stage_cap;
@@ -656,13 +721,13 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
         if (wave == 0u) {
             uint64_t r = 0ull;
             uint32_t t0 = 0u, nt = 0u;
-            uint32_t kind = loop_decide(a, poller, r, t0, nt);
+            uint32_t kind = loop_decide(a, r, t0, nt);
             const uint64_t now = wall_clock64();
             if (kind == kLoopWork) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // slot contents of the admitter
                 last_work = now;
                 idle_run = 0u;
-                if (lane_id() == 0u)
-                    __hip_atomic_store(&a.ctl->last_claim, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane_id() == 0u) st_agent64(&a.ctl->last_claim, now);
             } else if (kind == kLoopIdle) {
                 idle_run = min(idle_run + 1u, 4u);
                 if (now - last_work > a.idle_ticks) {
@@ -670,8 +735,7 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
                     const uint64_t g = bcast64(lane_id() == 0u ? ld_agent64(&a.ctl->last_claim) : 0ull);
                     if (g > last_work) last_work = g;
                     if (now - last_work > a.idle_ticks) {
-                        if (lane_id() == 0u)
-                            __hip_atomic_store(&a.ctl->quit, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (lane_id() == 0u) st_agent32(&a.ctl->quit, 1u);
                         kind = kLoopExit;
                     }
                 }
@@ -680,8 +744,7 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
                 cmd[0] = kind;
                 cmd[1] = t0;
                 cmd[2] = nt;
-                // pollers wake every ~3 us; the others back off to ~27 us
-                cmd[4] = (poller || idle_run == 0u) ? 1u : (1u << (idle_run - 1u));
+                cmd[4] = idle_run == 0u ? 1u : (1u << (idle_run - 1u));  // back off to ~27 us
                 cmd_r = r;
             }
         }
@@ -698,7 +761,6 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
         const uint32_t t0 = uniform(cmd[1]), nt = uniform(cmd[2]);
         const uint32_t slot = (uint32_t)(r % a.R);
         LoopSlot& S = a.slots[slot];
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // slot contents of the admitting group
         if (wave < nt) {
             const uint32_t t = t0 + wave;
             const uint32_t* vid = a.slot_vid + (uint64_t)slot * vw;
@@ -723,10 +785,9 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
                 if (zref) walk_tile<1, false, true, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
                 else walk_tile<1, false, false, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this wave's records
         }
-        __syncthreads();
-        if (tid == 0u) {
+        __syncthreads();  // every wave's records are stored
+        if (tid == 0u) {  // release them at agent scope with the count
             const uint32_t prev = __hip_atomic_fetch_add(&S.tiles_done, nt, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             cmd[3] = (prev + nt == S.tiles) ? 1u : 0u;
         }

@@ -521,7 +521,8 @@ int loop_launch(pcs_server* s, uint32_t nsys)
     hipError_t e = serve_loop_occupancy(a, nde, &gpc);
     if (e != hipSuccess) return hip_error(e, "serving loop occupancy");
     if (gpc < 1) return set_error(PCS_EINVAL, "serving loop does not fit a CU (n_k %u, n_de %u)", nk, nde);
-    const uint32_t grid = (uint32_t)L.n_cu * (uint32_t)std::min(gpc, 8);
+    // group 0 admits requests, the others walk them
+    const uint32_t grid = std::max(2u, (uint32_t)L.n_cu * (uint32_t)std::min(gpc, 8));
     e = hipMemsetAsync(&L.d_ctl->quit, 0, sizeof(uint32_t), L.stream);
     if (e == hipSuccess) e = launch_serve_loop(a, grid, nde, L.stream);
     if (e == hipSuccess) e = hipEventRecord(L.ev, L.stream);
@@ -552,6 +553,13 @@ void scheduler_loop(pcs_server* s)
             const hipError_t q = hipEventQuery(L.ev);
             if (q == hipSuccess) {
                 L.running = false;
+                uint32_t err = 0;
+                const hipError_t e = hipMemcpy(&err, &L.d_ctl->error, sizeof(err), hipMemcpyDeviceToHost);
+                if (e != hipSuccess || err != 0) {
+                    fail(s, e != hipSuccess ? hip_error(e, "serving loop")
+                                            : set_error(PCS_ESTATE, "serving loop: a tile claim could not be resolved"));
+                    return;
+                }
             } else if (q != hipErrorNotReady) {
                 fail(s, hip_error(q, "serving loop"));
                 return;
