@@ -414,9 +414,17 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     const bool l1 = sy.dist_kind == PCS_DIST_L1;
     const uint32_t* R = sy.obs_sat;
 
+    const uint32_t maxrow_obs = sy.maxrow_obs;
+    uint64_t racc_prev = 0ull;
     for (uint32_t i = 0; i <= last; ++i) {
         const uint32_t s0 = (i == 0u) ? 0u : rowc[i - 1u];
         const uint32_t s1 = rowc[i];
+        // Past the deepest observed row R repeats its row; a row without
+        // records repeats S's row, so it contributes what the row above did.
+        if (s1 == s0 && i > maxrow_obs && i != last) {
+            acc += racc_prev;
+            continue;
+        }
         if (s1 > s0) {
             for (uint32_t t = s0 + tid; t < s1; t += kDistThreads) atomicAdd(&colh[srt[t]], 1u);
             __syncthreads();
@@ -448,6 +456,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
         }
         // Rows past `last` repeat row `last` in both tables.
         acc += (i == last) ? racc * (uint64_t)(n_k - last) : racc;
+        racc_prev = racc;
     }
 
     red[tid] = acc;
