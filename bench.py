@@ -370,6 +370,11 @@ def main():
         cpu = cpu_baseline(model, props, frames, parts, obs, args.cpu_sample)
         log(f"cpu baseline: {cpu}")
 
+    def pick(sweep, key):
+        if not sweep:
+            return None
+        return sweep[1][key] if len(sweep) > 1 else sweep[0][key]
+
     ach = steps_per_gen * OPS_PER_STEP / walk_s if walk_s == walk_s else None
     # HBM bytes per launch: FETCH_SIZE/WRITE_SIZE per particle-step from the
     # committed rocprofv3 runs (copy-calibrated), times this launch's steps.
@@ -403,8 +408,10 @@ def main():
             "batched_vs_plain": (sims_s / plain["sims_per_s"]) if plain else None,
             "serving": serve,
             "serving_device_loop": serve_dev,
-            "p50_ms": serve[1]["p50_ms"] if serve and len(serve) > 1 else (serve[0]["p50_ms"] if serve else None),
-            "p99_ms": serve[1]["p99_ms"] if serve and len(serve) > 1 else (serve[0]["p99_ms"] if serve else None),
+            # headline latency: the device-resident loop at the second swept rate (5,000 req/s by default)
+            "p50_ms": pick(serve_dev, "p50_ms"),
+            "p99_ms": pick(serve_dev, "p99_ms"),
+            "latency_form": "device-resident loop" if serve_dev else None,
             "overflow_reruns": reruns,
             "tuning": tuning,
             "host_packing_s": t_pack,

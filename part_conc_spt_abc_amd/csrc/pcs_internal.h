@@ -117,6 +117,9 @@ struct LoopSlot {
 };
 static_assert(sizeof(LoopSlot) == 176, "LoopSlot layout");
 
+// CU index of a group: XCC id << 8 | SE/SH/CU id of HW_ID, below 2048.
+constexpr uint32_t kLoopCuSlots = 2048;
+
 struct LoopCtl {
     uint64_t cursor;      // oldest request that may have unclaimed tiles
     uint64_t last_claim;  // wall clock of the latest tile claim, any group
@@ -138,6 +141,7 @@ struct LoopArgs {
     LoopDone* done;
     const uint32_t* stop;   // host memory: non-zero = leave now
     LoopCtl* ctl;
+    uint32_t* cu_token;     // kLoopCuSlots poller tokens, one per CU (zero between launches)
     uint32_t R, maxv;
     uint32_t stage_cap;     // LDS records per wave
     uint32_t nk_lds, nde_lds;

@@ -364,6 +364,9 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     if (count > pr.rec_cap)  // record slice overflowed: counted, not stored
         return DistOut{0ull, count, PCS_EOVERFLOW};
     const uint32_t T = count;
+    // Issue priority over walking waves that share the SIMDs: a distance is
+    // one workgroup on the critical path of its simulation's latency.
+    __builtin_amdgcn_s_setprio(3);
 
     for (uint32_t i = tid; i < n_k; i += kDistThreads) rowc[i] = 0u;
     for (uint32_t i = tid; i < n_de; i += kDistThreads) colh[i] = 0u;
@@ -465,6 +468,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
         if (tid < s) red[tid] += red[tid + s];
         __syncthreads();
     }
+    __builtin_amdgcn_s_setprio(0);
     return DistOut{red[0], T, PCS_OK};
 }
 
@@ -667,6 +671,14 @@ __device__ uint32_t loop_resolve(const LoopArgs& a, LoopSlot& S, uint64_t r, uin
     return kLoopExit;
 }
 
+// Index of this wave's CU: XCC id << 8 | SE, SH, CU id of HW_ID (< 2048).
+__device__ __forceinline__ uint32_t hw_cu_slot()
+{
+    const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);   // HW_ID
+    const uint32_t xcc = __builtin_amdgcn_s_getreg((31 << 11) | 20); // XCC_ID
+    return (((xcc & 7u) << 8) | ((hw >> 8) & 0xffu)) & (kLoopCuSlots - 1u);
+}
+
 // Worker decision (wave 0): kLoopWork with (r, t0, nt) claimed, kLoopIdle or
 // kLoopExit.
 __device__ uint32_t loop_decide(const LoopArgs& a, uint64_t& r_out, uint32_t& t0, uint32_t& nt)
@@ -723,8 +735,17 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
     uint32_t* rowc = loop_lds + kLoopWaves * a.stage_cap;
     const DistLds L{rowc, rowc + a.nk_lds, wsum, &smax, red};
     const uint32_t vw = 3u * (a.maxv + 1u);
-    uint64_t last_work = wall_clock64();
+    // Polling: one idle group per CU holds the CU's poller token and polls
+    // every sleep unit (~3.4 us); the CU's other idle groups back off from 1
+    // to 64 units after their last work. A request that reaches a quiet GPU
+    // is therefore claimed first by one group per CU and its tiles run one
+    // wave per SIMD, while groups that just finished work stay responsive
+    // under load. The token passes on when its holder claims work; any group
+    // that finds open tiles claims them.
+    const uint32_t cu = hw_cu_slot();
+    bool poller = false;
     uint32_t idle_run = 0u;
+    uint64_t last_work = wall_clock64();
 
     for (;;) {
         if (wave == 0u) {
@@ -732,13 +753,28 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
             uint32_t t0 = 0u, nt = 0u;
             uint32_t kind = loop_decide(a, r, t0, nt);
             const uint64_t now = wall_clock64();
+            uint32_t units = 1u;
             if (kind == kLoopWork) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // slot contents of the admitter
                 last_work = now;
                 idle_run = 0u;
-                if (lane_id() == 0u) st_agent64(&a.ctl->last_claim, now);
+                if (lane_id() == 0u) {
+                    st_agent64(&a.ctl->last_claim, now);
+                    if (poller) st_agent32(&a.cu_token[cu], 0u);
+                }
+                poller = false;
             } else if (kind == kLoopIdle) {
-                idle_run = min(idle_run + 1u, 4u);
+                if (!poller) {
+                    uint32_t won = 0u;
+                    if (lane_id() == 0u) {
+                        uint32_t expect = 0u;
+                        won = __hip_atomic_compare_exchange_strong(&a.cu_token[cu], &expect, 1u, __ATOMIC_RELAXED,
+                                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1u : 0u;
+                    }
+                    poller = uniform(won) != 0u;
+                }
+                units = poller ? 1u : (1u << min(idle_run, 6u));
+                idle_run = min(idle_run + 1u, 6u);
                 if (now - last_work > a.idle_ticks) {
                     // idle here; has any group claimed work lately?
                     const uint64_t g = bcast64(lane_id() == 0u ? ld_agent64(&a.ctl->last_claim) : 0ull);
@@ -749,11 +785,15 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
                     }
                 }
             }
+            if (kind == kLoopExit && poller) {
+                if (lane_id() == 0u) st_agent32(&a.cu_token[cu], 0u);
+                poller = false;
+            }
             if (lane_id() == 0u) {
                 cmd[0] = kind;
                 cmd[1] = t0;
                 cmd[2] = nt;
-                cmd[4] = idle_run == 0u ? 1u : (1u << (idle_run - 1u));  // back off to ~27 us
+                cmd[4] = units;
                 cmd_r = r;
             }
         }

@@ -128,6 +128,7 @@ struct LoopHost {
     LoopSlot* d_slots = nullptr;
     uint32_t* d_slot_vid = nullptr;
     LoopCtl* d_ctl = nullptr;
+    uint32_t* d_cu_token = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     bool running = false, stopping = false;
@@ -511,6 +512,7 @@ int loop_launch(pcs_server* s, uint32_t nsys)
     a.done = L.h_done;
     a.stop = L.h_stop;
     a.ctl = L.d_ctl;
+    a.cu_token = L.d_cu_token;
     a.R = L.R;
     a.maxv = L.maxv;
     a.stage_cap = 256;
@@ -639,6 +641,8 @@ hipError_t loop_alloc(pcs_server* s)
     if (e == hipSuccess) e = dmalloc(&L.d_slots, L.R);
     if (e == hipSuccess) e = dmalloc(&L.d_slot_vid, vid);
     if (e == hipSuccess) e = dmalloc(&L.d_ctl, 1);
+    if (e == hipSuccess) e = dmalloc(&L.d_cu_token, kLoopCuSlots);
+    if (e == hipSuccess) e = hipMemset(L.d_cu_token, 0, kLoopCuSlots * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(L.d_slots, 0, L.R * sizeof(LoopSlot));
     if (e == hipSuccess) e = hipMemset(L.d_ctl, 0, sizeof(LoopCtl));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
@@ -662,7 +666,7 @@ void loop_free(pcs_server* s)
     LoopHost& L = s->loop;
     if (L.h_stop) __atomic_store_n(L.h_stop, 1u, __ATOMIC_RELEASE);
     if (L.stream) hipStreamSynchronize(L.stream);
-    void* dev[] = {L.d_slots, L.d_slot_vid, L.d_ctl};
+    void* dev[] = {L.d_slots, L.d_slot_vid, L.d_ctl, L.d_cu_token};
     for (void* p : dev)
         if (p) hipFree(p);
     void* host[] = {L.h_ring, L.h_ring_vid, L.h_done, L.h_stop};
