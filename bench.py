@@ -243,33 +243,6 @@ def cpu_baseline(model, props, frames, parts, obs, n_sample):
                       f"{cores} threads, {dt:.1f} s"}
 
 
-def profile_run(args):
-    """Workload for rocprofv3: the batched kernels launched directly (one
-    warmup, `steps` profiled replays) on the first `batched_sims` proposals of
-    the batched workload, then a device-to-device copy of 256 MiB."""
-    model = headline_model()
-    ctx = Context(model)
-    obs = observed_hist(ctx, model)
-    ctx.close()
-    ctx = Context(model, obs)
-    rng = np.random.default_rng(1)
-    n = min(args.batched_sims, 4000)
-    props, frames, parts = workload(model, n, rng, (500, 1000, 2000))
-    b = ctx.batch()
-    b.load(props, frames, parts)
-    walls, _, walk_ms = run_timed(b, args.steps, 1, _abi.PCS_MODE_BATCHED_DIRECT, 100)
-    b.close()
-    steps = particle_steps(frames, parts)
-    log(f"profile: {n} proposals, {steps:.3e} particle-steps, walk kernel {np.mean(walk_ms):.2f} ms")
-    import ctypes as C
-
-    ms = C.c_float(0.0)
-    _abi.check(_abi.load().pcs_selftest_copy(0, 256 << 20, C.byref(ms)), "pcs_selftest_copy")
-    log(f"profile: copied 268435456 bytes device to device in {ms.value:.3f} ms")
-    print(json.dumps({"proposals": n, "particle_steps": steps, "walk_ms": float(np.mean(walk_ms)),
-                      "copy_bytes": 256 << 20}), flush=True)
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,12 +258,13 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--large-sims", type=int, default=64)
-    ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: batched-direct replays of a reduced workload, then a 256 MiB "
-                         "device copy whose known traffic calibrates the FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--tuning", default="",
+                    help="launch shape 'particles_per_lane,waves_per_group,groups_per_cu' instead of the autotuner "
+                         "(profiling runs: no autotuner launches among the profiled ones)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the batched leg, copy 256 MiB device to device (pcs_selftest_copy): its known "
+                         "traffic calibrates rocprofv3's FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
-    if args.profile:
-        return profile_run(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is sharded by proposal "
               f"ranges across GPUs by design but nothing here runs on more than one.", file=sys.stderr)
@@ -309,7 +283,12 @@ def main():
     t_pack = time.perf_counter() - t_pack
     steps_per_gen = particle_steps(frames, parts)
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        ppl, wpg, gpc = (int(x) for x in args.tuning.split(","))
+        ctx.set_tuning(ppl, wpg, gpc, 256)
+        tuning = ctx.get_tuning()
+        log(f"tuning: {tuning}")
+    elif not args.no_autotune:
         # launch shape chosen by the autotuner on a calibration slice of the workload (untimed)
         k = min(len(props), 4000)
         tuning, tms = ctx.autotune(props[:k], frames[:k], parts[:k])
@@ -335,6 +314,12 @@ def main():
     walk_s = float(np.mean(walk_ms)) / 1e3 if min(walk_ms) > 0 else float("nan")
     log(f"batched: {sims_s:.0f} sims/s, {psteps_s:.3e} particle-steps/s, walk kernel {walk_s*1e3:.1f} ms, "
         f"graph {np.mean(total_ms):.1f} ms, wall {np.mean(walls)*1e3:.1f} ms, reruns {reruns}")
+    if args.calibrate_copy:
+        import ctypes as C
+
+        cms = C.c_float(0.0)
+        _abi.check(_abi.load().pcs_selftest_copy(0, 256 << 20, C.byref(cms)), "pcs_selftest_copy")
+        log(f"calibration: copied 268435456 bytes device to device in {cms.value:.3f} ms")
 
     plain = None
     if not args.no_plain:
