@@ -208,6 +208,8 @@ Estimator::Trial Estimator::perturb(std::size_t parent)
 
 void Estimator::simulate(const std::vector<Trial>& trials, std::vector<double>& dist)
 {
+    using Clock = std::chrono::steady_clock;
+    const auto t0 = Clock::now();
     const std::size_t n = trials.size();
     const std::size_t J = par_.frameCounts.size();
     const double volume = par_.Lx * par_.Ly * par_.Lz / 1e12;
@@ -240,10 +242,19 @@ void Estimator::simulate(const std::vector<Trial>& trials, std::vector<double>& 
         }
     }
     dist.resize(n);
+    const auto t1 = Clock::now();
     check(pcs_batch_load(batch_, props.data(), (uint32_t)n, frames.data(), parts.data(), (uint32_t)(n * J)),
           "pcs_batch_load");
+    const auto t2 = Clock::now();
     check(pcs_batch_run(batch_, rng_(), cfg_.mode), "pcs_batch_run");
     check(pcs_batch_read(batch_, dist.data(), nullptr, nullptr), "pcs_batch_read");
+    const auto t3 = Clock::now();
+    if (cfg_.verbosity >= 2)
+        std::printf("    round: %zu simulations, GPU run + read %.2f ms\n", n,
+                    std::chrono::duration<double, std::milli>(t3 - t2).count());
+    tPack_ += std::chrono::duration<double>(t1 - t0).count();
+    tLoad_ += std::chrono::duration<double>(t2 - t1).count();
+    tGpu_ += std::chrono::duration<double>(t3 - t2).count();
     simulations_ += n;
 }
 
@@ -284,6 +295,7 @@ void Estimator::run()
         std::partial_sum(w_.begin(), w_.end(), cum.begin());
         std::uniform_real_distribution<double> u01(0.0, 1.0);
 
+        tPack_ = tLoad_ = tGpu_ = tWeights_ = 0.0;
         std::vector<Trial> cur(N_);
         std::vector<double> distBis(N_, std::numeric_limits<double>::infinity());
         std::vector<std::size_t> active(N_);
@@ -291,11 +303,13 @@ void Estimator::run()
         double trialsTotal = 0.0, trialsAvg = 0.0;
         std::size_t rounds = 0;
         while (!active.empty() && trialsAvg < (double)par_.avgTrialCount) {
+            const auto tp = Clock::now();
             std::vector<Trial> batch(active.size());
             for (std::size_t a = 0; a < active.size(); ++a) {
                 const std::size_t parent = pick_by_weight(cum, u01(rng_));
                 batch[a] = perturb(parent);
             }
+            tPack_ += std::chrono::duration<double>(Clock::now() - tp).count();
             std::vector<double> d;
             simulate(batch, d);
             std::vector<std::size_t> still;
@@ -320,6 +334,7 @@ void Estimator::run()
             break;
         }
         // Weights from the new population against the previous one.
+        const auto tw = Clock::now();
         if (par_.weightingScheme == io::WeightingScheme::pmcStandard) {
             std::vector<std::vector<double>> tn, to;
             std::vector<double> tau;
@@ -359,6 +374,10 @@ void Estimator::run()
         } else {
             w_ = inverse_square_weights(distBis);
         }
+        tWeights_ = std::chrono::duration<double>(Clock::now() - tw).count();
+        if (cfg_.verbosity >= 1)
+            std::printf("  host packing %.3f s, upload + plan %.3f s, GPU run + read %.3f s, weights %.3f s\n", tPack_,
+                        tLoad_, tGpu_, tWeights_);
         for (std::size_t i = 0; i < N_; ++i) {
             for (std::size_t c = 0; c < C_; ++c) {
                 m_[c][i] = cur[i].m[c];

@@ -43,7 +43,7 @@ struct Config {
     uint64_t seed = 0;
     // pmc-standard weights move to the GPU (pcs_pmc_weights) from this
     // population size on; below it the host loop is cheaper than the copies.
-    std::size_t device_weights_min = 2048;
+    std::size_t device_weights_min = 256;
 };
 
 class Estimator {
@@ -80,6 +80,9 @@ private:
     double gamma_ = 0.0;
     bool converged_ = false;
     std::size_t iterations_ = 0, simulations_ = 0;
+    // seconds spent per generation: proposal packing on the host, upload +
+    // plan (pcs_batch_load), GPU run + read-back, weights
+    double tPack_ = 0.0, tLoad_ = 0.0, tGpu_ = 0.0, tWeights_ = 0.0;
     double lowerC_, upperC_;
     pcs_ctx* ctx_ = nullptr;
     pcs_batch* batch_ = nullptr;
