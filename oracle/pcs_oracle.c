@@ -10,11 +10,13 @@
  *   counter = (block, particle, video, sim_id)
  *   block 0             : initial state; words 0,1,2 -> x,y,z uniforms, word 3 -> species
  *   block 0xffffffff    : initial detection uniform (word 0), gaussian detection only
- *   chunk c (4 steps)   : blocks B*c+1 .. B*c+B, B = 3 (hard detection) or 4 (gaussian)
- *     words 0..11 -> six Box-Muller pairs (u1 word 2p, u2 word 2p+1),
+ *   chunk c (4 steps)   : blocks B*c+1 .. B*c+B, B = 2 (hard detection) or 3 (gaussian)
+ *     words 0..5 -> radius uniforms of six Box-Muller pairs (top 24 bits of word p),
+ *     angle of pair p: 16 bits = (low byte of word p) << 8 | byte (p % 4) of word 6 + p/4,
+ *     as a 24-bit turn fraction (angle16 << 8);
  *     pair p gives displacements d[2p] = (sigma*r)*sin, d[2p+1] = (sigma*r)*cos,
  *     step s of the chunk moves by (d[3s], d[3s+1], d[3s+2]);
- *     words 12..15 (gaussian only) -> detection uniform of step s = word 12+s.
+ *     words 8..11 (gaussian only) -> detection uniform of step s = word 8+s.
  * A video of F frames has F-1 steps (frames 2..F in src/cpu/simulate.jl:82).
  */
 #include "pcs_oracle.h"
@@ -123,6 +125,13 @@ static inline void sincos_i(uint32_t w, float* s_out, float* c_out)
 
 void orc_sincos_turn_u24(uint32_t w, float* s, float* c) { sincos_i(w, s, c); }
 
+/* 24-bit turn fraction of Box-Muller pair p's angle (see the stream layout above). */
+static inline uint32_t bm_angle24(uint32_t w_radius, uint32_t w_angle, uint32_t p)
+{
+    return ((w_radius & 0xffu) << 16) | (((w_angle >> (8u * (p & 3u))) & 0xffu) << 8);
+}
+uint32_t orc_bm_angle24(uint32_t w_radius, uint32_t w_angle, uint32_t p) { return bm_angle24(w_radius, w_angle, p); }
+
 void orc_bm_radius_n(const uint32_t* w, float* out, uint64_t n)
 {
     for (uint64_t i = 0; i < n; ++i) out[i] = orc_bm_radius(w[i]);
@@ -148,6 +157,30 @@ static void block(const stream_t* st, uint32_t blk, uint32_t particle, uint32_t 
 }
 
 static inline float u24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
+
+/* Philox blocks per 4-step chunk: 2 for the displacements, +1 for the
+ * Gaussian detection uniforms. */
+static inline uint32_t chunk_blocks(int gauss) { return gauss ? 3u : 2u; }
+
+/* Displacements of chunk c in FP32 (twelve, three per step); the detection
+ * words of the chunk go to wdet when gauss. */
+static void chunk_f32(const stream_t* st, uint32_t c, int gauss, uint32_t part, uint32_t video, float sigma,
+                      float d[12], uint32_t wdet[4])
+{
+    const uint32_t B = chunk_blocks(gauss);
+    uint32_t w[8];
+    block(st, B * c + 1u, part, video, w);
+    block(st, B * c + 2u, part, video, w + 4);
+    for (int pr = 0; pr < 6; ++pr) {
+        float r = bm_radius_i(w[pr]);
+        float sn, cs;
+        sincos_i(bm_angle24(w[pr], w[6 + pr / 4], (uint32_t)pr) << 8, &sn, &cs);
+        float rs = sigma * r;
+        d[2 * pr] = rs * sn;
+        d[2 * pr + 1] = rs * cs;
+    }
+    if (gauss) block(st, B * c + 3u, part, video, wdet);
+}
 
 /* Single periodic wrap (src/gpu/cusim/kernel-simulate-unroll.inl:24-32;
  * src/cpu/position_periodic.jl:1-9 in the centred frame). */
@@ -189,7 +222,6 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
     const float Lx = 2.0f * hx, Ly = 2.0f * hy, Lz = 2.0f * hz;
     const float hax = sys->half_a[0], hay = sys->half_a[1];
     const int gauss = sys->detect_model == 1;
-    const uint32_t B = gauss ? 4u : 3u;
     const float sig2 = sys->detect_sigma * sys->detect_sigma;
     const uint32_t kmin = sys->kmin;
 
@@ -219,19 +251,7 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
         uint32_t wdet[4] = {0, 0, 0, 0};
         for (uint32_t st_i = 0; st_i < steps; ++st_i) {
             uint32_t c = st_i >> 2, sl = st_i & 3u;
-            if (sl == 0) {
-                uint32_t w[12];
-                for (uint32_t b = 0; b < 3; ++b) block(st, B * c + 1u + b, part, video, w + 4 * b);
-                if (gauss) block(st, B * c + 4u, part, video, wdet);
-                for (int pr = 0; pr < 6; ++pr) {
-                    float r = bm_radius_i(w[2 * pr]);
-                    float sn, cs;
-                    sincos_i(w[2 * pr + 1], &sn, &cs);
-                    float rs = sigma * r;
-                    d[2 * pr] = rs * sn;
-                    d[2 * pr + 1] = rs * cs;
-                }
-            }
+            if (sl == 0) chunk_f32(st, c, gauss, part, video, sigma, d, wdet);
             const float dx = d[3 * sl], dy = d[3 * sl + 1], dz = d[3 * sl + 2];
             x = wrap_periodic(x + dx, hx, Lx);
             y = wrap_periodic(y + dy, hy, Ly);
@@ -329,6 +349,20 @@ uint64_t orc_tracks_f32(const orc_system* sys, const orc_proposal* p, uint64_t s
     return t.n;
 }
 
+void orc_chunk_normals_n(uint64_t seed, uint32_t sim_id, uint32_t video, uint32_t n_particles, uint32_t n_chunks,
+                         int gauss, float* out)
+{
+    stream_t st;
+    memset(&st, 0, sizeof(st));
+    st.key[0] = (uint32_t)seed;
+    st.key[1] = (uint32_t)(seed >> 32);
+    st.sim_id = sim_id;
+    uint32_t wdet[4];
+    for (uint32_t part = 0; part < n_particles; ++part)
+        for (uint32_t c = 0; c < n_chunks; ++c)
+            chunk_f32(&st, c, gauss, part, video, 1.0f, out + ((uint64_t)part * n_chunks + c) * 12u, wdet);
+}
+
 /* ------------------------------------------------------------ walk (FP64) */
 
 typedef struct {
@@ -358,11 +392,12 @@ static const double TWO_PI = 6.283185307179586476925286766559;
 static void normals_f64(const stream_t* st, uint32_t c, uint32_t B, uint32_t part, uint32_t video,
                         double sigma, double d[12])
 {
-    uint32_t w[12];
-    for (uint32_t b = 0; b < 3; ++b) block(st, B * c + 1u + b, part, video, w + 4 * b);
+    uint32_t w[8];
+    block(st, B * c + 1u, part, video, w);
+    block(st, B * c + 2u, part, video, w + 4);
     for (int pr = 0; pr < 6; ++pr) {
-        double u1 = (double)((w[2 * pr] >> 8) + 1u) * 0x1p-24;
-        double u2 = (double)(w[2 * pr + 1] >> 8) * 0x1p-24;
+        double u1 = (double)((w[pr] >> 8) + 1u) * 0x1p-24;
+        double u2 = (double)bm_angle24(w[pr], w[6 + pr / 4], (uint32_t)pr) * 0x1p-24;
         double r = sqrt(-2.0 * log(u1));
         d[2 * pr] = sigma * r * sin(TWO_PI * u2);
         d[2 * pr + 1] = sigma * r * cos(TWO_PI * u2);
@@ -385,7 +420,7 @@ static void walk_particle_f64(const stream_t* st, const orc_proposal* p, const o
     const double hx = sys->half_l[0], hy = sys->half_l[1], hz = sys->half_l[2];
     const double hax = sys->half_a[0], hay = sys->half_a[1];
     const int gauss = sys->detect_model == 1;
-    const uint32_t B = gauss ? 4u : 3u;
+    const uint32_t B = chunk_blocks(gauss);
     const double sig2 = (double)sys->detect_sigma * (double)sys->detect_sigma;
 
     uint32_t w0[4];
@@ -418,7 +453,7 @@ static void walk_particle_f64(const stream_t* st, const orc_proposal* p, const o
         uint32_t c = i >> 2, sl = i & 3u;
         if (sl == 0) {
             normals_f64(st, c, B, part, video, sigma, d);
-            if (gauss) block(st, B * c + 4u, part, video, wdet);
+            if (gauss) block(st, B * c + 3u, part, video, wdet);
         }
         const double dx = d[3 * sl], dy = d[3 * sl + 1], dz = d[3 * sl + 2];
         x = wrap64(x + dx, hx, 2.0 * hx, 0);
@@ -467,7 +502,8 @@ void orc_trace_particle(const orc_system* sys, const orc_proposal* p, const orc_
     /* FP32 trajectory: same steps as walk_video_f32, positions only. */
     const uint32_t F = p->frames[video];
     const float hx = sys->half_l[0], hy = sys->half_l[1], hz = sys->half_l[2];
-    const uint32_t B = sys->detect_model == 1 ? 4u : 3u;
+    const int gauss = sys->detect_model == 1;
+    uint32_t wdet[4];
     uint32_t w0[4];
     block(&st, 0u, part, video, w0);
     float x = hx * (2.0f * u24(w0[0]) - 1.0f);
@@ -479,18 +515,7 @@ void orc_trace_particle(const orc_system* sys, const orc_proposal* p, const orc_
     float d[12];
     for (uint32_t i = 0; i + 1 < F; ++i) {
         uint32_t c = i >> 2, sl = i & 3u;
-        if (sl == 0) {
-            uint32_t w[12];
-            for (uint32_t b = 0; b < 3; ++b) block(&st, B * c + 1u + b, part, video, w + 4 * b);
-            for (int pr = 0; pr < 6; ++pr) {
-                float r = orc_bm_radius(w[2 * pr]);
-                float sn, cs;
-                orc_sincos_turn_u24(w[2 * pr + 1], &sn, &cs);
-                float rs = sigma * r;
-                d[2 * pr] = rs * sn;
-                d[2 * pr + 1] = rs * cs;
-            }
-        }
+        if (sl == 0) chunk_f32(&st, c, gauss, part, video, sigma, d, wdet);
         x = wrap_periodic(x + d[3 * sl], hx, 2.0f * hx);
         y = wrap_periodic(y + d[3 * sl + 1], hy, 2.0f * hy);
         z = sys->z_boundary ? wrap_reflect(z + d[3 * sl + 2], hz, 2.0f * hz)

@@ -24,6 +24,8 @@
  *   orc_philox4x32_10     Random123 known-answer vectors + rocRAND header
  *   orc_neg2log_u24       libm FP64 log, ulp bound, all 2^24 inputs
  *   orc_sincos_turn_u24   libm FP64 sin/cos, ulp bound, all 2^24 inputs
+ *   orc_chunk_normals_n   moments, KS against N(0,1), zero correlation between
+ *                         the 12 normals of a chunk; orc_bm_angle24 byte layout
  *   orc_simulate          closed forms (sigma=0 tracks, chi-square DE law,
  *                         detected fraction, Poisson mean), brute-force
  *                         hand-checked tiny cases, FP64 twin agreement
@@ -50,6 +52,15 @@ float orc_neg2log_u24(uint32_t w);
 float orc_bm_radius(uint32_t w);
 /* sin and cos of 2*pi*t, t = (w >> 8) * 2^-24, in the project's FP32 op order. */
 void orc_sincos_turn_u24(uint32_t w, float* s, float* c);
+
+/* Angle of Box-Muller pair p of a chunk as a 24-bit turn fraction: the low
+ * byte of the pair's radius word above byte p % 4 of the chunk's angle word. */
+uint32_t orc_bm_angle24(uint32_t w_radius, uint32_t w_angle, uint32_t p);
+/* Unit displacements (sigma = 1) of chunks 0..n_chunks-1 of particles
+ * 0..n_particles-1 of one video, through the stream layout of pcs_oracle.c:
+ * out[(part * n_chunks + chunk) * 12 + i], i = 3 * step + axis. */
+void orc_chunk_normals_n(uint64_t seed, uint32_t sim_id, uint32_t video, uint32_t n_particles, uint32_t n_chunks,
+                         int gauss, float* out);
 
 /* Batched versions of the math helpers (for exhaustive tests). */
 void orc_bm_radius_n(const uint32_t* w, float* out, uint64_t n);

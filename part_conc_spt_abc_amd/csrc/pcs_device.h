@@ -143,12 +143,12 @@ __device__ __forceinline__ float sqrt_rn_domain(float y)
 // Box-Muller radius sqrt(-2 ln u), correctly rounded.
 __device__ __forceinline__ float bm_radius(uint32_t w) { return sqrt_rn_domain(neg2log_u24(w)); }
 
-// sin and cos of 2 pi t, t = (w >> 8) 2^-24: exact reduction to the nearest
-// quarter turn in integers, polynomial on |t| <= 1/8, quadrant rotation by
-// swap and sign flips.
-__device__ __forceinline__ void sincos_turn_u24(uint32_t w, float& s_out, float& c_out)
+// sin and cos of 2 pi t, t = a 2^-24 for a 24-bit integer a: exact reduction
+// to the nearest quarter turn in integers, polynomial on |t| <= 1/8, quadrant
+// rotation by swap and sign flips.
+__device__ __forceinline__ void sincos_turn24(uint32_t a, float& s_out, float& c_out)
 {
-    const uint32_t j = (w >> 8) + (1u << 21);
+    const uint32_t j = a + (1u << 21);
     const uint32_t q = (j >> 22) & 3u;
     const int32_t r = (int32_t)(j & ((1u << 22) - 1u)) - (1 << 21);
     const float t = (float)r * 0x1p-24f;  // exact
@@ -170,6 +170,20 @@ __device__ __forceinline__ void sincos_turn_u24(uint32_t w, float& s_out, float&
     const uint32_t cb = ((q + 1u) & 2u) << 30;
     s_out = __uint_as_float(__float_as_uint(so) ^ sb);
     c_out = __uint_as_float(__float_as_uint(co) ^ cb);
+}
+
+// The same for t = (w >> 8) 2^-24 (the top 24 bits of a word).
+__device__ __forceinline__ void sincos_turn_u24(uint32_t w, float& s_out, float& c_out)
+{
+    sincos_turn24(w >> 8, s_out, c_out);
+}
+
+// Angle of Box-Muller pair p of a chunk, 16 bits as a 24-bit turn fraction:
+// the low byte of the pair's radius word (whose top 24 bits give the radius
+// uniform) above byte p % 4 of the chunk's angle word.
+__device__ __forceinline__ uint32_t bm_angle24(uint32_t w_radius, uint32_t w_angle, uint32_t p)
+{
+    return ((w_radius << 16) & 0xff0000u) | ((w_angle >> (8u * (p & 3u)) << 8) & 0xff00u);
 }
 
 __device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }

@@ -136,7 +136,9 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
     const float sig2 = sy.detect_sigma * sy.detect_sigma;
     const uint32_t kmin = sy.kmin, n_k = sy.n_k, n_de = sy.n_de, C = sy.n_species;
     const float four_dt = sy.four_dt, dde = sy.de_bin_width;
-    constexpr uint32_t B = GAUSS ? 4u : 3u;
+    // Philox blocks per 4-step chunk: 2 for the 12 displacements (+1 with
+    // Gaussian detection for its 4 uniforms).
+    constexpr uint32_t B = GAUSS ? 3u : 2u;
 
     Particle pt[PPL];
 #pragma unroll
@@ -179,25 +181,26 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
         uint32_t wdet[PPL][4];
 #pragma unroll
         for (int i = 0; i < PPL; ++i) {
+            // Words 0..5: radius uniforms (top 24 bits) of pairs 0..5; words 6, 7:
+            // one angle byte per pair (pairs 0..3 from word 6, 4..5 from word 7).
             const U4 b0 = philox10(B * c + 1u, pt[i].id, video, sim, k0, k1);
             const U4 b1 = philox10(B * c + 2u, pt[i].id, video, sim, k0, k1);
-            const U4 b2 = philox10(B * c + 3u, pt[i].id, video, sim, k0, k1);
-            const uint32_t w[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+            const uint32_t w[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
             for (int p = 0; p < 6; ++p) {
-                const float r = bm_radius(w[2 * p]);
+                const float r = bm_radius(w[p]);
                 float sn, cs;
-                sincos_turn_u24(w[2 * p + 1], sn, cs);
+                sincos_turn24(bm_angle24(w[p], w[6 + p / 4], (uint32_t)p), sn, cs);
                 const float rs = pt[i].sigma * r;
                 d[i][2 * p] = rs * sn;
                 d[i][2 * p + 1] = rs * cs;
             }
             if (GAUSS) {
-                const U4 b3 = philox10(B * c + 4u, pt[i].id, video, sim, k0, k1);
-                wdet[i][0] = b3.x;
-                wdet[i][1] = b3.y;
-                wdet[i][2] = b3.z;
-                wdet[i][3] = b3.w;
+                const U4 b2 = philox10(B * c + 3u, pt[i].id, video, sim, k0, k1);
+                wdet[i][0] = b2.x;
+                wdet[i][1] = b2.y;
+                wdet[i][2] = b2.z;
+                wdet[i][3] = b2.w;
             }
         }
 #pragma unroll

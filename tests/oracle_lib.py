@@ -85,6 +85,9 @@ def lib():
         L.orc_reference_hist.argtypes = [vp, vp, C.c_uint64, C.c_double, C.c_uint32, C.c_uint32, vp, vp]
         L.orc_reference_hist.restype = C.c_uint64
         L.orc_sat_u32.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
+        L.orc_bm_angle24.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_bm_angle24.restype = C.c_uint32
+        L.orc_chunk_normals_n.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, vp]
         _LIB = L
     return _LIB
 
@@ -114,6 +117,13 @@ def sincos_turn(words: np.ndarray):
     c = np.empty(len(w), np.float32)
     lib().orc_sincos_turn_n(_p(w), _p(s), _p(c), len(w))
     return s, c
+
+
+def chunk_normals(seed: int, sim_id: int, video: int, n_particles: int, n_chunks: int, gauss: bool = False):
+    """Unit displacements of the walk's random stream: (n_particles, n_chunks, 12)."""
+    out = np.empty((n_particles, n_chunks, 12), np.float32)
+    lib().orc_chunk_normals_n(seed, sim_id, video, n_particles, n_chunks, int(gauss), _p(out))
+    return out
 
 
 def system_from_abi(sys) -> orc_system:

@@ -98,3 +98,38 @@ def test_box_muller_normals_are_standard():
         assert abs(n.var() - 1.0) < 5e-3
         assert abs((n ** 4).mean() - 3.0) < 3e-2
     assert abs(np.mean(n1 * n2)) < 5e-3
+
+
+def test_angle_layout():
+    """Pair p's 16-bit angle is the low byte of its radius word above byte
+    p % 4 of the chunk's angle word, as a 24-bit turn fraction."""
+    L = O.lib()
+    wr, wa = 0x123456AB, 0xDDCCBBAA
+    assert [L.orc_bm_angle24(wr, wa, p) for p in range(4)] == [0xABAA00, 0xABBB00, 0xABCC00, 0xABDD00]
+    assert L.orc_bm_angle24(0xFFFFFF00, 0xFFFFFFFF, 0) == 0x00FF00  # the radius bits never leak in
+    assert L.orc_bm_angle24(0x000000FF, 0, 1) == 0xFF0000
+
+
+@pytest.mark.parametrize("gauss", [False, True])
+def test_walk_stream_normals(gauss):
+    """The twelve displacements of a chunk, drawn through the walk's stream
+    layout (six radius words, angle bytes shared between pairs), are N(0,1)
+    and mutually uncorrelated, also between pairs that take their angle
+    bytes from the same word."""
+    from scipy import stats
+
+    d = O.chunk_normals(seed=0x5eed1234, sim_id=7, video=3, n_particles=8192, n_chunks=8, gauss=gauss)
+    x = d.reshape(-1, 12).astype(np.float64)
+    n = x.shape[0]
+    tol = 5.0 / np.sqrt(n)
+    assert np.all(np.abs(x.mean(axis=0)) < tol)
+    assert np.all(np.abs(x.var(axis=0) - 1.0) < 5.0 * np.sqrt(2.0 / n))
+    assert np.all(np.abs((x ** 4).mean(axis=0) - 3.0) < 5.0 * np.sqrt(96.0 / n))
+    corr = np.corrcoef(x, rowvar=False)
+    off = corr[~np.eye(12, dtype=bool)]
+    assert np.abs(off).max() < tol
+    # squares are uncorrelated too (sin/cos of one pair share the radius:
+    # their squares correlate through r^2 only as N(0,1) pairs do, zero)
+    csq = np.corrcoef(x ** 2, rowvar=False)[~np.eye(12, dtype=bool)]
+    assert np.abs(csq).max() < 6.0 / np.sqrt(n)
+    assert stats.kstest(x.ravel(), "norm").pvalue > 1e-3
