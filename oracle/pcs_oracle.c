@@ -12,8 +12,8 @@
  *   block 0xffffffff    : initial detection uniform (word 0), gaussian detection only
  *   chunk c (4 steps)   : blocks B*c+1 .. B*c+B, B = 2 (hard detection) or 3 (gaussian)
  *     words 0..5 -> radius uniforms of six Box-Muller pairs (top 24 bits of word p),
- *     angle of pair p: 16 bits = (low byte of word p) << 8 | byte (p % 4) of word 6 + p/4,
- *     as a 24-bit turn fraction (angle16 << 8);
+ *     angle of pair p: 16-bit code a = (low byte of word p) << 8 | byte (p % 4) of
+ *     word 6 + p/4, theta = 2 pi (a + 1/2) 2^-16;
  *     pair p gives displacements d[2p] = (sigma*r)*sin, d[2p+1] = (sigma*r)*cos,
  *     step s of the chunk moves by (d[3s], d[3s+1], d[3s+2]);
  *     words 8..11 (gaussian only) -> detection uniform of step s = word 8+s.
@@ -70,12 +70,77 @@ static const float LN2_HI = 0x1.62e4p-1f;
 static const float LN2_LO = 0x1.7f7d1cp-20f;
 static const float SQRT_HALF = 0x1.6a09e6p-1f;
 
-/* sin(2 pi t) = t * S(t^2), cos(2 pi t) = C(t^2) on |t| <= 1/8 turn. */
-static const float SIN_S[4] = {
-    0x1.921fb6p+2f, -0x1.4abbbap+5f, 0x1.465ec6p+6f, -0x1.2d9bcep+6f,
-};
-static const float COS_C[5] = {
-    0x1p+0f, -0x1.3bd3ccp+4f, 0x1.03c1dep+6f, -0x1.55c664p+6f, 0x1.d9f7bcp+5f,
+/* Box-Muller angle theta = 2 pi (a + 1/2) 2^-16 for the 16-bit code a = i << 8 | r:
+ * the table angle 2 pi (i + 1/2) / 256 (SINCOS_TAB: sin, cos correctly rounded,
+ * fit_math_coeffs.py) rotated by beta = fma(r, ANG_K, ANG_C) = 2 pi (r - 127.5) 2^-16,
+ * with sin beta = beta (1 + ANG_S3 beta^2), cos beta = 1 + beta^2 (ANG_C2 + ANG_C4 beta^2). */
+static const float ANG_K = 0x1.921fb6p-14f, ANG_C = -0x1.908d96p-7f;
+static const float ANG_S3 = -0x1.555556p-3f, ANG_C2 = -0x1p-1f, ANG_C4 = 0x1.555556p-5f;
+static const float SINCOS_TAB[512] = {
+    0x1.921d2p-7f, 0x1.fff622p-1f, 0x1.2d8658p-5f, 0x1.ffa72ep-1f, 0x1.f656e8p-5f, 0x1.ff0956p-1f, 0x1.5f6dp-4f, 0x1.fe1cbp-1f,
+    0x1.c3785cp-4f, 0x1.fce16p-1f, 0x1.139f0cp-3f, 0x1.fb5798p-1f, 0x1.45576cp-3f, 0x1.f97f92p-1f, 0x1.76dd9ep-3f, 0x1.f7599ap-1f,
+    0x1.a82a02p-3f, 0x1.f4e604p-1f, 0x1.d934fep-3f, 0x1.f2253p-1f, 0x1.04fb8p-2f, 0x1.ef178ap-1f, 0x1.1d3444p-2f, 0x1.ebbd8cp-1f,
+    0x1.35410cp-2f, 0x1.e817bap-1f, 0x1.4d1e24p-2f, 0x1.e426a4p-1f, 0x1.64c7dep-2f, 0x1.dfeae6p-1f, 0x1.7c3a94p-2f, 0x1.db6526p-1f,
+    0x1.9372a6p-2f, 0x1.d69618p-1f, 0x1.aa6c82p-2f, 0x1.d17e78p-1f, 0x1.c1249ep-2f, 0x1.cc1f1p-1f, 0x1.d79776p-2f, 0x1.c678b4p-1f,
+    0x1.edc196p-2f, 0x1.c08c42p-1f, 0x1.01cfc8p-1f, 0x1.ba5aa6p-1f, 0x1.0c9704p-1f, 0x1.b3e4d4p-1f, 0x1.1734d6p-1f, 0x1.ad2bcap-1f,
+    0x1.21a79ap-1f, 0x1.a63092p-1f, 0x1.2bedb2p-1f, 0x1.9ef43ep-1f, 0x1.36058cp-1f, 0x1.9777fp-1f, 0x1.3fed96p-1f, 0x1.8fbccap-1f,
+    0x1.49a44ap-1f, 0x1.87c4p-1f, 0x1.53282ap-1f, 0x1.7f8ecep-1f, 0x1.5c77bcp-1f, 0x1.771e76p-1f, 0x1.659192p-1f, 0x1.6e7446p-1f,
+    0x1.6e7446p-1f, 0x1.659192p-1f, 0x1.771e76p-1f, 0x1.5c77bcp-1f, 0x1.7f8ecep-1f, 0x1.53282ap-1f, 0x1.87c4p-1f, 0x1.49a44ap-1f,
+    0x1.8fbccap-1f, 0x1.3fed96p-1f, 0x1.9777fp-1f, 0x1.36058cp-1f, 0x1.9ef43ep-1f, 0x1.2bedb2p-1f, 0x1.a63092p-1f, 0x1.21a79ap-1f,
+    0x1.ad2bcap-1f, 0x1.1734d6p-1f, 0x1.b3e4d4p-1f, 0x1.0c9704p-1f, 0x1.ba5aa6p-1f, 0x1.01cfc8p-1f, 0x1.c08c42p-1f, 0x1.edc196p-2f,
+    0x1.c678b4p-1f, 0x1.d79776p-2f, 0x1.cc1f1p-1f, 0x1.c1249ep-2f, 0x1.d17e78p-1f, 0x1.aa6c82p-2f, 0x1.d69618p-1f, 0x1.9372a6p-2f,
+    0x1.db6526p-1f, 0x1.7c3a94p-2f, 0x1.dfeae6p-1f, 0x1.64c7dep-2f, 0x1.e426a4p-1f, 0x1.4d1e24p-2f, 0x1.e817bap-1f, 0x1.35410cp-2f,
+    0x1.ebbd8cp-1f, 0x1.1d3444p-2f, 0x1.ef178ap-1f, 0x1.04fb8p-2f, 0x1.f2253p-1f, 0x1.d934fep-3f, 0x1.f4e604p-1f, 0x1.a82a02p-3f,
+    0x1.f7599ap-1f, 0x1.76dd9ep-3f, 0x1.f97f92p-1f, 0x1.45576cp-3f, 0x1.fb5798p-1f, 0x1.139f0cp-3f, 0x1.fce16p-1f, 0x1.c3785cp-4f,
+    0x1.fe1cbp-1f, 0x1.5f6dp-4f, 0x1.ff0956p-1f, 0x1.f656e8p-5f, 0x1.ffa72ep-1f, 0x1.2d8658p-5f, 0x1.fff622p-1f, 0x1.921d2p-7f,
+    0x1.fff622p-1f, -0x1.921d2p-7f, 0x1.ffa72ep-1f, -0x1.2d8658p-5f, 0x1.ff0956p-1f, -0x1.f656e8p-5f, 0x1.fe1cbp-1f, -0x1.5f6dp-4f,
+    0x1.fce16p-1f, -0x1.c3785cp-4f, 0x1.fb5798p-1f, -0x1.139f0cp-3f, 0x1.f97f92p-1f, -0x1.45576cp-3f, 0x1.f7599ap-1f, -0x1.76dd9ep-3f,
+    0x1.f4e604p-1f, -0x1.a82a02p-3f, 0x1.f2253p-1f, -0x1.d934fep-3f, 0x1.ef178ap-1f, -0x1.04fb8p-2f, 0x1.ebbd8cp-1f, -0x1.1d3444p-2f,
+    0x1.e817bap-1f, -0x1.35410cp-2f, 0x1.e426a4p-1f, -0x1.4d1e24p-2f, 0x1.dfeae6p-1f, -0x1.64c7dep-2f, 0x1.db6526p-1f, -0x1.7c3a94p-2f,
+    0x1.d69618p-1f, -0x1.9372a6p-2f, 0x1.d17e78p-1f, -0x1.aa6c82p-2f, 0x1.cc1f1p-1f, -0x1.c1249ep-2f, 0x1.c678b4p-1f, -0x1.d79776p-2f,
+    0x1.c08c42p-1f, -0x1.edc196p-2f, 0x1.ba5aa6p-1f, -0x1.01cfc8p-1f, 0x1.b3e4d4p-1f, -0x1.0c9704p-1f, 0x1.ad2bcap-1f, -0x1.1734d6p-1f,
+    0x1.a63092p-1f, -0x1.21a79ap-1f, 0x1.9ef43ep-1f, -0x1.2bedb2p-1f, 0x1.9777fp-1f, -0x1.36058cp-1f, 0x1.8fbccap-1f, -0x1.3fed96p-1f,
+    0x1.87c4p-1f, -0x1.49a44ap-1f, 0x1.7f8ecep-1f, -0x1.53282ap-1f, 0x1.771e76p-1f, -0x1.5c77bcp-1f, 0x1.6e7446p-1f, -0x1.659192p-1f,
+    0x1.659192p-1f, -0x1.6e7446p-1f, 0x1.5c77bcp-1f, -0x1.771e76p-1f, 0x1.53282ap-1f, -0x1.7f8ecep-1f, 0x1.49a44ap-1f, -0x1.87c4p-1f,
+    0x1.3fed96p-1f, -0x1.8fbccap-1f, 0x1.36058cp-1f, -0x1.9777fp-1f, 0x1.2bedb2p-1f, -0x1.9ef43ep-1f, 0x1.21a79ap-1f, -0x1.a63092p-1f,
+    0x1.1734d6p-1f, -0x1.ad2bcap-1f, 0x1.0c9704p-1f, -0x1.b3e4d4p-1f, 0x1.01cfc8p-1f, -0x1.ba5aa6p-1f, 0x1.edc196p-2f, -0x1.c08c42p-1f,
+    0x1.d79776p-2f, -0x1.c678b4p-1f, 0x1.c1249ep-2f, -0x1.cc1f1p-1f, 0x1.aa6c82p-2f, -0x1.d17e78p-1f, 0x1.9372a6p-2f, -0x1.d69618p-1f,
+    0x1.7c3a94p-2f, -0x1.db6526p-1f, 0x1.64c7dep-2f, -0x1.dfeae6p-1f, 0x1.4d1e24p-2f, -0x1.e426a4p-1f, 0x1.35410cp-2f, -0x1.e817bap-1f,
+    0x1.1d3444p-2f, -0x1.ebbd8cp-1f, 0x1.04fb8p-2f, -0x1.ef178ap-1f, 0x1.d934fep-3f, -0x1.f2253p-1f, 0x1.a82a02p-3f, -0x1.f4e604p-1f,
+    0x1.76dd9ep-3f, -0x1.f7599ap-1f, 0x1.45576cp-3f, -0x1.f97f92p-1f, 0x1.139f0cp-3f, -0x1.fb5798p-1f, 0x1.c3785cp-4f, -0x1.fce16p-1f,
+    0x1.5f6dp-4f, -0x1.fe1cbp-1f, 0x1.f656e8p-5f, -0x1.ff0956p-1f, 0x1.2d8658p-5f, -0x1.ffa72ep-1f, 0x1.921d2p-7f, -0x1.fff622p-1f,
+    -0x1.921d2p-7f, -0x1.fff622p-1f, -0x1.2d8658p-5f, -0x1.ffa72ep-1f, -0x1.f656e8p-5f, -0x1.ff0956p-1f, -0x1.5f6dp-4f, -0x1.fe1cbp-1f,
+    -0x1.c3785cp-4f, -0x1.fce16p-1f, -0x1.139f0cp-3f, -0x1.fb5798p-1f, -0x1.45576cp-3f, -0x1.f97f92p-1f, -0x1.76dd9ep-3f, -0x1.f7599ap-1f,
+    -0x1.a82a02p-3f, -0x1.f4e604p-1f, -0x1.d934fep-3f, -0x1.f2253p-1f, -0x1.04fb8p-2f, -0x1.ef178ap-1f, -0x1.1d3444p-2f, -0x1.ebbd8cp-1f,
+    -0x1.35410cp-2f, -0x1.e817bap-1f, -0x1.4d1e24p-2f, -0x1.e426a4p-1f, -0x1.64c7dep-2f, -0x1.dfeae6p-1f, -0x1.7c3a94p-2f, -0x1.db6526p-1f,
+    -0x1.9372a6p-2f, -0x1.d69618p-1f, -0x1.aa6c82p-2f, -0x1.d17e78p-1f, -0x1.c1249ep-2f, -0x1.cc1f1p-1f, -0x1.d79776p-2f, -0x1.c678b4p-1f,
+    -0x1.edc196p-2f, -0x1.c08c42p-1f, -0x1.01cfc8p-1f, -0x1.ba5aa6p-1f, -0x1.0c9704p-1f, -0x1.b3e4d4p-1f, -0x1.1734d6p-1f, -0x1.ad2bcap-1f,
+    -0x1.21a79ap-1f, -0x1.a63092p-1f, -0x1.2bedb2p-1f, -0x1.9ef43ep-1f, -0x1.36058cp-1f, -0x1.9777fp-1f, -0x1.3fed96p-1f, -0x1.8fbccap-1f,
+    -0x1.49a44ap-1f, -0x1.87c4p-1f, -0x1.53282ap-1f, -0x1.7f8ecep-1f, -0x1.5c77bcp-1f, -0x1.771e76p-1f, -0x1.659192p-1f, -0x1.6e7446p-1f,
+    -0x1.6e7446p-1f, -0x1.659192p-1f, -0x1.771e76p-1f, -0x1.5c77bcp-1f, -0x1.7f8ecep-1f, -0x1.53282ap-1f, -0x1.87c4p-1f, -0x1.49a44ap-1f,
+    -0x1.8fbccap-1f, -0x1.3fed96p-1f, -0x1.9777fp-1f, -0x1.36058cp-1f, -0x1.9ef43ep-1f, -0x1.2bedb2p-1f, -0x1.a63092p-1f, -0x1.21a79ap-1f,
+    -0x1.ad2bcap-1f, -0x1.1734d6p-1f, -0x1.b3e4d4p-1f, -0x1.0c9704p-1f, -0x1.ba5aa6p-1f, -0x1.01cfc8p-1f, -0x1.c08c42p-1f, -0x1.edc196p-2f,
+    -0x1.c678b4p-1f, -0x1.d79776p-2f, -0x1.cc1f1p-1f, -0x1.c1249ep-2f, -0x1.d17e78p-1f, -0x1.aa6c82p-2f, -0x1.d69618p-1f, -0x1.9372a6p-2f,
+    -0x1.db6526p-1f, -0x1.7c3a94p-2f, -0x1.dfeae6p-1f, -0x1.64c7dep-2f, -0x1.e426a4p-1f, -0x1.4d1e24p-2f, -0x1.e817bap-1f, -0x1.35410cp-2f,
+    -0x1.ebbd8cp-1f, -0x1.1d3444p-2f, -0x1.ef178ap-1f, -0x1.04fb8p-2f, -0x1.f2253p-1f, -0x1.d934fep-3f, -0x1.f4e604p-1f, -0x1.a82a02p-3f,
+    -0x1.f7599ap-1f, -0x1.76dd9ep-3f, -0x1.f97f92p-1f, -0x1.45576cp-3f, -0x1.fb5798p-1f, -0x1.139f0cp-3f, -0x1.fce16p-1f, -0x1.c3785cp-4f,
+    -0x1.fe1cbp-1f, -0x1.5f6dp-4f, -0x1.ff0956p-1f, -0x1.f656e8p-5f, -0x1.ffa72ep-1f, -0x1.2d8658p-5f, -0x1.fff622p-1f, -0x1.921d2p-7f,
+    -0x1.fff622p-1f, 0x1.921d2p-7f, -0x1.ffa72ep-1f, 0x1.2d8658p-5f, -0x1.ff0956p-1f, 0x1.f656e8p-5f, -0x1.fe1cbp-1f, 0x1.5f6dp-4f,
+    -0x1.fce16p-1f, 0x1.c3785cp-4f, -0x1.fb5798p-1f, 0x1.139f0cp-3f, -0x1.f97f92p-1f, 0x1.45576cp-3f, -0x1.f7599ap-1f, 0x1.76dd9ep-3f,
+    -0x1.f4e604p-1f, 0x1.a82a02p-3f, -0x1.f2253p-1f, 0x1.d934fep-3f, -0x1.ef178ap-1f, 0x1.04fb8p-2f, -0x1.ebbd8cp-1f, 0x1.1d3444p-2f,
+    -0x1.e817bap-1f, 0x1.35410cp-2f, -0x1.e426a4p-1f, 0x1.4d1e24p-2f, -0x1.dfeae6p-1f, 0x1.64c7dep-2f, -0x1.db6526p-1f, 0x1.7c3a94p-2f,
+    -0x1.d69618p-1f, 0x1.9372a6p-2f, -0x1.d17e78p-1f, 0x1.aa6c82p-2f, -0x1.cc1f1p-1f, 0x1.c1249ep-2f, -0x1.c678b4p-1f, 0x1.d79776p-2f,
+    -0x1.c08c42p-1f, 0x1.edc196p-2f, -0x1.ba5aa6p-1f, 0x1.01cfc8p-1f, -0x1.b3e4d4p-1f, 0x1.0c9704p-1f, -0x1.ad2bcap-1f, 0x1.1734d6p-1f,
+    -0x1.a63092p-1f, 0x1.21a79ap-1f, -0x1.9ef43ep-1f, 0x1.2bedb2p-1f, -0x1.9777fp-1f, 0x1.36058cp-1f, -0x1.8fbccap-1f, 0x1.3fed96p-1f,
+    -0x1.87c4p-1f, 0x1.49a44ap-1f, -0x1.7f8ecep-1f, 0x1.53282ap-1f, -0x1.771e76p-1f, 0x1.5c77bcp-1f, -0x1.6e7446p-1f, 0x1.659192p-1f,
+    -0x1.659192p-1f, 0x1.6e7446p-1f, -0x1.5c77bcp-1f, 0x1.771e76p-1f, -0x1.53282ap-1f, 0x1.7f8ecep-1f, -0x1.49a44ap-1f, 0x1.87c4p-1f,
+    -0x1.3fed96p-1f, 0x1.8fbccap-1f, -0x1.36058cp-1f, 0x1.9777fp-1f, -0x1.2bedb2p-1f, 0x1.9ef43ep-1f, -0x1.21a79ap-1f, 0x1.a63092p-1f,
+    -0x1.1734d6p-1f, 0x1.ad2bcap-1f, -0x1.0c9704p-1f, 0x1.b3e4d4p-1f, -0x1.01cfc8p-1f, 0x1.ba5aa6p-1f, -0x1.edc196p-2f, 0x1.c08c42p-1f,
+    -0x1.d79776p-2f, 0x1.c678b4p-1f, -0x1.c1249ep-2f, 0x1.cc1f1p-1f, -0x1.aa6c82p-2f, 0x1.d17e78p-1f, -0x1.9372a6p-2f, 0x1.d69618p-1f,
+    -0x1.7c3a94p-2f, 0x1.db6526p-1f, -0x1.64c7dep-2f, 0x1.dfeae6p-1f, -0x1.4d1e24p-2f, 0x1.e426a4p-1f, -0x1.35410cp-2f, 0x1.e817bap-1f,
+    -0x1.1d3444p-2f, 0x1.ebbd8cp-1f, -0x1.04fb8p-2f, 0x1.ef178ap-1f, -0x1.d934fep-3f, 0x1.f2253p-1f, -0x1.a82a02p-3f, 0x1.f4e604p-1f,
+    -0x1.76dd9ep-3f, 0x1.f7599ap-1f, -0x1.45576cp-3f, 0x1.f97f92p-1f, -0x1.139f0cp-3f, 0x1.fb5798p-1f, -0x1.c3785cp-4f, 0x1.fce16p-1f,
+    -0x1.5f6dp-4f, 0x1.fe1cbp-1f, -0x1.f656e8p-5f, 0x1.ff0956p-1f, -0x1.2d8658p-5f, 0x1.ffa72ep-1f, -0x1.921d2p-7f, 0x1.fff622p-1f,
 };
 
 static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
@@ -102,44 +167,37 @@ float orc_neg2log_u24(uint32_t w) { return neg2log_i(w); }
 static inline float bm_radius_i(uint32_t w) { return sqrtf(neg2log_i(w)); }
 float orc_bm_radius(uint32_t w) { return bm_radius_i(w); }
 
-static inline void sincos_i(uint32_t w, float* s_out, float* c_out)
+/* sin and cos of the angle with code i << 8 | r (i, r bytes). */
+static inline void sincos_i(uint32_t i, uint32_t r, float* s_out, float* c_out)
 {
-    uint32_t j = (w >> 8) + (1u << 21);
-    uint32_t q = (j >> 22) & 3u;
-    int32_t r = (int32_t)(j & ((1u << 22) - 1u)) - (1 << 21);
-    float t = (float)r * 0x1p-24f;            /* exact, |t| <= 1/8 */
-    float t2 = t * t;
-    float sp = SIN_S[3];
-    for (int i = 2; i >= 0; --i) sp = fmaf(sp, t2, SIN_S[i]);
-    float s = t * sp;
-    float cp = COS_C[4];
-    for (int i = 3; i >= 0; --i) cp = fmaf(cp, t2, COS_C[i]);
-    float c = cp;
-    float so = (q & 1u) ? c : s;
-    float co = (q & 1u) ? s : c;
-    uint32_t sb = (q & 2u) ? 0x80000000u : 0u;
-    uint32_t cb = ((q + 1u) & 2u) ? 0x80000000u : 0u;
-    *s_out = u2f(f2u(so) ^ sb);
-    *c_out = u2f(f2u(co) ^ cb);
+    const float sa = SINCOS_TAB[2 * i], ca = SINCOS_TAB[2 * i + 1];
+    float b = fmaf((float)r, ANG_K, ANG_C);
+    float b2 = b * b;
+    float sb = b * fmaf(b2, ANG_S3, 1.0f);
+    float cb = fmaf(b2, fmaf(b2, ANG_C4, ANG_C2), 1.0f);
+    float t1 = ca * sb;
+    float t2 = sa * sb;
+    *s_out = fmaf(sa, cb, t1);
+    *c_out = fmaf(ca, cb, -t2);
 }
 
-void orc_sincos_turn_u24(uint32_t w, float* s, float* c) { sincos_i(w, s, c); }
+void orc_sincos16(uint32_t code, float* s, float* c) { sincos_i((code >> 8) & 0xffu, code & 0xffu, s, c); }
 
-/* 24-bit turn fraction of Box-Muller pair p's angle (see the stream layout above). */
-static inline uint32_t bm_angle24(uint32_t w_radius, uint32_t w_angle, uint32_t p)
+/* 16-bit code of Box-Muller pair p's angle (see the stream layout above). */
+static inline uint32_t bm_angle16(uint32_t w_radius, uint32_t w_angle, uint32_t p)
 {
-    return ((w_radius & 0xffu) << 16) | (((w_angle >> (8u * (p & 3u))) & 0xffu) << 8);
+    return ((w_radius & 0xffu) << 8) | ((w_angle >> (8u * (p & 3u))) & 0xffu);
 }
-uint32_t orc_bm_angle24(uint32_t w_radius, uint32_t w_angle, uint32_t p) { return bm_angle24(w_radius, w_angle, p); }
+uint32_t orc_bm_angle16(uint32_t w_radius, uint32_t w_angle, uint32_t p) { return bm_angle16(w_radius, w_angle, p); }
 
 void orc_bm_radius_n(const uint32_t* w, float* out, uint64_t n)
 {
     for (uint64_t i = 0; i < n; ++i) out[i] = orc_bm_radius(w[i]);
 }
 
-void orc_sincos_turn_n(const uint32_t* w, float* s, float* c, uint64_t n)
+void orc_sincos_angle_n(const uint32_t* w, float* s, float* c, uint64_t n)
 {
-    for (uint64_t i = 0; i < n; ++i) orc_sincos_turn_u24(w[i], &s[i], &c[i]);
+    for (uint64_t i = 0; i < n; ++i) orc_sincos16(w[i] >> 16, &s[i], &c[i]);
 }
 
 /* ------------------------------------------------------------ walk (FP32) */
@@ -174,7 +232,8 @@ static void chunk_f32(const stream_t* st, uint32_t c, int gauss, uint32_t part, 
     for (int pr = 0; pr < 6; ++pr) {
         float r = bm_radius_i(w[pr]);
         float sn, cs;
-        sincos_i(bm_angle24(w[pr], w[6 + pr / 4], (uint32_t)pr) << 8, &sn, &cs);
+        const uint32_t a = bm_angle16(w[pr], w[6 + pr / 4], (uint32_t)pr);
+        sincos_i(a >> 8, a & 0xffu, &sn, &cs);
         float rs = sigma * r;
         d[2 * pr] = rs * sn;
         d[2 * pr + 1] = rs * cs;
@@ -397,7 +456,7 @@ static void normals_f64(const stream_t* st, uint32_t c, uint32_t B, uint32_t par
     block(st, B * c + 2u, part, video, w + 4);
     for (int pr = 0; pr < 6; ++pr) {
         double u1 = (double)((w[pr] >> 8) + 1u) * 0x1p-24;
-        double u2 = (double)bm_angle24(w[pr], w[6 + pr / 4], (uint32_t)pr) * 0x1p-24;
+        double u2 = ((double)bm_angle16(w[pr], w[6 + pr / 4], (uint32_t)pr) + 0.5) * 0x1p-16;
         double r = sqrt(-2.0 * log(u1));
         d[2 * pr] = sigma * r * sin(TWO_PI * u2);
         d[2 * pr + 1] = sigma * r * cos(TWO_PI * u2);

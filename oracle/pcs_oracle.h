@@ -23,9 +23,10 @@
  * Pinning status (what checks each function; see tests/test_oracle_*.py):
  *   orc_philox4x32_10     Random123 known-answer vectors + rocRAND header
  *   orc_neg2log_u24       libm FP64 log, ulp bound, all 2^24 inputs
- *   orc_sincos_turn_u24   libm FP64 sin/cos, ulp bound, all 2^24 inputs
+ *   orc_sincos16          libm FP64 sin/cos, error bound, all 2^16 codes; the
+ *                         table against 50-digit values (fit_math_coeffs.py)
  *   orc_chunk_normals_n   moments, KS against N(0,1), zero correlation between
- *                         the 12 normals of a chunk; orc_bm_angle24 byte layout
+ *                         the 12 normals of a chunk; orc_bm_angle16 byte layout
  *   orc_simulate          closed forms (sigma=0 tracks, chi-square DE law,
  *                         detected fraction, Poisson mean), brute-force
  *                         hand-checked tiny cases, FP64 twin agreement
@@ -50,12 +51,13 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 float orc_neg2log_u24(uint32_t w);
 /* Box-Muller radius sqrt(-2 ln u) in FP32 (correctly rounded sqrt of the above). */
 float orc_bm_radius(uint32_t w);
-/* sin and cos of 2*pi*t, t = (w >> 8) * 2^-24, in the project's FP32 op order. */
-void orc_sincos_turn_u24(uint32_t w, float* s, float* c);
+/* sin and cos of the Box-Muller angle 2 pi (code + 1/2) 2^-16 for a 16-bit
+ * code, in the project's FP32 op order (table + rotation, see pcs_oracle.c). */
+void orc_sincos16(uint32_t code, float* s, float* c);
 
-/* Angle of Box-Muller pair p of a chunk as a 24-bit turn fraction: the low
- * byte of the pair's radius word above byte p % 4 of the chunk's angle word. */
-uint32_t orc_bm_angle24(uint32_t w_radius, uint32_t w_angle, uint32_t p);
+/* 16-bit angle code of Box-Muller pair p of a chunk: the low byte of the
+ * pair's radius word above byte p % 4 of the chunk's angle word. */
+uint32_t orc_bm_angle16(uint32_t w_radius, uint32_t w_angle, uint32_t p);
 /* Unit displacements (sigma = 1) of chunks 0..n_chunks-1 of particles
  * 0..n_particles-1 of one video, through the stream layout of pcs_oracle.c:
  * out[(part * n_chunks + chunk) * 12 + i], i = 3 * step + axis. */
@@ -64,7 +66,7 @@ void orc_chunk_normals_n(uint64_t seed, uint32_t sim_id, uint32_t video, uint32_
 
 /* Batched versions of the math helpers (for exhaustive tests). */
 void orc_bm_radius_n(const uint32_t* w, float* out, uint64_t n);
-void orc_sincos_turn_n(const uint32_t* w, float* s, float* c, uint64_t n);
+void orc_sincos_angle_n(const uint32_t* w, float* s, float* c, uint64_t n);  /* code = w >> 16 */
 
 /* Model description shared by all simulations of one ABC run. */
 typedef struct orc_system {

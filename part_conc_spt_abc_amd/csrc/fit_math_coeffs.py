@@ -3,14 +3,16 @@
 (pcs_device.h; the oracle carries the same constants) and prints them as
 hex floats.
 
-- ln(1+f) = f + f^2 P(f), P of degree 7, on [sqrt(1/2)-1, sqrt(2)-1];
-- sin(2 pi t) = t S(t^2), S of degree 3, and cos(2 pi t) = 1 + t^2 C(t^2),
-  C of degree 3, on |t| <= 1/8 turn.
-
-Method: weighted least squares on Chebyshev nodes, reweighted iteratively
-by the residual (Lawson's algorithm) to approach the minimax polynomial,
-then rounded to FP32.
+- ln(1+f) = f + f^2 P(f), P of degree 7, on [sqrt(1/2)-1, sqrt(2)-1]
+  (weighted least squares on Chebyshev nodes, reweighted iteratively by the
+  residual - Lawson's algorithm - to approach the minimax polynomial, then
+  rounded to FP32);
+- the Box-Muller angle table: sin and cos of 2 pi (i + 1/2) / 256 for
+  i = 0..255, computed to 50 digits and rounded to nearest-even FP32, and
+  the constants of the rotation by the remaining angle (2 pi / 2^16, its
+  offset -127.5 * 2 pi / 2^16, -1/6, -1/2, 1/24 in FP32).
 """
+import mpmath
 import numpy as np
 
 
@@ -33,20 +35,45 @@ def hexf(v):
     return float.hex(float(np.float32(v))).replace("0000000p", "p")
 
 
+def f32_rn(x):
+    """Nearest-even FP32 value of the mpmath number x, as a Python float."""
+    if x == 0:
+        return 0.0
+    sign = -1.0 if x < 0 else 1.0
+    ax = abs(x)
+    e = int(mpmath.floor(mpmath.log(ax, 2)))
+    if mpmath.mpf(2) ** e > ax:  # guard the floor of the logarithm
+        e -= 1
+    elif mpmath.mpf(2) ** (e + 1) <= ax:
+        e += 1
+    m = ax * mpmath.mpf(2) ** (23 - e)  # in [2^23, 2^24)
+    fl = int(mpmath.floor(m))
+    rem = m - fl
+    if rem > 0.5 or (rem == 0.5 and fl % 2 == 1):
+        fl += 1
+    return sign * float(fl) * 2.0 ** (e - 23)
+
+
+def sincos_table():
+    mpmath.mp.dps = 50
+    tab = []
+    for i in range(256):
+        th = 2 * mpmath.pi * (i + mpmath.mpf("0.5")) / 256
+        tab += [f32_rn(mpmath.sin(th)), f32_rn(mpmath.cos(th))]
+    k = 2 * mpmath.pi / 65536
+    consts = [f32_rn(k), f32_rn(-mpmath.mpf("127.5") * k), f32_rn(mpmath.mpf(-1) / 6), -0.5, f32_rn(mpmath.mpf(1) / 24)]
+    return tab, consts
+
+
 def main():
     a, b = np.sqrt(0.5) - 1, np.sqrt(2) - 1
     x = cheb(a, b)
     A = np.stack([x ** p for p in range(8)], 1)
     logp = lawson(A, (np.log1p(x) - x) / x ** 2, np.abs(x) + 1e-12, 80).astype(np.float32)
-    t = cheb(0, 0.125)
-    t2 = t * t
-    A = np.stack([t2 ** p for p in range(4)], 1)
-    sinp = lawson(A, np.sin(2 * np.pi * t) / t, np.ones_like(t), 100).astype(np.float32)
-    A = np.stack([t2 ** p for p in range(1, 5)], 1)
-    cosp = lawson(A, np.cos(2 * np.pi * t) - 1, np.ones_like(t), 100).astype(np.float32)
     print("LOG_P", [hexf(v) for v in logp])
-    print("SIN_S", [hexf(v) for v in sinp])
-    print("COS_C", ["0x1p+0"] + [hexf(v) for v in cosp])
+    tab, consts = sincos_table()
+    print("ANGLE_K", [float.hex(v) for v in consts])
+    print("SINCOS_TAB", [float.hex(v) for v in tab])
 
 
 if __name__ == "__main__":

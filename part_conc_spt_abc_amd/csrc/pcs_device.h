@@ -90,11 +90,12 @@ constexpr float kLogP0 = -0x1.fffff4p-2f, kLogP1 = 0x1.5556e8p-2f, kLogP2 = -0x1
 constexpr float kLn2Hi = 0x1.62e4p-1f;
 constexpr float kLn2Lo = 0x1.7f7d1cp-20f;
 constexpr float kSqrtHalf = 0x1.6a09e6p-1f;
-// sin(2 pi t) = t S(t^2), cos(2 pi t) = C(t^2) for |t| <= 1/8 turn.
-constexpr float kSin0 = 0x1.921fb6p+2f, kSin1 = -0x1.4abbbap+5f, kSin2 = 0x1.465ec6p+6f,
-                kSin3 = -0x1.2d9bcep+6f;
-constexpr float kCos1 = -0x1.3bd3ccp+4f, kCos2 = 0x1.03c1dep+6f, kCos3 = -0x1.55c664p+6f,
-                kCos4 = 0x1.d9f7bcp+5f;
+// Box-Muller angle: theta = 2 pi (a + 1/2) 2^-16 for a 16-bit code a = i << 8 | r,
+// taken as the table angle 2 pi (i + 1/2) / 256 (sin/cos from the 256-entry
+// table kSinCosTab of pcs_kernels.hip, correctly rounded) plus beta = 2 pi (r - 127.5) 2^-16,
+// |beta| < 0.0123, computed as fma(r, kAngK, kAngC).
+constexpr float kAngK = 0x1.921fb6p-14f, kAngC = -0x1.908d96p-7f;
+constexpr float kAngS3 = -0x1.555556p-3f, kAngC2 = -0x1p-1f, kAngC4 = 0x1.555556p-5f;
 
 // -2 ln(u) for u = ((w >> 8) + 1) 2^-24 in (0, 1].
 __device__ __forceinline__ float neg2log_u24(uint32_t w)
@@ -143,47 +144,19 @@ __device__ __forceinline__ float sqrt_rn_domain(float y)
 // Box-Muller radius sqrt(-2 ln u), correctly rounded.
 __device__ __forceinline__ float bm_radius(uint32_t w) { return sqrt_rn_domain(neg2log_u24(w)); }
 
-// sin and cos of 2 pi t, t = a 2^-24 for a 24-bit integer a: exact reduction
-// to the nearest quarter turn in integers, polynomial on |t| <= 1/8, quadrant
-// rotation by swap and sign flips.
-__device__ __forceinline__ void sincos_turn24(uint32_t a, float& s_out, float& c_out)
+// sin and cos of the Box-Muller angle with code i << 8 | r (i, r bytes) from
+// the table in LDS (tab[i] = (sin, cos) of 2 pi (i + 1/2) / 256): rotation by
+// beta with sin beta = beta (1 - beta^2 / 6), cos beta = 1 - beta^2 / 2 + beta^4 / 24
+// (the next terms are below 1e-9 relative).
+__device__ __forceinline__ void sincos_tab(const float2* tab, uint32_t i, uint32_t r, float& s_out, float& c_out)
 {
-    const uint32_t j = a + (1u << 21);
-    const uint32_t q = (j >> 22) & 3u;
-    const int32_t r = (int32_t)(j & ((1u << 22) - 1u)) - (1 << 21);
-    const float t = (float)r * 0x1p-24f;  // exact
-    const float t2 = t * t;
-    float sp = kSin3;
-    sp = __builtin_fmaf(sp, t2, kSin2);
-    sp = __builtin_fmaf(sp, t2, kSin1);
-    sp = __builtin_fmaf(sp, t2, kSin0);
-    const float s = t * sp;
-    float c = kCos4;
-    c = __builtin_fmaf(c, t2, kCos3);
-    c = __builtin_fmaf(c, t2, kCos2);
-    c = __builtin_fmaf(c, t2, kCos1);
-    c = __builtin_fmaf(c, t2, 1.0f);
-    const bool swap = (q & 1u) != 0u;
-    const float so = swap ? c : s;
-    const float co = swap ? s : c;
-    const uint32_t sb = (q & 2u) << 30;
-    const uint32_t cb = ((q + 1u) & 2u) << 30;
-    s_out = __uint_as_float(__float_as_uint(so) ^ sb);
-    c_out = __uint_as_float(__float_as_uint(co) ^ cb);
-}
-
-// The same for t = (w >> 8) 2^-24 (the top 24 bits of a word).
-__device__ __forceinline__ void sincos_turn_u24(uint32_t w, float& s_out, float& c_out)
-{
-    sincos_turn24(w >> 8, s_out, c_out);
-}
-
-// Angle of Box-Muller pair p of a chunk, 16 bits as a 24-bit turn fraction:
-// the low byte of the pair's radius word (whose top 24 bits give the radius
-// uniform) above byte p % 4 of the chunk's angle word.
-__device__ __forceinline__ uint32_t bm_angle24(uint32_t w_radius, uint32_t w_angle, uint32_t p)
-{
-    return ((w_radius << 16) & 0xff0000u) | ((w_angle >> (8u * (p & 3u)) << 8) & 0xff00u);
+    const float2 t = tab[i];
+    const float b = __builtin_fmaf((float)r, kAngK, kAngC);
+    const float b2 = b * b;
+    const float sb = b * __builtin_fmaf(b2, kAngS3, 1.0f);
+    const float cb = __builtin_fmaf(b2, __builtin_fmaf(b2, kAngC4, kAngC2), 1.0f);
+    s_out = __builtin_fmaf(t.x, cb, t.y * sb);
+    c_out = __builtin_fmaf(t.y, cb, -(t.x * sb));
 }
 
 __device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }

@@ -43,6 +43,84 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ------------------------------------------------------------ angle table
+
+// (sin, cos) of 2 pi (i + 1/2) / 256, i = 0..255, correctly rounded to FP32
+// (fit_math_coeffs.py). Every workgroup that walks copies it into LDS.
+constexpr int kSinCosTabFloats = 512;
+__constant__ float kSinCosTab[kSinCosTabFloats] = {
+    0x1.921d2p-7f, 0x1.fff622p-1f, 0x1.2d8658p-5f, 0x1.ffa72ep-1f, 0x1.f656e8p-5f, 0x1.ff0956p-1f, 0x1.5f6dp-4f, 0x1.fe1cbp-1f,
+    0x1.c3785cp-4f, 0x1.fce16p-1f, 0x1.139f0cp-3f, 0x1.fb5798p-1f, 0x1.45576cp-3f, 0x1.f97f92p-1f, 0x1.76dd9ep-3f, 0x1.f7599ap-1f,
+    0x1.a82a02p-3f, 0x1.f4e604p-1f, 0x1.d934fep-3f, 0x1.f2253p-1f, 0x1.04fb8p-2f, 0x1.ef178ap-1f, 0x1.1d3444p-2f, 0x1.ebbd8cp-1f,
+    0x1.35410cp-2f, 0x1.e817bap-1f, 0x1.4d1e24p-2f, 0x1.e426a4p-1f, 0x1.64c7dep-2f, 0x1.dfeae6p-1f, 0x1.7c3a94p-2f, 0x1.db6526p-1f,
+    0x1.9372a6p-2f, 0x1.d69618p-1f, 0x1.aa6c82p-2f, 0x1.d17e78p-1f, 0x1.c1249ep-2f, 0x1.cc1f1p-1f, 0x1.d79776p-2f, 0x1.c678b4p-1f,
+    0x1.edc196p-2f, 0x1.c08c42p-1f, 0x1.01cfc8p-1f, 0x1.ba5aa6p-1f, 0x1.0c9704p-1f, 0x1.b3e4d4p-1f, 0x1.1734d6p-1f, 0x1.ad2bcap-1f,
+    0x1.21a79ap-1f, 0x1.a63092p-1f, 0x1.2bedb2p-1f, 0x1.9ef43ep-1f, 0x1.36058cp-1f, 0x1.9777fp-1f, 0x1.3fed96p-1f, 0x1.8fbccap-1f,
+    0x1.49a44ap-1f, 0x1.87c4p-1f, 0x1.53282ap-1f, 0x1.7f8ecep-1f, 0x1.5c77bcp-1f, 0x1.771e76p-1f, 0x1.659192p-1f, 0x1.6e7446p-1f,
+    0x1.6e7446p-1f, 0x1.659192p-1f, 0x1.771e76p-1f, 0x1.5c77bcp-1f, 0x1.7f8ecep-1f, 0x1.53282ap-1f, 0x1.87c4p-1f, 0x1.49a44ap-1f,
+    0x1.8fbccap-1f, 0x1.3fed96p-1f, 0x1.9777fp-1f, 0x1.36058cp-1f, 0x1.9ef43ep-1f, 0x1.2bedb2p-1f, 0x1.a63092p-1f, 0x1.21a79ap-1f,
+    0x1.ad2bcap-1f, 0x1.1734d6p-1f, 0x1.b3e4d4p-1f, 0x1.0c9704p-1f, 0x1.ba5aa6p-1f, 0x1.01cfc8p-1f, 0x1.c08c42p-1f, 0x1.edc196p-2f,
+    0x1.c678b4p-1f, 0x1.d79776p-2f, 0x1.cc1f1p-1f, 0x1.c1249ep-2f, 0x1.d17e78p-1f, 0x1.aa6c82p-2f, 0x1.d69618p-1f, 0x1.9372a6p-2f,
+    0x1.db6526p-1f, 0x1.7c3a94p-2f, 0x1.dfeae6p-1f, 0x1.64c7dep-2f, 0x1.e426a4p-1f, 0x1.4d1e24p-2f, 0x1.e817bap-1f, 0x1.35410cp-2f,
+    0x1.ebbd8cp-1f, 0x1.1d3444p-2f, 0x1.ef178ap-1f, 0x1.04fb8p-2f, 0x1.f2253p-1f, 0x1.d934fep-3f, 0x1.f4e604p-1f, 0x1.a82a02p-3f,
+    0x1.f7599ap-1f, 0x1.76dd9ep-3f, 0x1.f97f92p-1f, 0x1.45576cp-3f, 0x1.fb5798p-1f, 0x1.139f0cp-3f, 0x1.fce16p-1f, 0x1.c3785cp-4f,
+    0x1.fe1cbp-1f, 0x1.5f6dp-4f, 0x1.ff0956p-1f, 0x1.f656e8p-5f, 0x1.ffa72ep-1f, 0x1.2d8658p-5f, 0x1.fff622p-1f, 0x1.921d2p-7f,
+    0x1.fff622p-1f, -0x1.921d2p-7f, 0x1.ffa72ep-1f, -0x1.2d8658p-5f, 0x1.ff0956p-1f, -0x1.f656e8p-5f, 0x1.fe1cbp-1f, -0x1.5f6dp-4f,
+    0x1.fce16p-1f, -0x1.c3785cp-4f, 0x1.fb5798p-1f, -0x1.139f0cp-3f, 0x1.f97f92p-1f, -0x1.45576cp-3f, 0x1.f7599ap-1f, -0x1.76dd9ep-3f,
+    0x1.f4e604p-1f, -0x1.a82a02p-3f, 0x1.f2253p-1f, -0x1.d934fep-3f, 0x1.ef178ap-1f, -0x1.04fb8p-2f, 0x1.ebbd8cp-1f, -0x1.1d3444p-2f,
+    0x1.e817bap-1f, -0x1.35410cp-2f, 0x1.e426a4p-1f, -0x1.4d1e24p-2f, 0x1.dfeae6p-1f, -0x1.64c7dep-2f, 0x1.db6526p-1f, -0x1.7c3a94p-2f,
+    0x1.d69618p-1f, -0x1.9372a6p-2f, 0x1.d17e78p-1f, -0x1.aa6c82p-2f, 0x1.cc1f1p-1f, -0x1.c1249ep-2f, 0x1.c678b4p-1f, -0x1.d79776p-2f,
+    0x1.c08c42p-1f, -0x1.edc196p-2f, 0x1.ba5aa6p-1f, -0x1.01cfc8p-1f, 0x1.b3e4d4p-1f, -0x1.0c9704p-1f, 0x1.ad2bcap-1f, -0x1.1734d6p-1f,
+    0x1.a63092p-1f, -0x1.21a79ap-1f, 0x1.9ef43ep-1f, -0x1.2bedb2p-1f, 0x1.9777fp-1f, -0x1.36058cp-1f, 0x1.8fbccap-1f, -0x1.3fed96p-1f,
+    0x1.87c4p-1f, -0x1.49a44ap-1f, 0x1.7f8ecep-1f, -0x1.53282ap-1f, 0x1.771e76p-1f, -0x1.5c77bcp-1f, 0x1.6e7446p-1f, -0x1.659192p-1f,
+    0x1.659192p-1f, -0x1.6e7446p-1f, 0x1.5c77bcp-1f, -0x1.771e76p-1f, 0x1.53282ap-1f, -0x1.7f8ecep-1f, 0x1.49a44ap-1f, -0x1.87c4p-1f,
+    0x1.3fed96p-1f, -0x1.8fbccap-1f, 0x1.36058cp-1f, -0x1.9777fp-1f, 0x1.2bedb2p-1f, -0x1.9ef43ep-1f, 0x1.21a79ap-1f, -0x1.a63092p-1f,
+    0x1.1734d6p-1f, -0x1.ad2bcap-1f, 0x1.0c9704p-1f, -0x1.b3e4d4p-1f, 0x1.01cfc8p-1f, -0x1.ba5aa6p-1f, 0x1.edc196p-2f, -0x1.c08c42p-1f,
+    0x1.d79776p-2f, -0x1.c678b4p-1f, 0x1.c1249ep-2f, -0x1.cc1f1p-1f, 0x1.aa6c82p-2f, -0x1.d17e78p-1f, 0x1.9372a6p-2f, -0x1.d69618p-1f,
+    0x1.7c3a94p-2f, -0x1.db6526p-1f, 0x1.64c7dep-2f, -0x1.dfeae6p-1f, 0x1.4d1e24p-2f, -0x1.e426a4p-1f, 0x1.35410cp-2f, -0x1.e817bap-1f,
+    0x1.1d3444p-2f, -0x1.ebbd8cp-1f, 0x1.04fb8p-2f, -0x1.ef178ap-1f, 0x1.d934fep-3f, -0x1.f2253p-1f, 0x1.a82a02p-3f, -0x1.f4e604p-1f,
+    0x1.76dd9ep-3f, -0x1.f7599ap-1f, 0x1.45576cp-3f, -0x1.f97f92p-1f, 0x1.139f0cp-3f, -0x1.fb5798p-1f, 0x1.c3785cp-4f, -0x1.fce16p-1f,
+    0x1.5f6dp-4f, -0x1.fe1cbp-1f, 0x1.f656e8p-5f, -0x1.ff0956p-1f, 0x1.2d8658p-5f, -0x1.ffa72ep-1f, 0x1.921d2p-7f, -0x1.fff622p-1f,
+    -0x1.921d2p-7f, -0x1.fff622p-1f, -0x1.2d8658p-5f, -0x1.ffa72ep-1f, -0x1.f656e8p-5f, -0x1.ff0956p-1f, -0x1.5f6dp-4f, -0x1.fe1cbp-1f,
+    -0x1.c3785cp-4f, -0x1.fce16p-1f, -0x1.139f0cp-3f, -0x1.fb5798p-1f, -0x1.45576cp-3f, -0x1.f97f92p-1f, -0x1.76dd9ep-3f, -0x1.f7599ap-1f,
+    -0x1.a82a02p-3f, -0x1.f4e604p-1f, -0x1.d934fep-3f, -0x1.f2253p-1f, -0x1.04fb8p-2f, -0x1.ef178ap-1f, -0x1.1d3444p-2f, -0x1.ebbd8cp-1f,
+    -0x1.35410cp-2f, -0x1.e817bap-1f, -0x1.4d1e24p-2f, -0x1.e426a4p-1f, -0x1.64c7dep-2f, -0x1.dfeae6p-1f, -0x1.7c3a94p-2f, -0x1.db6526p-1f,
+    -0x1.9372a6p-2f, -0x1.d69618p-1f, -0x1.aa6c82p-2f, -0x1.d17e78p-1f, -0x1.c1249ep-2f, -0x1.cc1f1p-1f, -0x1.d79776p-2f, -0x1.c678b4p-1f,
+    -0x1.edc196p-2f, -0x1.c08c42p-1f, -0x1.01cfc8p-1f, -0x1.ba5aa6p-1f, -0x1.0c9704p-1f, -0x1.b3e4d4p-1f, -0x1.1734d6p-1f, -0x1.ad2bcap-1f,
+    -0x1.21a79ap-1f, -0x1.a63092p-1f, -0x1.2bedb2p-1f, -0x1.9ef43ep-1f, -0x1.36058cp-1f, -0x1.9777fp-1f, -0x1.3fed96p-1f, -0x1.8fbccap-1f,
+    -0x1.49a44ap-1f, -0x1.87c4p-1f, -0x1.53282ap-1f, -0x1.7f8ecep-1f, -0x1.5c77bcp-1f, -0x1.771e76p-1f, -0x1.659192p-1f, -0x1.6e7446p-1f,
+    -0x1.6e7446p-1f, -0x1.659192p-1f, -0x1.771e76p-1f, -0x1.5c77bcp-1f, -0x1.7f8ecep-1f, -0x1.53282ap-1f, -0x1.87c4p-1f, -0x1.49a44ap-1f,
+    -0x1.8fbccap-1f, -0x1.3fed96p-1f, -0x1.9777fp-1f, -0x1.36058cp-1f, -0x1.9ef43ep-1f, -0x1.2bedb2p-1f, -0x1.a63092p-1f, -0x1.21a79ap-1f,
+    -0x1.ad2bcap-1f, -0x1.1734d6p-1f, -0x1.b3e4d4p-1f, -0x1.0c9704p-1f, -0x1.ba5aa6p-1f, -0x1.01cfc8p-1f, -0x1.c08c42p-1f, -0x1.edc196p-2f,
+    -0x1.c678b4p-1f, -0x1.d79776p-2f, -0x1.cc1f1p-1f, -0x1.c1249ep-2f, -0x1.d17e78p-1f, -0x1.aa6c82p-2f, -0x1.d69618p-1f, -0x1.9372a6p-2f,
+    -0x1.db6526p-1f, -0x1.7c3a94p-2f, -0x1.dfeae6p-1f, -0x1.64c7dep-2f, -0x1.e426a4p-1f, -0x1.4d1e24p-2f, -0x1.e817bap-1f, -0x1.35410cp-2f,
+    -0x1.ebbd8cp-1f, -0x1.1d3444p-2f, -0x1.ef178ap-1f, -0x1.04fb8p-2f, -0x1.f2253p-1f, -0x1.d934fep-3f, -0x1.f4e604p-1f, -0x1.a82a02p-3f,
+    -0x1.f7599ap-1f, -0x1.76dd9ep-3f, -0x1.f97f92p-1f, -0x1.45576cp-3f, -0x1.fb5798p-1f, -0x1.139f0cp-3f, -0x1.fce16p-1f, -0x1.c3785cp-4f,
+    -0x1.fe1cbp-1f, -0x1.5f6dp-4f, -0x1.ff0956p-1f, -0x1.f656e8p-5f, -0x1.ffa72ep-1f, -0x1.2d8658p-5f, -0x1.fff622p-1f, -0x1.921d2p-7f,
+    -0x1.fff622p-1f, 0x1.921d2p-7f, -0x1.ffa72ep-1f, 0x1.2d8658p-5f, -0x1.ff0956p-1f, 0x1.f656e8p-5f, -0x1.fe1cbp-1f, 0x1.5f6dp-4f,
+    -0x1.fce16p-1f, 0x1.c3785cp-4f, -0x1.fb5798p-1f, 0x1.139f0cp-3f, -0x1.f97f92p-1f, 0x1.45576cp-3f, -0x1.f7599ap-1f, 0x1.76dd9ep-3f,
+    -0x1.f4e604p-1f, 0x1.a82a02p-3f, -0x1.f2253p-1f, 0x1.d934fep-3f, -0x1.ef178ap-1f, 0x1.04fb8p-2f, -0x1.ebbd8cp-1f, 0x1.1d3444p-2f,
+    -0x1.e817bap-1f, 0x1.35410cp-2f, -0x1.e426a4p-1f, 0x1.4d1e24p-2f, -0x1.dfeae6p-1f, 0x1.64c7dep-2f, -0x1.db6526p-1f, 0x1.7c3a94p-2f,
+    -0x1.d69618p-1f, 0x1.9372a6p-2f, -0x1.d17e78p-1f, 0x1.aa6c82p-2f, -0x1.cc1f1p-1f, 0x1.c1249ep-2f, -0x1.c678b4p-1f, 0x1.d79776p-2f,
+    -0x1.c08c42p-1f, 0x1.edc196p-2f, -0x1.ba5aa6p-1f, 0x1.01cfc8p-1f, -0x1.b3e4d4p-1f, 0x1.0c9704p-1f, -0x1.ad2bcap-1f, 0x1.1734d6p-1f,
+    -0x1.a63092p-1f, 0x1.21a79ap-1f, -0x1.9ef43ep-1f, 0x1.2bedb2p-1f, -0x1.9777fp-1f, 0x1.36058cp-1f, -0x1.8fbccap-1f, 0x1.3fed96p-1f,
+    -0x1.87c4p-1f, 0x1.49a44ap-1f, -0x1.7f8ecep-1f, 0x1.53282ap-1f, -0x1.771e76p-1f, 0x1.5c77bcp-1f, -0x1.6e7446p-1f, 0x1.659192p-1f,
+    -0x1.659192p-1f, 0x1.6e7446p-1f, -0x1.5c77bcp-1f, 0x1.771e76p-1f, -0x1.53282ap-1f, 0x1.7f8ecep-1f, -0x1.49a44ap-1f, 0x1.87c4p-1f,
+    -0x1.3fed96p-1f, 0x1.8fbccap-1f, -0x1.36058cp-1f, 0x1.9777fp-1f, -0x1.2bedb2p-1f, 0x1.9ef43ep-1f, -0x1.21a79ap-1f, 0x1.a63092p-1f,
+    -0x1.1734d6p-1f, 0x1.ad2bcap-1f, -0x1.0c9704p-1f, 0x1.b3e4d4p-1f, -0x1.01cfc8p-1f, 0x1.ba5aa6p-1f, -0x1.edc196p-2f, 0x1.c08c42p-1f,
+    -0x1.d79776p-2f, 0x1.c678b4p-1f, -0x1.c1249ep-2f, 0x1.cc1f1p-1f, -0x1.aa6c82p-2f, 0x1.d17e78p-1f, -0x1.9372a6p-2f, 0x1.d69618p-1f,
+    -0x1.7c3a94p-2f, 0x1.db6526p-1f, -0x1.64c7dep-2f, 0x1.dfeae6p-1f, -0x1.4d1e24p-2f, 0x1.e426a4p-1f, -0x1.35410cp-2f, 0x1.e817bap-1f,
+    -0x1.1d3444p-2f, 0x1.ebbd8cp-1f, -0x1.04fb8p-2f, 0x1.ef178ap-1f, -0x1.d934fep-3f, 0x1.f2253p-1f, -0x1.a82a02p-3f, 0x1.f4e604p-1f,
+    -0x1.76dd9ep-3f, 0x1.f7599ap-1f, -0x1.45576cp-3f, 0x1.f97f92p-1f, -0x1.139f0cp-3f, 0x1.fb5798p-1f, -0x1.c3785cp-4f, 0x1.fce16p-1f,
+    -0x1.5f6dp-4f, 0x1.fe1cbp-1f, -0x1.f656e8p-5f, 0x1.ff0956p-1f, -0x1.2d8658p-5f, 0x1.ffa72ep-1f, -0x1.921d2p-7f, 0x1.fff622p-1f,
+};
+
+// Copy the angle table into LDS (whole workgroup; the caller synchronises).
+__device__ __forceinline__ void load_angle_table(float* lds)
+{
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kSinCosTabFloats; i += blockDim.x) lds[i] = kSinCosTab[i];
+}
+
 // ------------------------------------------------------------- record stage
 
 struct Stage {
@@ -124,7 +202,8 @@ struct Particle {
 // record slice and counter.
 template <int PPL, bool GAUSS, bool ZREF, bool TRACKS>
 __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, uint32_t F, uint32_t N,
-                                          uint32_t video, uint32_t p0, Stage& st, uint32_t s0, uint32_t s1)
+                                          uint32_t video, uint32_t p0, Stage& st, const float2* tab,
+                                          uint32_t s0, uint32_t s1)
 {
     if (p0 >= N) return;  // empty tile (a request without particles)
     const uint32_t sim = pr.sim_id;
@@ -181,8 +260,8 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
         uint32_t wdet[PPL][4];
 #pragma unroll
         for (int i = 0; i < PPL; ++i) {
-            // Words 0..5: radius uniforms (top 24 bits) of pairs 0..5; words 6, 7:
-            // one angle byte per pair (pairs 0..3 from word 6, 4..5 from word 7).
+            // Words 0..5: radius uniforms (top 24 bits) of pairs 0..5; the angle of
+            // pair p has code (low byte of word p) << 8 | byte p % 4 of word 6 + p / 4.
             const U4 b0 = philox10(B * c + 1u, pt[i].id, video, sim, k0, k1);
             const U4 b1 = philox10(B * c + 2u, pt[i].id, video, sim, k0, k1);
             const uint32_t w[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
@@ -190,7 +269,7 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
             for (int p = 0; p < 6; ++p) {
                 const float r = bm_radius(w[p]);
                 float sn, cs;
-                sincos_turn24(bm_angle24(w[p], w[6 + p / 4], (uint32_t)p), sn, cs);
+                sincos_tab(tab, w[p] & 0xffu, (w[6 + p / 4] >> (8 * (p & 3))) & 0xffu, sn, cs);
                 const float rs = pt[i].sigma * r;
                 d[i][2 * p] = rs * sn;
                 d[i][2 * p + 1] = rs * cs;
@@ -262,7 +341,8 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
 // Tile t of a batch plan: the video owning it is the largest v with
 // vtile[v] <= t (wave-uniform binary search).
 template <int PPL, bool GAUSS, bool ZREF, bool TRACKS>
-__device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stage& st, uint32_t s0, uint32_t s1)
+__device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stage& st, const float2* tab,
+                                             uint32_t s0, uint32_t s1)
 {
     uint32_t lo = 0u, hi = a.n_videos;
     while (hi - lo > 1u) {
@@ -276,7 +356,7 @@ __device__ __forceinline__ void process_tile(const WalkArgs& a, uint32_t t, Stag
     st.slice_cap = pr.rec_cap;
     st.counter = a.rec_count + P;
     walk_tile<PPL, GAUSS, ZREF, TRACKS>(pr, a.sys[pr.sys_id], a.vframes[v], a.vparticles[v], v - pr.first_video,
-                                        (t - a.vtile[v]) * (uint32_t)(kWave * PPL), st, s0, s1);
+                                        (t - a.vtile[v]) * (uint32_t)(kWave * PPL), st, tab, s0, s1);
 }
 
 template <int PPL, bool PERSIST, bool GAUSS, bool ZREF, bool TRACKS = false>
@@ -289,6 +369,10 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
     st.buf = stage_lds + wave * a.stage_cap;
     st.cap = a.stage_cap;
     st.cnt = 0u;
+    float* tabf = reinterpret_cast<float*>(stage_lds + wpg * a.stage_cap);  // after the stages, 8-byte aligned
+    load_angle_table(tabf);
+    __syncthreads();
+    const float2* tab = reinterpret_cast<const float2*>(tabf);
     const uint64_t seed = *a.seed;
     const uint32_t k0 = uniform((uint32_t)seed), k1 = uniform((uint32_t)(seed >> 32));
     if (PERSIST) {
@@ -297,11 +381,11 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
             if (lane_id() == 0u) t = atomicAdd(a.tile_counter, 1u);
             t = uniform(t) + a.tile_lo;
             if (t >= a.tile_hi) break;
-            process_tile<PPL, GAUSS, ZREF, TRACKS>(a, t, st, k0, k1);
+            process_tile<PPL, GAUSS, ZREF, TRACKS>(a, t, st, tab, k0, k1);
         }
     } else {
         const uint32_t t = a.tile_lo + blockIdx.x * wpg + wave;
-        if (t < a.tile_hi) process_tile<PPL, GAUSS, ZREF, TRACKS>(a, t, st, k0, k1);
+        if (t < a.tile_hi) process_tile<PPL, GAUSS, ZREF, TRACKS>(a, t, st, tab, k0, k1);
     }
 }
 
@@ -735,7 +819,11 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
     st.buf = loop_lds + wave * a.stage_cap;
     st.cap = a.stage_cap;
     st.cnt = 0u;
-    uint32_t* rowc = loop_lds + kLoopWaves * a.stage_cap;
+    float* tabf = reinterpret_cast<float*>(loop_lds + kLoopWaves * a.stage_cap);
+    load_angle_table(tabf);
+    __syncthreads();
+    const float2* tab = reinterpret_cast<const float2*>(tabf);
+    uint32_t* rowc = loop_lds + kLoopWaves * a.stage_cap + kSinCosTabFloats;
     const DistLds L{rowc, rowc + a.nk_lds, wsum, &smax, red};
     const uint32_t vw = 3u * (a.maxv + 1u);
     // Polling: one idle group per CU holds the CU's poller token and polls
@@ -831,11 +919,11 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
             const bool gauss = sy.detect_model == PCS_DETECT_GAUSS_Z;
             const bool zref = sy.z_boundary == PCS_Z_REFLECT;
             if (gauss) {
-                if (zref) walk_tile<1, true, true, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
-                else walk_tile<1, true, false, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
+                if (zref) walk_tile<1, true, true, false>(pr, sy, F, N, lo, p0, st, tab, 0u, 0u);
+                else walk_tile<1, true, false, false>(pr, sy, F, N, lo, p0, st, tab, 0u, 0u);
             } else {
-                if (zref) walk_tile<1, false, true, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
-                else walk_tile<1, false, false, false>(pr, sy, F, N, lo, p0, st, 0u, 0u);
+                if (zref) walk_tile<1, false, true, false>(pr, sy, F, N, lo, p0, st, tab, 0u, 0u);
+                else walk_tile<1, false, false, false>(pr, sy, F, N, lo, p0, st, tab, 0u, 0u);
             }
         }
         __syncthreads();  // every wave's records are stored
@@ -885,14 +973,19 @@ __global__ void dense_hist_kernel(const DevProp* props, uint32_t prop_lo, const 
     }
 }
 
-__global__ void selftest_math_kernel(const uint32_t* w, float* radius, float* s, float* c, uint64_t n)
+// radius of word w[i] and sin/cos of the angle whose 16-bit code is w[i] >> 16
+__global__ __launch_bounds__(256) void selftest_math_kernel(const uint32_t* w, float* radius, float* s, float* c,
+                                                            uint64_t n)
 {
+    __shared__ float tabf[kSinCosTabFloats];
+    load_angle_table(tabf);
+    __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t v = w[i];
     radius[i] = bm_radius(v);
     float sn, cs;
-    sincos_turn_u24(v, sn, cs);
+    sincos_tab(reinterpret_cast<const float2*>(tabf), v >> 24, (v >> 16) & 0xffu, sn, cs);
     s[i] = sn;
     c[i] = cs;
 }
@@ -983,7 +1076,7 @@ template <int PPL, bool PERSIST, bool GAUSS, bool ZREF>
 static hipError_t launch_walk_t(const WalkArgs& a, int wpg, uint32_t grid, hipStream_t s)
 {
     (void)hipGetLastError();  // report only this launch's error
-    const size_t shmem = (size_t)wpg * a.stage_cap * sizeof(uint32_t);
+    const size_t shmem = ((size_t)wpg * a.stage_cap + kSinCosTabFloats) * sizeof(uint32_t);
     hipLaunchKernelGGL((walk_kernel<PPL, PERSIST, GAUSS, ZREF>), dim3(grid), dim3(wpg * kWave), shmem, s, a);
     return hipGetLastError();
 }
@@ -1018,7 +1111,7 @@ hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s
     (void)hipGetLastError();  // report only this launch's error
     if (a.tile_hi <= a.tile_lo) return hipSuccess;
     const uint32_t grid = (a.tile_hi - a.tile_lo + 3u) / 4u;
-    const size_t shmem = (size_t)4 * a.stage_cap * sizeof(uint32_t);
+    const size_t shmem = ((size_t)4 * a.stage_cap + kSinCosTabFloats) * sizeof(uint32_t);
     if (gauss && zref)
         hipLaunchKernelGGL((walk_kernel<1, false, true, true, true>), dim3(grid), dim3(256), shmem, s, a);
     else if (gauss)
@@ -1045,7 +1138,7 @@ hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s)
 
 size_t serve_loop_lds(const LoopArgs& a)
 {
-    return ((size_t)kLoopWaves * a.stage_cap + a.nk_lds + a.nde_lds) * sizeof(uint32_t);
+    return ((size_t)kLoopWaves * a.stage_cap + kSinCosTabFloats + a.nk_lds + a.nde_lds) * sizeof(uint32_t);
 }
 
 template <int SEG>

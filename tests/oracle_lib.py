@@ -67,7 +67,7 @@ def lib():
         L.orc_bm_radius.argtypes = [C.c_uint32]
         L.orc_bm_radius.restype = C.c_float
         L.orc_bm_radius_n.argtypes = [vp, vp, C.c_uint64]
-        L.orc_sincos_turn_n.argtypes = [vp, vp, vp, C.c_uint64]
+        L.orc_sincos_angle_n.argtypes = [vp, vp, vp, C.c_uint64]
         L.orc_simulate_f32.argtypes = [C.POINTER(orc_system), C.POINTER(orc_proposal), C.c_uint64, vp, vp]
         L.orc_simulate_f32.restype = C.c_uint64
         L.orc_simulate_f64.argtypes = [C.POINTER(orc_system), C.POINTER(orc_proposal), C.POINTER(orc_proposal_f64),
@@ -85,8 +85,8 @@ def lib():
         L.orc_reference_hist.argtypes = [vp, vp, C.c_uint64, C.c_double, C.c_uint32, C.c_uint32, vp, vp]
         L.orc_reference_hist.restype = C.c_uint64
         L.orc_sat_u32.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
-        L.orc_bm_angle24.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
-        L.orc_bm_angle24.restype = C.c_uint32
+        L.orc_bm_angle16.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_bm_angle16.restype = C.c_uint32
         L.orc_chunk_normals_n.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, vp]
         _LIB = L
     return _LIB
@@ -111,11 +111,12 @@ def bm_radius(words: np.ndarray) -> np.ndarray:
     return out
 
 
-def sincos_turn(words: np.ndarray):
+def sincos_angle(words: np.ndarray):
+    """sin and cos of the Box-Muller angle whose 16-bit code is word >> 16."""
     w = np.ascontiguousarray(words, dtype=np.uint32)
     s = np.empty(len(w), np.float32)
     c = np.empty(len(w), np.float32)
-    lib().orc_sincos_turn_n(_p(w), _p(s), _p(c), len(w))
+    lib().orc_sincos_angle_n(_p(w), _p(s), _p(c), len(w))
     return s, c
 
 

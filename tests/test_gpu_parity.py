@@ -48,11 +48,12 @@ def check_against_oracle(model, obs, props, frames, particles, seed, dist, track
 
 
 def test_device_math_bit_exact_all_inputs(gpu_lib):
-    # The walk consumes 24 bits of each word: every possible input.
+    # Every radius input (the top 24 bits of a word) and every angle code
+    # (the top 16 bits) through the device functions of the walk.
     w = (np.arange(1 << 24, dtype=np.uint64) << 8).astype(np.uint32)
     r, s, c = selftest_math(w)
     ro = O.bm_radius(w)
-    so, co = O.sincos_turn(w)
+    so, co = O.sincos_angle(w)
     assert np.array_equal(r.view(np.uint32), ro.view(np.uint32))
     assert np.array_equal(s.view(np.uint32), so.view(np.uint32))
     assert np.array_equal(c.view(np.uint32), co.view(np.uint32))
