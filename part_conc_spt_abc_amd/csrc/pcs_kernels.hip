@@ -253,6 +253,10 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
         q.de = 0.0f;
     }
 
+    uint64_t m_active[PPL];  // lanes holding a particle, per particle slot
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) m_active[i] = ballot(pt[i].active);
+
     const uint32_t steps = F > 0u ? F - 1u : 0u;
     for (uint32_t c = 0; 4u * c < steps; ++c) {
         const uint32_t nst = min(4u, steps - 4u * c);
@@ -292,15 +296,14 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                 q.x = wrap_periodic(q.x + dx, hx, Lx);
                 q.y = wrap_periodic(q.y + dy, hy, Ly);
                 q.z = ZREF ? wrap_reflect(q.z + dz, hz, Lz) : wrap_periodic(q.z + dz, hz, Lz);
-                bool det;
-                if (GAUSS)
-                    det = __builtin_fabsf(q.x) <= hax && __builtin_fabsf(q.y) <= hay &&
-                          (q.z * q.z) <= sig2 * neg2log_u24(wdet[i][s]);
-                else
-                    det = __builtin_fabsf(q.z) <= q.haz && __builtin_fabsf(q.x) <= hax &&
-                          __builtin_fabsf(q.y) <= hay;
-                const bool closing = q.active && !det && q.k >= kmin;
-                if (ballot(closing) != 0ull) {
+                const bool cx = __builtin_fabsf(q.x) <= hax, cy = __builtin_fabsf(q.y) <= hay;
+                const bool cz = GAUSS ? (q.z * q.z) <= sig2 * neg2log_u24(wdet[i][s]) : __builtin_fabsf(q.z) <= q.haz;
+                const bool det = cx && cy && cz;
+                // Lanes closing a track: the wave mask is formed from the masks of
+                // the single comparisons (no per-lane boolean is materialised).
+                const uint64_t m_closing = m_active[i] & ~(ballot(cx) & ballot(cy) & ballot(cz)) & ballot(q.k >= kmin);
+                if (m_closing != 0ull) {
+                    const bool closing = ((m_closing >> lane_id()) & 1ull) != 0ull;
                     if (TRACKS) {
                         float v = 0.0f;
                         if (closing) v = q.de / ((float)(q.k - 1u) * four_dt);
