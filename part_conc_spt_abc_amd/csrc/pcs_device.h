@@ -90,6 +90,8 @@ constexpr float kLogP0 = -0x1.fffff4p-2f, kLogP1 = 0x1.5556e8p-2f, kLogP2 = -0x1
 constexpr float kLn2Hi = 0x1.62e4p-1f;
 constexpr float kLn2Lo = 0x1.7f7d1cp-20f;
 constexpr float kSqrtHalf = 0x1.6a09e6p-1f;
+constexpr uint32_t kSqrtHalfBits = 0x3f3504f3u;  // bits of kSqrtHalf
+static_assert(__builtin_bit_cast(uint32_t, kSqrtHalf) == kSqrtHalfBits, "bits of sqrt(1/2)");
 // Box-Muller angle: theta = 2 pi (a + 1/2) 2^-16 for a 16-bit code a = i << 8 | r,
 // taken as the table angle 2 pi (i + 1/2) / 256 (sin/cos from the 256-entry
 // table kSinCosTab of pcs_kernels.hip, correctly rounded) plus beta = 2 pi (r - 127.5) 2^-16,
@@ -101,14 +103,15 @@ constexpr float kAngS3 = -0x1.555556p-3f, kAngC2 = -0x1p-1f, kAngC4 = 0x1.555556
 __device__ __forceinline__ float neg2log_u24(uint32_t w)
 {
     const uint32_t j = (w >> 8) + 1u;
-    const float xf = (float)j;  // exact: j <= 2^24
-    int e;
-    float m = __builtin_frexpf(xf, &e);  // v_frexp_mant / v_frexp_exp
-    const bool lo = m < kSqrtHalf;
-    m = lo ? m * 2.0f : m;
-    e = lo ? e - 1 : e;
+    // x = (float)j = m 2^e with m in [sqrt(1/2), sqrt(2)), in integer steps on
+    // the bits of x: t = bits(x) - bits(sqrt(1/2)) - (24 << 23) carries e - 24
+    // in its top 9 bits (arithmetic shift) and m's bits are the low 23 bits of t
+    // plus bits(sqrt(1/2)) (a borrow from the mantissa lands m in [1, sqrt 2)).
+    // Same m and e as frexp followed by the sqrt(1/2) fold-up.
+    const int32_t t = (int32_t)(__float_as_uint((float)j) - (kSqrtHalfBits + (24u << 23)));
+    const float m = __uint_as_float(((uint32_t)t & 0x7fffffu) + kSqrtHalfBits);
     const float f = m - 1.0f;  // exact (Sterbenz)
-    const float E = (float)(e - 24);
+    const float E = (float)(t >> 23);
     const float f2 = f * f;
     float P = kLogP7;
     P = __builtin_fmaf(P, f, kLogP6);
