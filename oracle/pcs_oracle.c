@@ -14,8 +14,8 @@
  *     words 0..5 -> radius uniforms of six Box-Muller pairs (top 24 bits of word p),
  *     angle of pair p: 16-bit code a = (low byte of word p) << 8 | byte (p % 4) of
  *     word 6 + p/4, theta = 2 pi (a + 1/2) 2^-16;
- *     pair p gives displacements d[2p] = (sigma*r)*sin, d[2p+1] = (sigma*r)*cos,
- *     step s of the chunk moves by (d[3s], d[3s+1], d[3s+2]);
+ *     pair p gives unit normals n[2p] = r*sin, n[2p+1] = r*cos, step s of the
+ *     chunk moves each coordinate by fma(sigma, n, x) with n = (n[3s], n[3s+1], n[3s+2]);
  *     words 8..11 (gaussian only) -> detection uniform of step s = word 8+s.
  * A video of F frames has F-1 steps (frames 2..F in src/cpu/simulate.jl:82).
  */
@@ -70,12 +70,14 @@ static const float LN2_HI = 0x1.62e4p-1f;
 static const float LN2_LO = 0x1.7f7d1cp-20f;
 static const float SQRT_HALF = 0x1.6a09e6p-1f;
 
+/* -2 ln(1+f) = g + g^2 Q(g) in g = -2f with Q(g) = -P(-g/2)/2: Q's coefficients
+ * are LOG_P[k] * LOG_QS[k], exact power-of-two scalings. */
+static const float LOG_QS[8] = { -0.5f, 0.25f, -0.125f, 0.0625f, -0.03125f, 0.015625f, -0.0078125f, 0.00390625f };
+
 /* Box-Muller angle theta = 2 pi (a + 1/2) 2^-16 for the 16-bit code a = i << 8 | r:
- * the table angle 2 pi (i + 1/2) / 256 (SINCOS_TAB: sin, cos correctly rounded,
- * fit_math_coeffs.py) rotated by beta = fma(r, ANG_K, ANG_C) = 2 pi (r - 127.5) 2^-16,
- * with sin beta = beta (1 + ANG_S3 beta^2), cos beta = 1 + beta^2 (ANG_C2 + ANG_C4 beta^2). */
-static const float ANG_K = 0x1.921fb6p-14f, ANG_C = -0x1.908d96p-7f;
-static const float ANG_S3 = -0x1.555556p-3f, ANG_C2 = -0x1p-1f, ANG_C4 = 0x1.555556p-5f;
+ * the sector centre 2 pi (i + 1/2) / 256 plus beta = 2 pi (r - 127.5) 2^-16.
+ * SINCOS_TAB holds (sin, cos) of the sector centres, BETA_TAB (sin beta,
+ * cos beta - 1); every entry correctly rounded to FP32 (fit_math_coeffs.py). */
 static const float SINCOS_TAB[512] = {
     0x1.921d2p-7f, 0x1.fff622p-1f, 0x1.2d8658p-5f, 0x1.ffa72ep-1f, 0x1.f656e8p-5f, 0x1.ff0956p-1f, 0x1.5f6dp-4f, 0x1.fe1cbp-1f,
     0x1.c3785cp-4f, 0x1.fce16p-1f, 0x1.139f0cp-3f, 0x1.fb5798p-1f, 0x1.45576cp-3f, 0x1.f97f92p-1f, 0x1.76dd9ep-3f, 0x1.f7599ap-1f,
@@ -142,10 +144,80 @@ static const float SINCOS_TAB[512] = {
     -0x1.76dd9ep-3f, 0x1.f7599ap-1f, -0x1.45576cp-3f, 0x1.f97f92p-1f, -0x1.139f0cp-3f, 0x1.fb5798p-1f, -0x1.c3785cp-4f, 0x1.fce16p-1f,
     -0x1.5f6dp-4f, 0x1.fe1cbp-1f, -0x1.f656e8p-5f, 0x1.ff0956p-1f, -0x1.2d8658p-5f, 0x1.ffa72ep-1f, -0x1.921d2p-7f, 0x1.fff622p-1f,
 };
+static const float BETA_TAB[512] = {
+    -0x1.908b08p-7f, -0x1.395c62p-14f, -0x1.8d66d8p-7f, -0x1.3476f8p-14f, -0x1.8a42a8p-7f, -0x1.2f9b6ep-14f, -0x1.871e76p-7f, -0x1.2ac9c2p-14f,
+    -0x1.83fa46p-7f, -0x1.2601f6p-14f, -0x1.80d614p-7f, -0x1.214406p-14f, -0x1.7db1e4p-7f, -0x1.1c8ff6p-14f, -0x1.7a8db2p-7f, -0x1.17e5c4p-14f,
+    -0x1.77698p-7f, -0x1.13457p-14f, -0x1.74454ep-7f, -0x1.0eaefcp-14f, -0x1.71211cp-7f, -0x1.0a2264p-14f, -0x1.6dfceap-7f, -0x1.059facp-14f,
+    -0x1.6ad8b6p-7f, -0x1.0126d2p-14f, -0x1.67b484p-7f, -0x1.f96faep-15f, -0x1.64905p-7f, -0x1.f0a574p-15f, -0x1.616c1ep-7f, -0x1.e7eef8p-15f,
+    -0x1.5e47eap-7f, -0x1.df4c38p-15f, -0x1.5b23b6p-7f, -0x1.d6bd36p-15f, -0x1.57ff82p-7f, -0x1.ce41fp-15f, -0x1.54db4ep-7f, -0x1.c5da68p-15f,
+    -0x1.51b71ap-7f, -0x1.bd869cp-15f, -0x1.4e92e4p-7f, -0x1.b5468cp-15f, -0x1.4b6ebp-7f, -0x1.ad1a3ap-15f, -0x1.484a7cp-7f, -0x1.a501a6p-15f,
+    -0x1.452646p-7f, -0x1.9cfccep-15f, -0x1.42021p-7f, -0x1.950bb2p-15f, -0x1.3edddap-7f, -0x1.8d2e54p-15f, -0x1.3bb9a6p-7f, -0x1.8564b4p-15f,
+    -0x1.38957p-7f, -0x1.7daedp-15f, -0x1.35713ap-7f, -0x1.760ca8p-15f, -0x1.324d02p-7f, -0x1.6e7e3ep-15f, -0x1.2f28ccp-7f, -0x1.670392p-15f,
+    -0x1.2c0496p-7f, -0x1.5f9ca2p-15f, -0x1.28e05ep-7f, -0x1.58496ep-15f, -0x1.25bc28p-7f, -0x1.5109f8p-15f, -0x1.2297fp-7f, -0x1.49de4p-15f,
+    -0x1.1f73b8p-7f, -0x1.42c644p-15f, -0x1.1c4f82p-7f, -0x1.3bc206p-15f, -0x1.192b4ap-7f, -0x1.34d184p-15f, -0x1.160712p-7f, -0x1.2df4bep-15f,
+    -0x1.12e2dap-7f, -0x1.272bb8p-15f, -0x1.0fbea2p-7f, -0x1.20766cp-15f, -0x1.0c9a68p-7f, -0x1.19d4ep-15f, -0x1.09763p-7f, -0x1.13470ep-15f,
+    -0x1.0651f8p-7f, -0x1.0cccfcp-15f, -0x1.032dbep-7f, -0x1.0666a6p-15f, -0x1.000986p-7f, -0x1.00140cp-15f, -0x1.f9ca98p-8f, -0x1.f3aa6p-16f,
+    -0x1.f38226p-8f, -0x1.e75422p-16f, -0x1.ed39b4p-8f, -0x1.db255ep-16f, -0x1.e6f14p-8f, -0x1.cf1e14p-16f, -0x1.e0a8ccp-8f, -0x1.c33e44p-16f,
+    -0x1.da6058p-8f, -0x1.b785fp-16f, -0x1.d417e4p-8f, -0x1.abf514p-16f, -0x1.cdcf7p-8f, -0x1.a08bb4p-16f, -0x1.c786fap-8f, -0x1.9549ccp-16f,
+    -0x1.c13e86p-8f, -0x1.8a2f6p-16f, -0x1.baf61p-8f, -0x1.7f3c6ep-16f, -0x1.b4ad9cp-8f, -0x1.7470f6p-16f, -0x1.ae6526p-8f, -0x1.69ccf8p-16f,
+    -0x1.a81cbp-8f, -0x1.5f5074p-16f, -0x1.a1d438p-8f, -0x1.54fb6cp-16f, -0x1.9b8bc2p-8f, -0x1.4acddcp-16f, -0x1.95434cp-8f, -0x1.40c7c8p-16f,
+    -0x1.8efad4p-8f, -0x1.36e92ep-16f, -0x1.88b25ep-8f, -0x1.2d320ep-16f, -0x1.8269e6p-8f, -0x1.23a268p-16f, -0x1.7c216ep-8f, -0x1.1a3a3cp-16f,
+    -0x1.75d8f6p-8f, -0x1.10f98cp-16f, -0x1.6f907ep-8f, -0x1.07e054p-16f, -0x1.694804p-8f, -0x1.fddd3p-17f, -0x1.62ff8cp-8f, -0x1.ec48acp-17f,
+    -0x1.5cb714p-8f, -0x1.db031cp-17f, -0x1.566e9ap-8f, -0x1.ca0c8p-17f, -0x1.50262p-8f, -0x1.b964dap-17f, -0x1.49dda8p-8f, -0x1.a90c28p-17f,
+    -0x1.43952ep-8f, -0x1.99026cp-17f, -0x1.3d4cb4p-8f, -0x1.8947a4p-17f, -0x1.37043ap-8f, -0x1.79dbdp-17f, -0x1.30bbcp-8f, -0x1.6abefp-17f,
+    -0x1.2a7344p-8f, -0x1.5bf106p-17f, -0x1.242acap-8f, -0x1.4d7212p-17f, -0x1.1de25p-8f, -0x1.3f421p-17f, -0x1.1799d4p-8f, -0x1.316104p-17f,
+    -0x1.11515ap-8f, -0x1.23ceeep-17f, -0x1.0b08dep-8f, -0x1.168bccp-17f, -0x1.04c062p-8f, -0x1.09979ep-17f, -0x1.fcefcep-9f, -0x1.f9e4cap-18f,
+    -0x1.f05ed6p-9f, -0x1.e13842p-18f, -0x1.e3cddep-9f, -0x1.c929a2p-18f, -0x1.d73ce6p-9f, -0x1.b1b8eep-18f, -0x1.caabeep-9f, -0x1.9ae622p-18f,
+    -0x1.be1af4p-9f, -0x1.84b142p-18f, -0x1.b189fcp-9f, -0x1.6f1a4ap-18f, -0x1.a4f902p-9f, -0x1.5a213cp-18f, -0x1.986808p-9f, -0x1.45c616p-18f,
+    -0x1.8bd71p-9f, -0x1.3208dcp-18f, -0x1.7f4614p-9f, -0x1.1ee98cp-18f, -0x1.72b51ap-9f, -0x1.0c6824p-18f, -0x1.66242p-9f, -0x1.f5094cp-19f,
+    -0x1.599326p-9f, -0x1.d27e24p-19f, -0x1.4d022ap-9f, -0x1.b12edp-19f, -0x1.40713p-9f, -0x1.911b5p-19f, -0x1.33e034p-9f, -0x1.7243a4p-19f,
+    -0x1.274f38p-9f, -0x1.54a7cap-19f, -0x1.1abe3ep-9f, -0x1.3847c4p-19f, -0x1.0e2d42p-9f, -0x1.1d2392p-19f, -0x1.019c46p-9f, -0x1.033b32p-19f,
+    -0x1.ea1692p-10f, -0x1.d51d5p-20f, -0x1.d0f49ap-10f, -0x1.a63be2p-20f, -0x1.b7d2ap-10f, -0x1.79d21ap-20f, -0x1.9eb0a8p-10f, -0x1.4fdffap-20f,
+    -0x1.858eaep-10f, -0x1.286582p-20f, -0x1.6c6cb4p-10f, -0x1.0362bp-20f, -0x1.534abap-10f, -0x1.c1af0cp-21f, -0x1.3a28cp-10f, -0x1.818808p-21f,
+    -0x1.2106c6p-10f, -0x1.465052p-21f, -0x1.07e4ccp-10f, -0x1.1007ecp-21f, -0x1.dd85a2p-11f, -0x1.bd5daap-22f, -0x1.ab41aep-11f, -0x1.648a18p-22f,
+    -0x1.78fdb8p-11f, -0x1.159526p-22f, -0x1.46b9c2p-11f, -0x1.a0fda4p-23f, -0x1.1475ccp-11f, -0x1.2a8e36p-23f, -0x1.c463acp-12f, -0x1.8fb80ep-24f,
+    -0x1.5fdbbep-12f, -0x1.e39c52p-25f, -0x1.f6a7a2p-13f, -0x1.ed7afp-26f, -0x1.2d97c8p-13f, -0x1.634e46p-27f, -0x1.921fb6p-15f, -0x1.3bd3ccp-30f,
+    0x1.921fb6p-15f, -0x1.3bd3ccp-30f, 0x1.2d97c8p-13f, -0x1.634e46p-27f, 0x1.f6a7a2p-13f, -0x1.ed7afp-26f, 0x1.5fdbbep-12f, -0x1.e39c52p-25f,
+    0x1.c463acp-12f, -0x1.8fb80ep-24f, 0x1.1475ccp-11f, -0x1.2a8e36p-23f, 0x1.46b9c2p-11f, -0x1.a0fda4p-23f, 0x1.78fdb8p-11f, -0x1.159526p-22f,
+    0x1.ab41aep-11f, -0x1.648a18p-22f, 0x1.dd85a2p-11f, -0x1.bd5daap-22f, 0x1.07e4ccp-10f, -0x1.1007ecp-21f, 0x1.2106c6p-10f, -0x1.465052p-21f,
+    0x1.3a28cp-10f, -0x1.818808p-21f, 0x1.534abap-10f, -0x1.c1af0cp-21f, 0x1.6c6cb4p-10f, -0x1.0362bp-20f, 0x1.858eaep-10f, -0x1.286582p-20f,
+    0x1.9eb0a8p-10f, -0x1.4fdffap-20f, 0x1.b7d2ap-10f, -0x1.79d21ap-20f, 0x1.d0f49ap-10f, -0x1.a63be2p-20f, 0x1.ea1692p-10f, -0x1.d51d5p-20f,
+    0x1.019c46p-9f, -0x1.033b32p-19f, 0x1.0e2d42p-9f, -0x1.1d2392p-19f, 0x1.1abe3ep-9f, -0x1.3847c4p-19f, 0x1.274f38p-9f, -0x1.54a7cap-19f,
+    0x1.33e034p-9f, -0x1.7243a4p-19f, 0x1.40713p-9f, -0x1.911b5p-19f, 0x1.4d022ap-9f, -0x1.b12edp-19f, 0x1.599326p-9f, -0x1.d27e24p-19f,
+    0x1.66242p-9f, -0x1.f5094cp-19f, 0x1.72b51ap-9f, -0x1.0c6824p-18f, 0x1.7f4614p-9f, -0x1.1ee98cp-18f, 0x1.8bd71p-9f, -0x1.3208dcp-18f,
+    0x1.986808p-9f, -0x1.45c616p-18f, 0x1.a4f902p-9f, -0x1.5a213cp-18f, 0x1.b189fcp-9f, -0x1.6f1a4ap-18f, 0x1.be1af4p-9f, -0x1.84b142p-18f,
+    0x1.caabeep-9f, -0x1.9ae622p-18f, 0x1.d73ce6p-9f, -0x1.b1b8eep-18f, 0x1.e3cddep-9f, -0x1.c929a2p-18f, 0x1.f05ed6p-9f, -0x1.e13842p-18f,
+    0x1.fcefcep-9f, -0x1.f9e4cap-18f, 0x1.04c062p-8f, -0x1.09979ep-17f, 0x1.0b08dep-8f, -0x1.168bccp-17f, 0x1.11515ap-8f, -0x1.23ceeep-17f,
+    0x1.1799d4p-8f, -0x1.316104p-17f, 0x1.1de25p-8f, -0x1.3f421p-17f, 0x1.242acap-8f, -0x1.4d7212p-17f, 0x1.2a7344p-8f, -0x1.5bf106p-17f,
+    0x1.30bbcp-8f, -0x1.6abefp-17f, 0x1.37043ap-8f, -0x1.79dbdp-17f, 0x1.3d4cb4p-8f, -0x1.8947a4p-17f, 0x1.43952ep-8f, -0x1.99026cp-17f,
+    0x1.49dda8p-8f, -0x1.a90c28p-17f, 0x1.50262p-8f, -0x1.b964dap-17f, 0x1.566e9ap-8f, -0x1.ca0c8p-17f, 0x1.5cb714p-8f, -0x1.db031cp-17f,
+    0x1.62ff8cp-8f, -0x1.ec48acp-17f, 0x1.694804p-8f, -0x1.fddd3p-17f, 0x1.6f907ep-8f, -0x1.07e054p-16f, 0x1.75d8f6p-8f, -0x1.10f98cp-16f,
+    0x1.7c216ep-8f, -0x1.1a3a3cp-16f, 0x1.8269e6p-8f, -0x1.23a268p-16f, 0x1.88b25ep-8f, -0x1.2d320ep-16f, 0x1.8efad4p-8f, -0x1.36e92ep-16f,
+    0x1.95434cp-8f, -0x1.40c7c8p-16f, 0x1.9b8bc2p-8f, -0x1.4acddcp-16f, 0x1.a1d438p-8f, -0x1.54fb6cp-16f, 0x1.a81cbp-8f, -0x1.5f5074p-16f,
+    0x1.ae6526p-8f, -0x1.69ccf8p-16f, 0x1.b4ad9cp-8f, -0x1.7470f6p-16f, 0x1.baf61p-8f, -0x1.7f3c6ep-16f, 0x1.c13e86p-8f, -0x1.8a2f6p-16f,
+    0x1.c786fap-8f, -0x1.9549ccp-16f, 0x1.cdcf7p-8f, -0x1.a08bb4p-16f, 0x1.d417e4p-8f, -0x1.abf514p-16f, 0x1.da6058p-8f, -0x1.b785fp-16f,
+    0x1.e0a8ccp-8f, -0x1.c33e44p-16f, 0x1.e6f14p-8f, -0x1.cf1e14p-16f, 0x1.ed39b4p-8f, -0x1.db255ep-16f, 0x1.f38226p-8f, -0x1.e75422p-16f,
+    0x1.f9ca98p-8f, -0x1.f3aa6p-16f, 0x1.000986p-7f, -0x1.00140cp-15f, 0x1.032dbep-7f, -0x1.0666a6p-15f, 0x1.0651f8p-7f, -0x1.0cccfcp-15f,
+    0x1.09763p-7f, -0x1.13470ep-15f, 0x1.0c9a68p-7f, -0x1.19d4ep-15f, 0x1.0fbea2p-7f, -0x1.20766cp-15f, 0x1.12e2dap-7f, -0x1.272bb8p-15f,
+    0x1.160712p-7f, -0x1.2df4bep-15f, 0x1.192b4ap-7f, -0x1.34d184p-15f, 0x1.1c4f82p-7f, -0x1.3bc206p-15f, 0x1.1f73b8p-7f, -0x1.42c644p-15f,
+    0x1.2297fp-7f, -0x1.49de4p-15f, 0x1.25bc28p-7f, -0x1.5109f8p-15f, 0x1.28e05ep-7f, -0x1.58496ep-15f, 0x1.2c0496p-7f, -0x1.5f9ca2p-15f,
+    0x1.2f28ccp-7f, -0x1.670392p-15f, 0x1.324d02p-7f, -0x1.6e7e3ep-15f, 0x1.35713ap-7f, -0x1.760ca8p-15f, 0x1.38957p-7f, -0x1.7daedp-15f,
+    0x1.3bb9a6p-7f, -0x1.8564b4p-15f, 0x1.3edddap-7f, -0x1.8d2e54p-15f, 0x1.42021p-7f, -0x1.950bb2p-15f, 0x1.452646p-7f, -0x1.9cfccep-15f,
+    0x1.484a7cp-7f, -0x1.a501a6p-15f, 0x1.4b6ebp-7f, -0x1.ad1a3ap-15f, 0x1.4e92e4p-7f, -0x1.b5468cp-15f, 0x1.51b71ap-7f, -0x1.bd869cp-15f,
+    0x1.54db4ep-7f, -0x1.c5da68p-15f, 0x1.57ff82p-7f, -0x1.ce41fp-15f, 0x1.5b23b6p-7f, -0x1.d6bd36p-15f, 0x1.5e47eap-7f, -0x1.df4c38p-15f,
+    0x1.616c1ep-7f, -0x1.e7eef8p-15f, 0x1.64905p-7f, -0x1.f0a574p-15f, 0x1.67b484p-7f, -0x1.f96faep-15f, 0x1.6ad8b6p-7f, -0x1.0126d2p-14f,
+    0x1.6dfceap-7f, -0x1.059facp-14f, 0x1.71211cp-7f, -0x1.0a2264p-14f, 0x1.74454ep-7f, -0x1.0eaefcp-14f, 0x1.77698p-7f, -0x1.13457p-14f,
+    0x1.7a8db2p-7f, -0x1.17e5c4p-14f, 0x1.7db1e4p-7f, -0x1.1c8ff6p-14f, 0x1.80d614p-7f, -0x1.214406p-14f, 0x1.83fa46p-7f, -0x1.2601f6p-14f,
+    0x1.871e76p-7f, -0x1.2ac9c2p-14f, 0x1.8a42a8p-7f, -0x1.2f9b6ep-14f, 0x1.8d66d8p-7f, -0x1.3476f8p-14f, 0x1.908b08p-7f, -0x1.395c62p-14f,
+};
 
 static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
+/* -2 ln u for u = ((w >> 8) + 1) 2^-24, positive for every w: the series in
+ * g = -2 (m - 1) (exact), each Horner step the one in f = m - 1 scaled by a
+ * power of two; g^2 carries an added 2^-100, so u = 1 gives about 2^-102
+ * rather than 0 (and a g^2 exactly halfway between two floats rounds up). */
 static inline float neg2log_i(uint32_t w)
 {
     uint32_t j = (w >> 8) + 1u;              /* 1 .. 2^24 */
@@ -153,32 +225,27 @@ static inline float neg2log_i(uint32_t w)
     int e;
     float m = frexpf(xf, &e);                 /* xf = m * 2^e, m in [1/2, 1) */
     if (m < SQRT_HALF) { m = m * 2.0f; e = e - 1; }
-    float f = m - 1.0f;                       /* exact */
+    float g = fmaf(m, -2.0f, 2.0f);           /* -2 (m - 1), exact */
     float E = (float)(e - 24);                /* u = xf * 2^-24 */
-    float f2 = f * f;
-    float P = LOG_P[7];
-    for (int i = 6; i >= 0; --i) P = fmaf(P, f, LOG_P[i]);
-    float l1p = fmaf(f2, P, f);
-    float L = fmaf(E, LN2_HI, fmaf(E, LN2_LO, l1p));
-    return fmaf(-2.0f, L, 0.0f);              /* -2 L, with +0 for L = 0 */
+    float g2 = fmaf(g, g, 0x1p-100f);
+    float Q = LOG_P[7] * LOG_QS[7];
+    for (int i = 6; i >= 0; --i) Q = fmaf(Q, g, LOG_P[i] * LOG_QS[i]);
+    float l = fmaf(g2, Q, g);
+    return fmaf(E, -2.0f * LN2_HI, fmaf(E, -2.0f * LN2_LO, l));
 }
 
 float orc_neg2log_u24(uint32_t w) { return neg2log_i(w); }
 static inline float bm_radius_i(uint32_t w) { return sqrtf(neg2log_i(w)); }
 float orc_bm_radius(uint32_t w) { return bm_radius_i(w); }
 
-/* sin and cos of the angle with code i << 8 | r (i, r bytes). */
+/* sin and cos of the angle with code i << 8 | r (i, r bytes):
+ * sin(a + beta) = (sa + ca sb) + sa (cb - 1), cos(a + beta) = (ca - sa sb) + ca (cb - 1). */
 static inline void sincos_i(uint32_t i, uint32_t r, float* s_out, float* c_out)
 {
     const float sa = SINCOS_TAB[2 * i], ca = SINCOS_TAB[2 * i + 1];
-    float b = fmaf((float)r, ANG_K, ANG_C);
-    float b2 = b * b;
-    float sb = b * fmaf(b2, ANG_S3, 1.0f);
-    float cb = fmaf(b2, fmaf(b2, ANG_C4, ANG_C2), 1.0f);
-    float t1 = ca * sb;
-    float t2 = sa * sb;
-    *s_out = fmaf(sa, cb, t1);
-    *c_out = fmaf(ca, cb, -t2);
+    const float sb = BETA_TAB[2 * r], cm = BETA_TAB[2 * r + 1];
+    *s_out = fmaf(sa, cm, fmaf(ca, sb, sa));
+    *c_out = fmaf(ca, cm, fmaf(-sa, sb, ca));
 }
 
 void orc_sincos16(uint32_t code, float* s, float* c) { sincos_i((code >> 8) & 0xffu, code & 0xffu, s, c); }
@@ -220,9 +287,9 @@ static inline float u24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
  * Gaussian detection uniforms. */
 static inline uint32_t chunk_blocks(int gauss) { return gauss ? 3u : 2u; }
 
-/* Displacements of chunk c in FP32 (twelve, three per step); the detection
+/* Unit normals of chunk c in FP32 (twelve, three per step); the detection
  * words of the chunk go to wdet when gauss. */
-static void chunk_f32(const stream_t* st, uint32_t c, int gauss, uint32_t part, uint32_t video, float sigma,
+static void chunk_f32(const stream_t* st, uint32_t c, int gauss, uint32_t part, uint32_t video,
                       float d[12], uint32_t wdet[4])
 {
     const uint32_t B = chunk_blocks(gauss);
@@ -234,9 +301,8 @@ static void chunk_f32(const stream_t* st, uint32_t c, int gauss, uint32_t part, 
         float sn, cs;
         const uint32_t a = bm_angle16(w[pr], w[6 + pr / 4], (uint32_t)pr);
         sincos_i(a >> 8, a & 0xffu, &sn, &cs);
-        float rs = sigma * r;
-        d[2 * pr] = rs * sn;
-        d[2 * pr + 1] = rs * cs;
+        d[2 * pr] = r * sn;
+        d[2 * pr + 1] = r * cs;
     }
     if (gauss) block(st, B * c + 3u, part, video, wdet);
 }
@@ -310,30 +376,32 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
         uint32_t wdet[4] = {0, 0, 0, 0};
         for (uint32_t st_i = 0; st_i < steps; ++st_i) {
             uint32_t c = st_i >> 2, sl = st_i & 3u;
-            if (sl == 0) chunk_f32(st, c, gauss, part, video, sigma, d, wdet);
-            const float dx = d[3 * sl], dy = d[3 * sl + 1], dz = d[3 * sl + 2];
-            x = wrap_periodic(x + dx, hx, Lx);
-            y = wrap_periodic(y + dy, hy, Ly);
-            z = sys->z_boundary ? wrap_reflect(z + dz, hz, Lz) : wrap_periodic(z + dz, hz, Lz);
+            if (sl == 0) chunk_f32(st, c, gauss, part, video, d, wdet);
+            const float nx = d[3 * sl], ny = d[3 * sl + 1], nz = d[3 * sl + 2];
+            x = wrap_periodic(fmaf(sigma, nx, x), hx, Lx);
+            y = wrap_periodic(fmaf(sigma, ny, y), hy, Ly);
+            const float zn = fmaf(sigma, nz, z);
+            z = sys->z_boundary ? wrap_reflect(zn, hz, Lz) : wrap_periodic(zn, hz, Lz);
 
             if (gauss)
                 det = fabsf(x) <= hax && fabsf(y) <= hay && (z * z) <= sig2 * neg2log_i(wdet[sl]);
             else
                 det = fabsf(z) <= haz && fabsf(x) <= hax && fabsf(y) <= hay;
 
+            /* de: squared lateral steps in units of sigma^2, scaled on record. */
             if (det) {
                 if (k >= 1u) {
-                    de = fmaf(dx, dx, de);
-                    de = fmaf(dy, dy, de);
+                    de = fmaf(nx, nx, de);
+                    de = fmaf(ny, ny, de);
                 }
                 k = k + 1u;
             } else {
-                if (k >= kmin) rec(rctx, k, de);
+                if (k >= kmin) rec(rctx, k, (sigma * sigma) * de);
                 k = 0u;
                 de = 0.0f;
             }
         }
-        if (k >= kmin) rec(rctx, k, de);
+        if (k >= kmin) rec(rctx, k, (sigma * sigma) * de);
     }
 }
 
@@ -419,7 +487,7 @@ void orc_chunk_normals_n(uint64_t seed, uint32_t sim_id, uint32_t video, uint32_
     uint32_t wdet[4];
     for (uint32_t part = 0; part < n_particles; ++part)
         for (uint32_t c = 0; c < n_chunks; ++c)
-            chunk_f32(&st, c, gauss, part, video, 1.0f, out + ((uint64_t)part * n_chunks + c) * 12u, wdet);
+            chunk_f32(&st, c, gauss, part, video, out + ((uint64_t)part * n_chunks + c) * 12u, wdet);
 }
 
 /* ------------------------------------------------------------ walk (FP64) */
@@ -574,11 +642,11 @@ void orc_trace_particle(const orc_system* sys, const orc_proposal* p, const orc_
     float d[12];
     for (uint32_t i = 0; i + 1 < F; ++i) {
         uint32_t c = i >> 2, sl = i & 3u;
-        if (sl == 0) chunk_f32(&st, c, gauss, part, video, sigma, d, wdet);
-        x = wrap_periodic(x + d[3 * sl], hx, 2.0f * hx);
-        y = wrap_periodic(y + d[3 * sl + 1], hy, 2.0f * hy);
-        z = sys->z_boundary ? wrap_reflect(z + d[3 * sl + 2], hz, 2.0f * hz)
-                            : wrap_periodic(z + d[3 * sl + 2], hz, 2.0f * hz);
+        if (sl == 0) chunk_f32(&st, c, gauss, part, video, d, wdet);
+        x = wrap_periodic(fmaf(sigma, d[3 * sl], x), hx, 2.0f * hx);
+        y = wrap_periodic(fmaf(sigma, d[3 * sl + 1], y), hy, 2.0f * hy);
+        const float zn = fmaf(sigma, d[3 * sl + 2], z);
+        z = sys->z_boundary ? wrap_reflect(zn, hz, 2.0f * hz) : wrap_periodic(zn, hz, 2.0f * hz);
         xyz32[3 * (i + 1)] = x; xyz32[3 * (i + 1) + 1] = y; xyz32[3 * (i + 1) + 2] = z;
     }
 }

@@ -7,10 +7,12 @@ hex floats.
   (weighted least squares on Chebyshev nodes, reweighted iteratively by the
   residual - Lawson's algorithm - to approach the minimax polynomial, then
   rounded to FP32);
-- the Box-Muller angle table: sin and cos of 2 pi (i + 1/2) / 256 for
-  i = 0..255, computed to 50 digits and rounded to nearest-even FP32, and
-  the constants of the rotation by the remaining angle (2 pi / 2^16, its
-  offset -127.5 * 2 pi / 2^16, -1/6, -1/2, 1/24 in FP32).
+- the two Box-Muller angle tables for the angle 2 pi (a + 1/2) / 2^16 of a
+  16-bit code a = i << 8 | r, which is the sector centre 2 pi (i + 1/2) / 256
+  plus beta = 2 pi (r - 127.5) / 2^16: SINCOS_TAB holds sin and cos of the
+  sector centres (i = 0..255), BETA_TAB holds sin beta and cos beta - 1
+  (r = 0..255); every entry computed to 50 digits and rounded to
+  nearest-even FP32.
 """
 import mpmath
 import numpy as np
@@ -60,9 +62,16 @@ def sincos_table():
     for i in range(256):
         th = 2 * mpmath.pi * (i + mpmath.mpf("0.5")) / 256
         tab += [f32_rn(mpmath.sin(th)), f32_rn(mpmath.cos(th))]
-    k = 2 * mpmath.pi / 65536
-    consts = [f32_rn(k), f32_rn(-mpmath.mpf("127.5") * k), f32_rn(mpmath.mpf(-1) / 6), -0.5, f32_rn(mpmath.mpf(1) / 24)]
-    return tab, consts
+    return tab
+
+
+def beta_table():
+    mpmath.mp.dps = 50
+    tab = []
+    for r in range(256):
+        b = 2 * mpmath.pi * (r - mpmath.mpf("127.5")) / 65536
+        tab += [f32_rn(mpmath.sin(b)), f32_rn(mpmath.cos(b) - 1)]
+    return tab
 
 
 def main():
@@ -71,9 +80,8 @@ def main():
     A = np.stack([x ** p for p in range(8)], 1)
     logp = lawson(A, (np.log1p(x) - x) / x ** 2, np.abs(x) + 1e-12, 80).astype(np.float32)
     print("LOG_P", [hexf(v) for v in logp])
-    tab, consts = sincos_table()
-    print("ANGLE_K", [float.hex(v) for v in consts])
-    print("SINCOS_TAB", [float.hex(v) for v in tab])
+    print("SINCOS_TAB", [float.hex(v) for v in sincos_table()])
+    print("BETA_TAB", [float.hex(v) for v in beta_table()])
 
 
 if __name__ == "__main__":

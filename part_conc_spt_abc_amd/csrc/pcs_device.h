@@ -92,14 +92,21 @@ constexpr float kLn2Lo = 0x1.7f7d1cp-20f;
 constexpr float kSqrtHalf = 0x1.6a09e6p-1f;
 constexpr uint32_t kSqrtHalfBits = 0x3f3504f3u;  // bits of kSqrtHalf
 static_assert(__builtin_bit_cast(uint32_t, kSqrtHalf) == kSqrtHalfBits, "bits of sqrt(1/2)");
-// Box-Muller angle: theta = 2 pi (a + 1/2) 2^-16 for a 16-bit code a = i << 8 | r,
-// taken as the table angle 2 pi (i + 1/2) / 256 (sin/cos from the 256-entry
-// table kSinCosTab of pcs_kernels.hip, correctly rounded) plus beta = 2 pi (r - 127.5) 2^-16,
-// |beta| < 0.0123, computed as fma(r, kAngK, kAngC).
-constexpr float kAngK = 0x1.921fb6p-14f, kAngC = -0x1.908d96p-7f;
-constexpr float kAngS3 = -0x1.555556p-3f, kAngC2 = -0x1p-1f, kAngC4 = 0x1.555556p-5f;
+// The series in g = -2 (m - 1): -2 ln(1+f) = g + g^2 Q(g) with Q(g) = -P(-g/2)/2,
+// whose coefficients are P's times powers of two (exact in FP32).
+constexpr float kLogQ0 = kLogP0 * -0.5f, kLogQ1 = kLogP1 * 0.25f, kLogQ2 = kLogP2 * -0.125f,
+                kLogQ3 = kLogP3 * 0.0625f, kLogQ4 = kLogP4 * -0.03125f, kLogQ5 = kLogP5 * 0.015625f,
+                kLogQ6 = kLogP6 * -0.0078125f, kLogQ7 = kLogP7 * 0.00390625f;
+constexpr float kNeg2Ln2Hi = -2.0f * kLn2Hi, kNeg2Ln2Lo = -2.0f * kLn2Lo;
+constexpr float kLogTiny = 0x1p-100f;
 
-// -2 ln(u) for u = ((w >> 8) + 1) 2^-24 in (0, 1].
+// -2 ln(u) for u = ((w >> 8) + 1) 2^-24 in (0, 1]; positive for every w.
+// With m, E as below, g = -2 (m - 1) = fma(m, -2, 2) is exact and every Horner
+// step in g is the corresponding step in f = m - 1 scaled by a power of two, so
+// the result is -2 (E ln 2 + f + f^2 P(f)) with the same roundings. g^2 carries
+// an added 2^-100, which changes the result only where u = 1 (then 2^-100 Q(0),
+// about 2^-102, instead of 0, so the square root below never sees 0) and where
+// g^2 is exactly halfway between two floats (rounded up instead of to even).
 __device__ __forceinline__ float neg2log_u24(uint32_t w)
 {
     const uint32_t j = (w >> 8) + 1u;
@@ -110,24 +117,23 @@ __device__ __forceinline__ float neg2log_u24(uint32_t w)
     // Same m and e as frexp followed by the sqrt(1/2) fold-up.
     const int32_t t = (int32_t)(__float_as_uint((float)j) - (kSqrtHalfBits + (24u << 23)));
     const float m = __uint_as_float(((uint32_t)t & 0x7fffffu) + kSqrtHalfBits);
-    const float f = m - 1.0f;  // exact (Sterbenz)
+    const float g = __builtin_fmaf(m, -2.0f, 2.0f);  // exact (Sterbenz)
     const float E = (float)(t >> 23);
-    const float f2 = f * f;
-    float P = kLogP7;
-    P = __builtin_fmaf(P, f, kLogP6);
-    P = __builtin_fmaf(P, f, kLogP5);
-    P = __builtin_fmaf(P, f, kLogP4);
-    P = __builtin_fmaf(P, f, kLogP3);
-    P = __builtin_fmaf(P, f, kLogP2);
-    P = __builtin_fmaf(P, f, kLogP1);
-    P = __builtin_fmaf(P, f, kLogP0);
-    const float l1p = __builtin_fmaf(f2, P, f);
-    const float L = __builtin_fmaf(E, kLn2Hi, __builtin_fmaf(E, kLn2Lo, l1p));
-    return __builtin_fmaf(-2.0f, L, 0.0f);
+    const float g2 = __builtin_fmaf(g, g, kLogTiny);
+    float Q = kLogQ7;
+    Q = __builtin_fmaf(Q, g, kLogQ6);
+    Q = __builtin_fmaf(Q, g, kLogQ5);
+    Q = __builtin_fmaf(Q, g, kLogQ4);
+    Q = __builtin_fmaf(Q, g, kLogQ3);
+    Q = __builtin_fmaf(Q, g, kLogQ2);
+    Q = __builtin_fmaf(Q, g, kLogQ1);
+    Q = __builtin_fmaf(Q, g, kLogQ0);
+    const float l = __builtin_fmaf(g2, Q, g);
+    return __builtin_fmaf(E, kNeg2Ln2Hi, __builtin_fmaf(E, kNeg2Ln2Lo, l));
 }
 
-// Correctly rounded sqrt on the Box-Muller domain y = -2 ln u (y = 0 or
-// 1.19e-7 <= y <= 33.3, 2^24 values): s = y rsq(y) with one correction by
+// Correctly rounded sqrt on the Box-Muller domain y = -2 ln u (about 2^-102,
+// or 1.19e-7 <= y <= 33.3; 2^24 values): s = y rsq(y) with one correction by
 // the residual y - s^2 (Markstein). It equals the correctly rounded square
 // root on every input of the domain (checked exhaustively against the CPU's
 // sqrtf, bit for bit); the hardware square root alone misses on 15 % of them.
@@ -137,26 +143,24 @@ __device__ __forceinline__ float sqrt_rn_domain(float y)
     const float s = y * q;
     const float h = 0.5f * q;
     const float r = __builtin_fmaf(-s, s, y);
-    const float v = __builtin_fmaf(r, h, s);
-    return y == 0.0f ? 0.0f : v;  // rsq(0) = +inf
+    return __builtin_fmaf(r, h, s);
 }
 
 // Box-Muller radius sqrt(-2 ln u), correctly rounded.
 __device__ __forceinline__ float bm_radius(uint32_t w) { return sqrt_rn_domain(neg2log_u24(w)); }
 
-// sin and cos of the Box-Muller angle with code i << 8 | r (i, r bytes) from
-// the table in LDS (tab[i] = (sin, cos) of 2 pi (i + 1/2) / 256): rotation by
-// beta with sin beta = beta (1 - beta^2 / 6), cos beta = 1 - beta^2 / 2 + beta^4 / 24
-// (the next terms are below 1e-9 relative).
+// sin and cos of the Box-Muller angle 2 pi (a + 1/2) 2^-16 of the code
+// a = i << 8 | r (i, r bytes), from the angle tables in LDS: tab[i] = (sin, cos)
+// of the sector centre 2 pi (i + 1/2) / 256, tab[256 + r] = (sin beta, cos beta - 1)
+// of the rest beta = 2 pi (r - 127.5) 2^-16. sin(a + beta) = (sa + ca sb) + sa (cb - 1):
+// the fused sa + ca sb cancels exactly near zero before its one rounding, then
+// the small term is added (cos likewise).
 __device__ __forceinline__ void sincos_tab(const float2* tab, uint32_t i, uint32_t r, float& s_out, float& c_out)
 {
-    const float2 t = tab[i];
-    const float b = __builtin_fmaf((float)r, kAngK, kAngC);
-    const float b2 = b * b;
-    const float sb = b * __builtin_fmaf(b2, kAngS3, 1.0f);
-    const float cb = __builtin_fmaf(b2, __builtin_fmaf(b2, kAngC4, kAngC2), 1.0f);
-    s_out = __builtin_fmaf(t.x, cb, t.y * sb);
-    c_out = __builtin_fmaf(t.y, cb, -(t.x * sb));
+    const float2 ta = tab[i];
+    const float2 tb = tab[256u + r];
+    s_out = __builtin_fmaf(ta.x, tb.y, __builtin_fmaf(ta.y, tb.x, ta.x));
+    c_out = __builtin_fmaf(ta.y, tb.y, __builtin_fmaf(-ta.x, tb.x, ta.y));
 }
 
 __device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }

@@ -45,9 +45,13 @@ __device__ __forceinline__ void wave_lds_sync()
 
 // ------------------------------------------------------------ angle table
 
-// (sin, cos) of 2 pi (i + 1/2) / 256, i = 0..255, correctly rounded to FP32
-// (fit_math_coeffs.py). Every workgroup that walks copies it into LDS.
+// The Box-Muller angle of code i << 8 | r is the sector centre 2 pi (i + 1/2) / 256
+// plus beta = 2 pi (r - 127.5) / 2^16. kSinCosTab holds (sin, cos) of the sector
+// centres, kBetaTab (sin beta, cos beta - 1); every entry is correctly rounded
+// to FP32 (fit_math_coeffs.py). Every workgroup that walks copies both into
+// LDS, kBetaTab right after kSinCosTab.
 constexpr int kSinCosTabFloats = 512;
+constexpr int kAngleTabFloats = 2 * kSinCosTabFloats;
 __constant__ float kSinCosTab[kSinCosTabFloats] = {
     0x1.921d2p-7f, 0x1.fff622p-1f, 0x1.2d8658p-5f, 0x1.ffa72ep-1f, 0x1.f656e8p-5f, 0x1.ff0956p-1f, 0x1.5f6dp-4f, 0x1.fe1cbp-1f,
     0x1.c3785cp-4f, 0x1.fce16p-1f, 0x1.139f0cp-3f, 0x1.fb5798p-1f, 0x1.45576cp-3f, 0x1.f97f92p-1f, 0x1.76dd9ep-3f, 0x1.f7599ap-1f,
@@ -114,11 +118,80 @@ __constant__ float kSinCosTab[kSinCosTabFloats] = {
     -0x1.76dd9ep-3f, 0x1.f7599ap-1f, -0x1.45576cp-3f, 0x1.f97f92p-1f, -0x1.139f0cp-3f, 0x1.fb5798p-1f, -0x1.c3785cp-4f, 0x1.fce16p-1f,
     -0x1.5f6dp-4f, 0x1.fe1cbp-1f, -0x1.f656e8p-5f, 0x1.ff0956p-1f, -0x1.2d8658p-5f, 0x1.ffa72ep-1f, -0x1.921d2p-7f, 0x1.fff622p-1f,
 };
+__constant__ float kBetaTab[kSinCosTabFloats] = {
+    -0x1.908b08p-7f, -0x1.395c62p-14f, -0x1.8d66d8p-7f, -0x1.3476f8p-14f, -0x1.8a42a8p-7f, -0x1.2f9b6ep-14f, -0x1.871e76p-7f, -0x1.2ac9c2p-14f,
+    -0x1.83fa46p-7f, -0x1.2601f6p-14f, -0x1.80d614p-7f, -0x1.214406p-14f, -0x1.7db1e4p-7f, -0x1.1c8ff6p-14f, -0x1.7a8db2p-7f, -0x1.17e5c4p-14f,
+    -0x1.77698p-7f, -0x1.13457p-14f, -0x1.74454ep-7f, -0x1.0eaefcp-14f, -0x1.71211cp-7f, -0x1.0a2264p-14f, -0x1.6dfceap-7f, -0x1.059facp-14f,
+    -0x1.6ad8b6p-7f, -0x1.0126d2p-14f, -0x1.67b484p-7f, -0x1.f96faep-15f, -0x1.64905p-7f, -0x1.f0a574p-15f, -0x1.616c1ep-7f, -0x1.e7eef8p-15f,
+    -0x1.5e47eap-7f, -0x1.df4c38p-15f, -0x1.5b23b6p-7f, -0x1.d6bd36p-15f, -0x1.57ff82p-7f, -0x1.ce41fp-15f, -0x1.54db4ep-7f, -0x1.c5da68p-15f,
+    -0x1.51b71ap-7f, -0x1.bd869cp-15f, -0x1.4e92e4p-7f, -0x1.b5468cp-15f, -0x1.4b6ebp-7f, -0x1.ad1a3ap-15f, -0x1.484a7cp-7f, -0x1.a501a6p-15f,
+    -0x1.452646p-7f, -0x1.9cfccep-15f, -0x1.42021p-7f, -0x1.950bb2p-15f, -0x1.3edddap-7f, -0x1.8d2e54p-15f, -0x1.3bb9a6p-7f, -0x1.8564b4p-15f,
+    -0x1.38957p-7f, -0x1.7daedp-15f, -0x1.35713ap-7f, -0x1.760ca8p-15f, -0x1.324d02p-7f, -0x1.6e7e3ep-15f, -0x1.2f28ccp-7f, -0x1.670392p-15f,
+    -0x1.2c0496p-7f, -0x1.5f9ca2p-15f, -0x1.28e05ep-7f, -0x1.58496ep-15f, -0x1.25bc28p-7f, -0x1.5109f8p-15f, -0x1.2297fp-7f, -0x1.49de4p-15f,
+    -0x1.1f73b8p-7f, -0x1.42c644p-15f, -0x1.1c4f82p-7f, -0x1.3bc206p-15f, -0x1.192b4ap-7f, -0x1.34d184p-15f, -0x1.160712p-7f, -0x1.2df4bep-15f,
+    -0x1.12e2dap-7f, -0x1.272bb8p-15f, -0x1.0fbea2p-7f, -0x1.20766cp-15f, -0x1.0c9a68p-7f, -0x1.19d4ep-15f, -0x1.09763p-7f, -0x1.13470ep-15f,
+    -0x1.0651f8p-7f, -0x1.0cccfcp-15f, -0x1.032dbep-7f, -0x1.0666a6p-15f, -0x1.000986p-7f, -0x1.00140cp-15f, -0x1.f9ca98p-8f, -0x1.f3aa6p-16f,
+    -0x1.f38226p-8f, -0x1.e75422p-16f, -0x1.ed39b4p-8f, -0x1.db255ep-16f, -0x1.e6f14p-8f, -0x1.cf1e14p-16f, -0x1.e0a8ccp-8f, -0x1.c33e44p-16f,
+    -0x1.da6058p-8f, -0x1.b785fp-16f, -0x1.d417e4p-8f, -0x1.abf514p-16f, -0x1.cdcf7p-8f, -0x1.a08bb4p-16f, -0x1.c786fap-8f, -0x1.9549ccp-16f,
+    -0x1.c13e86p-8f, -0x1.8a2f6p-16f, -0x1.baf61p-8f, -0x1.7f3c6ep-16f, -0x1.b4ad9cp-8f, -0x1.7470f6p-16f, -0x1.ae6526p-8f, -0x1.69ccf8p-16f,
+    -0x1.a81cbp-8f, -0x1.5f5074p-16f, -0x1.a1d438p-8f, -0x1.54fb6cp-16f, -0x1.9b8bc2p-8f, -0x1.4acddcp-16f, -0x1.95434cp-8f, -0x1.40c7c8p-16f,
+    -0x1.8efad4p-8f, -0x1.36e92ep-16f, -0x1.88b25ep-8f, -0x1.2d320ep-16f, -0x1.8269e6p-8f, -0x1.23a268p-16f, -0x1.7c216ep-8f, -0x1.1a3a3cp-16f,
+    -0x1.75d8f6p-8f, -0x1.10f98cp-16f, -0x1.6f907ep-8f, -0x1.07e054p-16f, -0x1.694804p-8f, -0x1.fddd3p-17f, -0x1.62ff8cp-8f, -0x1.ec48acp-17f,
+    -0x1.5cb714p-8f, -0x1.db031cp-17f, -0x1.566e9ap-8f, -0x1.ca0c8p-17f, -0x1.50262p-8f, -0x1.b964dap-17f, -0x1.49dda8p-8f, -0x1.a90c28p-17f,
+    -0x1.43952ep-8f, -0x1.99026cp-17f, -0x1.3d4cb4p-8f, -0x1.8947a4p-17f, -0x1.37043ap-8f, -0x1.79dbdp-17f, -0x1.30bbcp-8f, -0x1.6abefp-17f,
+    -0x1.2a7344p-8f, -0x1.5bf106p-17f, -0x1.242acap-8f, -0x1.4d7212p-17f, -0x1.1de25p-8f, -0x1.3f421p-17f, -0x1.1799d4p-8f, -0x1.316104p-17f,
+    -0x1.11515ap-8f, -0x1.23ceeep-17f, -0x1.0b08dep-8f, -0x1.168bccp-17f, -0x1.04c062p-8f, -0x1.09979ep-17f, -0x1.fcefcep-9f, -0x1.f9e4cap-18f,
+    -0x1.f05ed6p-9f, -0x1.e13842p-18f, -0x1.e3cddep-9f, -0x1.c929a2p-18f, -0x1.d73ce6p-9f, -0x1.b1b8eep-18f, -0x1.caabeep-9f, -0x1.9ae622p-18f,
+    -0x1.be1af4p-9f, -0x1.84b142p-18f, -0x1.b189fcp-9f, -0x1.6f1a4ap-18f, -0x1.a4f902p-9f, -0x1.5a213cp-18f, -0x1.986808p-9f, -0x1.45c616p-18f,
+    -0x1.8bd71p-9f, -0x1.3208dcp-18f, -0x1.7f4614p-9f, -0x1.1ee98cp-18f, -0x1.72b51ap-9f, -0x1.0c6824p-18f, -0x1.66242p-9f, -0x1.f5094cp-19f,
+    -0x1.599326p-9f, -0x1.d27e24p-19f, -0x1.4d022ap-9f, -0x1.b12edp-19f, -0x1.40713p-9f, -0x1.911b5p-19f, -0x1.33e034p-9f, -0x1.7243a4p-19f,
+    -0x1.274f38p-9f, -0x1.54a7cap-19f, -0x1.1abe3ep-9f, -0x1.3847c4p-19f, -0x1.0e2d42p-9f, -0x1.1d2392p-19f, -0x1.019c46p-9f, -0x1.033b32p-19f,
+    -0x1.ea1692p-10f, -0x1.d51d5p-20f, -0x1.d0f49ap-10f, -0x1.a63be2p-20f, -0x1.b7d2ap-10f, -0x1.79d21ap-20f, -0x1.9eb0a8p-10f, -0x1.4fdffap-20f,
+    -0x1.858eaep-10f, -0x1.286582p-20f, -0x1.6c6cb4p-10f, -0x1.0362bp-20f, -0x1.534abap-10f, -0x1.c1af0cp-21f, -0x1.3a28cp-10f, -0x1.818808p-21f,
+    -0x1.2106c6p-10f, -0x1.465052p-21f, -0x1.07e4ccp-10f, -0x1.1007ecp-21f, -0x1.dd85a2p-11f, -0x1.bd5daap-22f, -0x1.ab41aep-11f, -0x1.648a18p-22f,
+    -0x1.78fdb8p-11f, -0x1.159526p-22f, -0x1.46b9c2p-11f, -0x1.a0fda4p-23f, -0x1.1475ccp-11f, -0x1.2a8e36p-23f, -0x1.c463acp-12f, -0x1.8fb80ep-24f,
+    -0x1.5fdbbep-12f, -0x1.e39c52p-25f, -0x1.f6a7a2p-13f, -0x1.ed7afp-26f, -0x1.2d97c8p-13f, -0x1.634e46p-27f, -0x1.921fb6p-15f, -0x1.3bd3ccp-30f,
+    0x1.921fb6p-15f, -0x1.3bd3ccp-30f, 0x1.2d97c8p-13f, -0x1.634e46p-27f, 0x1.f6a7a2p-13f, -0x1.ed7afp-26f, 0x1.5fdbbep-12f, -0x1.e39c52p-25f,
+    0x1.c463acp-12f, -0x1.8fb80ep-24f, 0x1.1475ccp-11f, -0x1.2a8e36p-23f, 0x1.46b9c2p-11f, -0x1.a0fda4p-23f, 0x1.78fdb8p-11f, -0x1.159526p-22f,
+    0x1.ab41aep-11f, -0x1.648a18p-22f, 0x1.dd85a2p-11f, -0x1.bd5daap-22f, 0x1.07e4ccp-10f, -0x1.1007ecp-21f, 0x1.2106c6p-10f, -0x1.465052p-21f,
+    0x1.3a28cp-10f, -0x1.818808p-21f, 0x1.534abap-10f, -0x1.c1af0cp-21f, 0x1.6c6cb4p-10f, -0x1.0362bp-20f, 0x1.858eaep-10f, -0x1.286582p-20f,
+    0x1.9eb0a8p-10f, -0x1.4fdffap-20f, 0x1.b7d2ap-10f, -0x1.79d21ap-20f, 0x1.d0f49ap-10f, -0x1.a63be2p-20f, 0x1.ea1692p-10f, -0x1.d51d5p-20f,
+    0x1.019c46p-9f, -0x1.033b32p-19f, 0x1.0e2d42p-9f, -0x1.1d2392p-19f, 0x1.1abe3ep-9f, -0x1.3847c4p-19f, 0x1.274f38p-9f, -0x1.54a7cap-19f,
+    0x1.33e034p-9f, -0x1.7243a4p-19f, 0x1.40713p-9f, -0x1.911b5p-19f, 0x1.4d022ap-9f, -0x1.b12edp-19f, 0x1.599326p-9f, -0x1.d27e24p-19f,
+    0x1.66242p-9f, -0x1.f5094cp-19f, 0x1.72b51ap-9f, -0x1.0c6824p-18f, 0x1.7f4614p-9f, -0x1.1ee98cp-18f, 0x1.8bd71p-9f, -0x1.3208dcp-18f,
+    0x1.986808p-9f, -0x1.45c616p-18f, 0x1.a4f902p-9f, -0x1.5a213cp-18f, 0x1.b189fcp-9f, -0x1.6f1a4ap-18f, 0x1.be1af4p-9f, -0x1.84b142p-18f,
+    0x1.caabeep-9f, -0x1.9ae622p-18f, 0x1.d73ce6p-9f, -0x1.b1b8eep-18f, 0x1.e3cddep-9f, -0x1.c929a2p-18f, 0x1.f05ed6p-9f, -0x1.e13842p-18f,
+    0x1.fcefcep-9f, -0x1.f9e4cap-18f, 0x1.04c062p-8f, -0x1.09979ep-17f, 0x1.0b08dep-8f, -0x1.168bccp-17f, 0x1.11515ap-8f, -0x1.23ceeep-17f,
+    0x1.1799d4p-8f, -0x1.316104p-17f, 0x1.1de25p-8f, -0x1.3f421p-17f, 0x1.242acap-8f, -0x1.4d7212p-17f, 0x1.2a7344p-8f, -0x1.5bf106p-17f,
+    0x1.30bbcp-8f, -0x1.6abefp-17f, 0x1.37043ap-8f, -0x1.79dbdp-17f, 0x1.3d4cb4p-8f, -0x1.8947a4p-17f, 0x1.43952ep-8f, -0x1.99026cp-17f,
+    0x1.49dda8p-8f, -0x1.a90c28p-17f, 0x1.50262p-8f, -0x1.b964dap-17f, 0x1.566e9ap-8f, -0x1.ca0c8p-17f, 0x1.5cb714p-8f, -0x1.db031cp-17f,
+    0x1.62ff8cp-8f, -0x1.ec48acp-17f, 0x1.694804p-8f, -0x1.fddd3p-17f, 0x1.6f907ep-8f, -0x1.07e054p-16f, 0x1.75d8f6p-8f, -0x1.10f98cp-16f,
+    0x1.7c216ep-8f, -0x1.1a3a3cp-16f, 0x1.8269e6p-8f, -0x1.23a268p-16f, 0x1.88b25ep-8f, -0x1.2d320ep-16f, 0x1.8efad4p-8f, -0x1.36e92ep-16f,
+    0x1.95434cp-8f, -0x1.40c7c8p-16f, 0x1.9b8bc2p-8f, -0x1.4acddcp-16f, 0x1.a1d438p-8f, -0x1.54fb6cp-16f, 0x1.a81cbp-8f, -0x1.5f5074p-16f,
+    0x1.ae6526p-8f, -0x1.69ccf8p-16f, 0x1.b4ad9cp-8f, -0x1.7470f6p-16f, 0x1.baf61p-8f, -0x1.7f3c6ep-16f, 0x1.c13e86p-8f, -0x1.8a2f6p-16f,
+    0x1.c786fap-8f, -0x1.9549ccp-16f, 0x1.cdcf7p-8f, -0x1.a08bb4p-16f, 0x1.d417e4p-8f, -0x1.abf514p-16f, 0x1.da6058p-8f, -0x1.b785fp-16f,
+    0x1.e0a8ccp-8f, -0x1.c33e44p-16f, 0x1.e6f14p-8f, -0x1.cf1e14p-16f, 0x1.ed39b4p-8f, -0x1.db255ep-16f, 0x1.f38226p-8f, -0x1.e75422p-16f,
+    0x1.f9ca98p-8f, -0x1.f3aa6p-16f, 0x1.000986p-7f, -0x1.00140cp-15f, 0x1.032dbep-7f, -0x1.0666a6p-15f, 0x1.0651f8p-7f, -0x1.0cccfcp-15f,
+    0x1.09763p-7f, -0x1.13470ep-15f, 0x1.0c9a68p-7f, -0x1.19d4ep-15f, 0x1.0fbea2p-7f, -0x1.20766cp-15f, 0x1.12e2dap-7f, -0x1.272bb8p-15f,
+    0x1.160712p-7f, -0x1.2df4bep-15f, 0x1.192b4ap-7f, -0x1.34d184p-15f, 0x1.1c4f82p-7f, -0x1.3bc206p-15f, 0x1.1f73b8p-7f, -0x1.42c644p-15f,
+    0x1.2297fp-7f, -0x1.49de4p-15f, 0x1.25bc28p-7f, -0x1.5109f8p-15f, 0x1.28e05ep-7f, -0x1.58496ep-15f, 0x1.2c0496p-7f, -0x1.5f9ca2p-15f,
+    0x1.2f28ccp-7f, -0x1.670392p-15f, 0x1.324d02p-7f, -0x1.6e7e3ep-15f, 0x1.35713ap-7f, -0x1.760ca8p-15f, 0x1.38957p-7f, -0x1.7daedp-15f,
+    0x1.3bb9a6p-7f, -0x1.8564b4p-15f, 0x1.3edddap-7f, -0x1.8d2e54p-15f, 0x1.42021p-7f, -0x1.950bb2p-15f, 0x1.452646p-7f, -0x1.9cfccep-15f,
+    0x1.484a7cp-7f, -0x1.a501a6p-15f, 0x1.4b6ebp-7f, -0x1.ad1a3ap-15f, 0x1.4e92e4p-7f, -0x1.b5468cp-15f, 0x1.51b71ap-7f, -0x1.bd869cp-15f,
+    0x1.54db4ep-7f, -0x1.c5da68p-15f, 0x1.57ff82p-7f, -0x1.ce41fp-15f, 0x1.5b23b6p-7f, -0x1.d6bd36p-15f, 0x1.5e47eap-7f, -0x1.df4c38p-15f,
+    0x1.616c1ep-7f, -0x1.e7eef8p-15f, 0x1.64905p-7f, -0x1.f0a574p-15f, 0x1.67b484p-7f, -0x1.f96faep-15f, 0x1.6ad8b6p-7f, -0x1.0126d2p-14f,
+    0x1.6dfceap-7f, -0x1.059facp-14f, 0x1.71211cp-7f, -0x1.0a2264p-14f, 0x1.74454ep-7f, -0x1.0eaefcp-14f, 0x1.77698p-7f, -0x1.13457p-14f,
+    0x1.7a8db2p-7f, -0x1.17e5c4p-14f, 0x1.7db1e4p-7f, -0x1.1c8ff6p-14f, 0x1.80d614p-7f, -0x1.214406p-14f, 0x1.83fa46p-7f, -0x1.2601f6p-14f,
+    0x1.871e76p-7f, -0x1.2ac9c2p-14f, 0x1.8a42a8p-7f, -0x1.2f9b6ep-14f, 0x1.8d66d8p-7f, -0x1.3476f8p-14f, 0x1.908b08p-7f, -0x1.395c62p-14f,
+};
 
-// Copy the angle table into LDS (whole workgroup; the caller synchronises).
+// Copy both angle tables into LDS (whole workgroup; the caller synchronises).
 __device__ __forceinline__ void load_angle_table(float* lds)
 {
-    for (uint32_t i = threadIdx.x; i < (uint32_t)kSinCosTabFloats; i += blockDim.x) lds[i] = kSinCosTab[i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kSinCosTabFloats; i += blockDim.x) {
+        lds[i] = kSinCosTab[i];
+        lds[kSinCosTabFloats + i] = kBetaTab[i];
+    }
 }
 
 // ------------------------------------------------------------- record stage
@@ -187,12 +260,16 @@ __device__ __forceinline__ bool track_record(uint32_t k, float de, float four_dt
 
 // ------------------------------------------------------------------- walk
 
+// de is the track's sum of squared lateral steps in units of sigma^2 (the sum
+// of nx^2 + ny^2 over its unit normals); track_de gives it in um^2.
 struct Particle {
     float x, y, z, de, sigma, haz;
     uint32_t k;
     uint32_t id;
     bool active;
 };
+
+__device__ __forceinline__ float track_de(const Particle& q) { return (q.sigma * q.sigma) * q.de; }
 
 // TRACKS = true: the track-list form of src/cpu/generate_experiment.jl -
 // every track with k >= kmin is emitted as (k, de / ((k-1) 4 dt)) without
@@ -274,9 +351,8 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                 const float r = bm_radius(w[p]);
                 float sn, cs;
                 sincos_tab(tab, w[p] & 0xffu, (w[6 + p / 4] >> (8 * (p & 3))) & 0xffu, sn, cs);
-                const float rs = pt[i].sigma * r;
-                d[i][2 * p] = rs * sn;
-                d[i][2 * p + 1] = rs * cs;
+                d[i][2 * p] = r * sn;
+                d[i][2 * p + 1] = r * cs;
             }
             if (GAUSS) {
                 const U4 b2 = philox10(B * c + 3u, pt[i].id, video, sim, k0, k1);
@@ -292,10 +368,12 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
 #pragma unroll
             for (int i = 0; i < PPL; ++i) {
                 Particle& q = pt[i];
-                const float dx = d[i][3 * s], dy = d[i][3 * s + 1], dz = d[i][3 * s + 2];
-                q.x = wrap_periodic(q.x + dx, hx, Lx);
-                q.y = wrap_periodic(q.y + dy, hy, Ly);
-                q.z = ZREF ? wrap_reflect(q.z + dz, hz, Lz) : wrap_periodic(q.z + dz, hz, Lz);
+                // Unit normals of this step; the particle moves by sigma * n.
+                const float nx = d[i][3 * s], ny = d[i][3 * s + 1], nz = d[i][3 * s + 2];
+                q.x = wrap_periodic(__builtin_fmaf(q.sigma, nx, q.x), hx, Lx);
+                q.y = wrap_periodic(__builtin_fmaf(q.sigma, ny, q.y), hy, Ly);
+                const float zn = __builtin_fmaf(q.sigma, nz, q.z);
+                q.z = ZREF ? wrap_reflect(zn, hz, Lz) : wrap_periodic(zn, hz, Lz);
                 const bool cx = __builtin_fabsf(q.x) <= hax, cy = __builtin_fabsf(q.y) <= hay;
                 const bool cz = GAUSS ? (q.z * q.z) <= sig2 * neg2log_u24(wdet[i][s]) : __builtin_fabsf(q.z) <= q.haz;
                 const bool det = cx && cy && cz;
@@ -306,16 +384,16 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                     const bool closing = ((m_closing >> lane_id()) & 1ull) != 0ull;
                     if (TRACKS) {
                         float v = 0.0f;
-                        if (closing) v = q.de / ((float)(q.k - 1u) * four_dt);
+                        if (closing) v = track_de(q) / ((float)(q.k - 1u) * four_dt);
                         stage_push2(st, closing, q.k, __float_as_uint(v));
                     } else {
                         uint32_t rv = 0u;
                         bool ok = false;
-                        if (closing) ok = track_record(q.k, q.de, four_dt, dde, n_k, n_de, rv);
+                        if (closing) ok = track_record(q.k, track_de(q), four_dt, dde, n_k, n_de, rv);
                         stage_push(st, ok, rv);
                     }
                 }
-                const float de_n = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, q.de));
+                const float de_n = __builtin_fmaf(ny, ny, __builtin_fmaf(nx, nx, q.de));
                 q.de = (det && q.k >= 1u) ? de_n : 0.0f;
                 q.k = det ? q.k + 1u : 0u;
             }
@@ -328,12 +406,12 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
         if (ballot(closing) != 0ull) {
             if (TRACKS) {
                 float v = 0.0f;
-                if (closing) v = pt[i].de / ((float)(pt[i].k - 1u) * four_dt);
+                if (closing) v = track_de(pt[i]) / ((float)(pt[i].k - 1u) * four_dt);
                 stage_push2(st, closing, pt[i].k, __float_as_uint(v));
             } else {
                 uint32_t rv = 0u;
                 bool ok = false;
-                if (closing) ok = track_record(pt[i].k, pt[i].de, four_dt, dde, n_k, n_de, rv);
+                if (closing) ok = track_record(pt[i].k, track_de(pt[i]), four_dt, dde, n_k, n_de, rv);
                 stage_push(st, ok, rv);
             }
         }
@@ -826,7 +904,7 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
     load_angle_table(tabf);
     __syncthreads();
     const float2* tab = reinterpret_cast<const float2*>(tabf);
-    uint32_t* rowc = loop_lds + kLoopWaves * a.stage_cap + kSinCosTabFloats;
+    uint32_t* rowc = loop_lds + kLoopWaves * a.stage_cap + kAngleTabFloats;
     const DistLds L{rowc, rowc + a.nk_lds, wsum, &smax, red};
     const uint32_t vw = 3u * (a.maxv + 1u);
     // Polling: one idle group per CU holds the CU's poller token and polls
@@ -980,7 +1058,7 @@ __global__ void dense_hist_kernel(const DevProp* props, uint32_t prop_lo, const 
 __global__ __launch_bounds__(256) void selftest_math_kernel(const uint32_t* w, float* radius, float* s, float* c,
                                                             uint64_t n)
 {
-    __shared__ float tabf[kSinCosTabFloats];
+    __shared__ float tabf[kAngleTabFloats];
     load_angle_table(tabf);
     __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1079,7 +1157,7 @@ template <int PPL, bool PERSIST, bool GAUSS, bool ZREF>
 static hipError_t launch_walk_t(const WalkArgs& a, int wpg, uint32_t grid, hipStream_t s)
 {
     (void)hipGetLastError();  // report only this launch's error
-    const size_t shmem = ((size_t)wpg * a.stage_cap + kSinCosTabFloats) * sizeof(uint32_t);
+    const size_t shmem = ((size_t)wpg * a.stage_cap + kAngleTabFloats) * sizeof(uint32_t);
     hipLaunchKernelGGL((walk_kernel<PPL, PERSIST, GAUSS, ZREF>), dim3(grid), dim3(wpg * kWave), shmem, s, a);
     return hipGetLastError();
 }
@@ -1114,7 +1192,7 @@ hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s
     (void)hipGetLastError();  // report only this launch's error
     if (a.tile_hi <= a.tile_lo) return hipSuccess;
     const uint32_t grid = (a.tile_hi - a.tile_lo + 3u) / 4u;
-    const size_t shmem = ((size_t)4 * a.stage_cap + kSinCosTabFloats) * sizeof(uint32_t);
+    const size_t shmem = ((size_t)4 * a.stage_cap + kAngleTabFloats) * sizeof(uint32_t);
     if (gauss && zref)
         hipLaunchKernelGGL((walk_kernel<1, false, true, true, true>), dim3(grid), dim3(256), shmem, s, a);
     else if (gauss)
@@ -1141,7 +1219,7 @@ hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s)
 
 size_t serve_loop_lds(const LoopArgs& a)
 {
-    return ((size_t)kLoopWaves * a.stage_cap + kSinCosTabFloats + a.nk_lds + a.nde_lds) * sizeof(uint32_t);
+    return ((size_t)kLoopWaves * a.stage_cap + kAngleTabFloats + a.nk_lds + a.nde_lds) * sizeof(uint32_t);
 }
 
 template <int SEG>

@@ -29,14 +29,19 @@ def test_device_and_oracle_constants_match_fit():
     orc = (ROOT / "oracle/pcs_oracle.c").read_text()
     assert floats(re.search(r"constexpr float kLogP0.*?;", dev, re.S).group(0)) == v["LOG_P"]
     assert floats(re.search(r"LOG_P\[8\] = \{.*?\};", orc, re.S).group(0)) == v["LOG_P"]
-    # angle rotation constants K, C, -1/6, -1/2, 1/24 in that order
-    ang_dev = floats(re.search(r"constexpr float kAngK.*?kAngC4 = [^;]*;", dev, re.S).group(0))
-    ang_orc = floats(re.search(r"static const float ANG_K.*?ANG_C4 = [^;]*;", orc, re.S).group(0))
-    assert ang_dev == v["ANGLE_K"] and ang_orc == v["ANGLE_K"]
-    # the 256-entry (sin, cos) table, bit for bit in both copies
-    tab_dev = floats(re.search(r"kSinCosTab\[kSinCosTabFloats\] = \{.*?\};", ker, re.S).group(0))
-    tab_orc = floats(re.search(r"SINCOS_TAB\[512\] = \{.*?\};", orc, re.S).group(0))
-    assert tab_dev == v["SINCOS_TAB"] and tab_orc == v["SINCOS_TAB"] and len(tab_dev) == 512
+    # the two 256-entry angle tables, bit for bit in both copies
+    for name_dev, name_orc, key in (("kSinCosTab", "SINCOS_TAB", "SINCOS_TAB"), ("kBetaTab", "BETA_TAB", "BETA_TAB")):
+        tab_dev = floats(re.search(name_dev + r"\[kSinCosTabFloats\] = \{.*?\};", ker, re.S).group(0))
+        tab_orc = floats(re.search(name_orc + r"\[512\] = \{.*?\};", orc, re.S).group(0))
+        assert tab_dev == v[key] and tab_orc == v[key] and len(tab_dev) == 512
+
+
+def _max_ulp(got, want):
+    import numpy as np
+
+    got = np.array(got)
+    sp = np.spacing(np.abs(got).astype(np.float32)).astype(np.float64)
+    return (np.abs(got - want) / sp).max()
 
 
 def test_angle_table_is_correctly_rounded():
@@ -44,10 +49,17 @@ def test_angle_table_is_correctly_rounded():
     in particular within half an ulp of FP64 sin/cos."""
     import numpy as np
 
-    v = fitted()["SINCOS_TAB"]
     i = np.arange(256)
     th = 2.0 * np.pi * (i + 0.5) / 256
-    want = np.stack([np.sin(th), np.cos(th)], 1).ravel()
-    got = np.array(v)
-    sp = np.spacing(np.abs(got).astype(np.float32)).astype(np.float64)
-    assert (np.abs(got - want) / sp).max() <= 0.5 + 1e-6
+    assert _max_ulp(fitted()["SINCOS_TAB"], np.stack([np.sin(th), np.cos(th)], 1).ravel()) <= 0.5 + 1e-6
+
+
+def test_beta_table_is_correctly_rounded():
+    """sin beta and cos beta - 1 for beta = 2 pi (r - 127.5) / 2^16; cos beta - 1
+    is taken as -2 sin^2(beta/2), which FP64 gives to full relative precision
+    (the direct difference would cancel)."""
+    import numpy as np
+
+    b = 2.0 * np.pi * (np.arange(256) - 127.5) / 65536
+    want = np.stack([np.sin(b), -2.0 * np.sin(b / 2) ** 2], 1).ravel()
+    assert _max_ulp(fitted()["BETA_TAB"], want) <= 0.5 + 1e-6

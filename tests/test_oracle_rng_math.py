@@ -53,7 +53,10 @@ def test_radius_all_inputs(all_words):
     r = O.bm_radius(all_words)
     j = np.arange(1 << 24, dtype=np.float64)
     want = np.sqrt(-2.0 * np.log((j + 1.0) * 2.0 ** -24))
-    assert r[-1] == 0.0 and not np.signbit(r[-1])  # u = 1 -> radius +0
+    # u = 1: -2 ln u is 2^-100 Q(0) with Q(0) = -P0/2 (the tiny term that keeps
+    # the square root's input positive), P0 = -0x1.fffff4p-2; its root, 4.4e-16
+    y1 = np.float64(2.0 ** -100) * (-float.fromhex("-0x1.fffff4p-2") / 2.0)
+    assert r[-1] == np.float32(np.sqrt(y1)) and 0.0 < r[-1] < 1e-15
     e = ulps(r[:-1], want[:-1])
     assert e.max() <= 1.5, e.max()
     # radius is non-increasing in u
