@@ -307,6 +307,16 @@ static void chunk_f32(const stream_t* st, uint32_t c, int gauss, uint32_t part, 
     if (gauss) block(st, B * c + 3u, part, video, wdet);
 }
 
+/* Positions along periodic axes in box units of a frame whose origin is the
+ * observation window's lower edge: q in [0, 1), inside iff q <= 2 ha / L, a
+ * step is q = fract(q + (sigma / L) n) - the single periodic wrap of
+ * src/gpu/cusim/kernel-simulate-unroll.inl:24-32 and
+ * src/cpu/position_periodic.jl:1-9 for steps shorter than the box. x and y
+ * always; z unless it has reflecting walls or Gaussian detection (then z is
+ * in um, centred, wrapped below). fract is min(v - floor(v), 1 - 2^-24), the
+ * GPU's fract instruction. */
+static inline float fract_box(float v) { return fminf(v - floorf(v), 0x1.fffffep-1f); }
+
 /* Single periodic wrap (src/gpu/cusim/kernel-simulate-unroll.inl:24-32;
  * src/cpu/position_periodic.jl:1-9 in the centred frame). */
 static inline float wrap_periodic(float v, float h, float L)
@@ -343,30 +353,43 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
     const orc_system* sys = st->sys;
     const uint32_t F = p->frames[video];
     const uint32_t N = p->particles[video];
-    const float hx = sys->half_l[0], hy = sys->half_l[1], hz = sys->half_l[2];
-    const float Lx = 2.0f * hx, Ly = 2.0f * hy, Lz = 2.0f * hz;
+    const float hz = sys->half_l[2];
+    const float Lx = 2.0f * sys->half_l[0], Ly = 2.0f * sys->half_l[1], Lz = 2.0f * hz;
     const float hax = sys->half_a[0], hay = sys->half_a[1];
+    const float ox = hax / Lx - 0.5f, oy = hay / Ly - 0.5f;
+    const float wx = (hax + hax) / Lx, wy = (hay + hay) / Ly;
     const int gauss = sys->detect_model == 1;
-    const float sig2 = sys->detect_sigma * sys->detect_sigma;
+    const int zbox = !gauss && !sys->z_boundary;
+    const float sig2d = sys->detect_sigma * sys->detect_sigma;
     const uint32_t kmin = sys->kmin;
 
     for (uint32_t part = 0; part < N; ++part) {
         uint32_t w0[4];
         block(st, 0u, part, video, w0);
-        float x = hx * (2.0f * u24(w0[0]) - 1.0f);
-        float y = hy * (2.0f * u24(w0[1]) - 1.0f);
-        float z = hz * (2.0f * u24(w0[2]) - 1.0f);
+        float x = fract_box(u24(w0[0]) + ox);
+        float y = fract_box(u24(w0[1]) + oy);
         unsigned s = species_of(u24(w0[3]), p->cum_prob, sys->n_species);
         const float sigma = p->sigma[s];
         const float haz = p->half_az[s];
+        const float sx = sigma / Lx, sy = sigma / Ly, sig2 = sigma * sigma;
+        float z, sz, wz;
+        if (zbox) {
+            sz = sigma / Lz;
+            wz = (haz + haz) / Lz;
+            z = fract_box(u24(w0[2]) + (haz / Lz - 0.5f));
+        } else {
+            sz = sigma;
+            wz = haz;
+            z = hz * (2.0f * u24(w0[2]) - 1.0f);
+        }
 
         int det;
         if (gauss) {
             uint32_t wd[4];
             block(st, 0xffffffffu, part, video, wd);
-            det = fabsf(x) <= hax && fabsf(y) <= hay && (z * z) <= sig2 * neg2log_i(wd[0]);
+            det = x <= wx && y <= wy && (z * z) <= sig2d * neg2log_i(wd[0]);
         } else {
-            det = fabsf(z) <= haz && fabsf(x) <= hax && fabsf(y) <= hay;
+            det = x <= wx && y <= wy && (zbox ? z <= wz : fabsf(z) <= wz);
         }
         uint32_t k = det ? 1u : 0u;
         float de = 0.0f;
@@ -378,15 +401,19 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
             uint32_t c = st_i >> 2, sl = st_i & 3u;
             if (sl == 0) chunk_f32(st, c, gauss, part, video, d, wdet);
             const float nx = d[3 * sl], ny = d[3 * sl + 1], nz = d[3 * sl + 2];
-            x = wrap_periodic(fmaf(sigma, nx, x), hx, Lx);
-            y = wrap_periodic(fmaf(sigma, ny, y), hy, Ly);
-            const float zn = fmaf(sigma, nz, z);
-            z = sys->z_boundary ? wrap_reflect(zn, hz, Lz) : wrap_periodic(zn, hz, Lz);
+            x = fract_box(fmaf(sx, nx, x));
+            y = fract_box(fmaf(sy, ny, y));
+            if (zbox) {
+                z = fract_box(fmaf(sz, nz, z));
+            } else {
+                const float zn = fmaf(sz, nz, z);
+                z = sys->z_boundary ? wrap_reflect(zn, hz, Lz) : wrap_periodic(zn, hz, Lz);
+            }
 
             if (gauss)
-                det = fabsf(x) <= hax && fabsf(y) <= hay && (z * z) <= sig2 * neg2log_i(wdet[sl]);
+                det = x <= wx && y <= wy && (z * z) <= sig2d * neg2log_i(wdet[sl]);
             else
-                det = fabsf(z) <= haz && fabsf(x) <= hax && fabsf(y) <= hay;
+                det = x <= wx && y <= wy && (zbox ? z <= wz : fabsf(z) <= wz);
 
             /* de: squared lateral steps in units of sigma^2, scaled on record. */
             if (det) {
@@ -396,12 +423,12 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
                 }
                 k = k + 1u;
             } else {
-                if (k >= kmin) rec(rctx, k, (sigma * sigma) * de);
+                if (k >= kmin) rec(rctx, k, sig2 * de);
                 k = 0u;
                 de = 0.0f;
             }
         }
-        if (k >= kmin) rec(rctx, k, (sigma * sigma) * de);
+        if (k >= kmin) rec(rctx, k, sig2 * de);
     }
 }
 
@@ -626,28 +653,50 @@ void orc_trace_particle(const orc_system* sys, const orc_proposal* p, const orc_
     make_stream(&st, sys, seed, p->sim_id);
     if (xyz64) walk_particle_f64(&st, p, pd, video, part, NULL, xyz64);
     if (!xyz32) return;
-    /* FP32 trajectory: same steps as walk_video_f32, positions only. */
+    /* FP32 trajectory: same steps as walk_video_f32, positions only, reported
+     * in um centred on the box (box-unit axes converted in FP64, then rounded). */
     const uint32_t F = p->frames[video];
-    const float hx = sys->half_l[0], hy = sys->half_l[1], hz = sys->half_l[2];
+    const double hd[3] = { sys->half_l[0], sys->half_l[1], sys->half_l[2] };
+    const float hz = sys->half_l[2];
+    const float L[3] = { 2.0f * sys->half_l[0], 2.0f * sys->half_l[1], 2.0f * hz };
     const int gauss = sys->detect_model == 1;
+    const int zbox = !gauss && !sys->z_boundary;
     uint32_t wdet[4];
     uint32_t w0[4];
     block(&st, 0u, part, video, w0);
-    float x = hx * (2.0f * u24(w0[0]) - 1.0f);
-    float y = hy * (2.0f * u24(w0[1]) - 1.0f);
-    float z = hz * (2.0f * u24(w0[2]) - 1.0f);
     unsigned s = species_of(u24(w0[3]), p->cum_prob, sys->n_species);
     const float sigma = p->sigma[s];
-    xyz32[0] = x; xyz32[1] = y; xyz32[2] = z;
+    const float haz = p->half_az[s];
+    const float ha[3] = { sys->half_a[0], sys->half_a[1], haz };
+    float q[3], sc[3];
+    for (int a = 0; a < 3; ++a) {
+        if (a < 2 || zbox) {
+            q[a] = fract_box(u24(w0[a]) + (ha[a] / L[a] - 0.5f));
+            sc[a] = sigma / L[a];
+        } else {
+            q[a] = hz * (2.0f * u24(w0[a]) - 1.0f);
+            sc[a] = sigma;
+        }
+    }
     float d[12];
-    for (uint32_t i = 0; i + 1 < F; ++i) {
-        uint32_t c = i >> 2, sl = i & 3u;
-        if (sl == 0) chunk_f32(&st, c, gauss, part, video, d, wdet);
-        x = wrap_periodic(fmaf(sigma, d[3 * sl], x), hx, 2.0f * hx);
-        y = wrap_periodic(fmaf(sigma, d[3 * sl + 1], y), hy, 2.0f * hy);
-        const float zn = fmaf(sigma, d[3 * sl + 2], z);
-        z = sys->z_boundary ? wrap_reflect(zn, hz, 2.0f * hz) : wrap_periodic(zn, hz, 2.0f * hz);
-        xyz32[3 * (i + 1)] = x; xyz32[3 * (i + 1) + 1] = y; xyz32[3 * (i + 1) + 2] = z;
+    for (uint32_t i = 0; i < F; ++i) {
+        if (i > 0) {
+            uint32_t c = (i - 1) >> 2, sl = (i - 1) & 3u;
+            if (sl == 0) chunk_f32(&st, c, gauss, part, video, d, wdet);
+            for (int a = 0; a < 3; ++a) {
+                const float v = fmaf(sc[a], d[3 * sl + a], q[a]);
+                if (a < 2 || zbox) q[a] = fract_box(v);
+                else q[a] = sys->z_boundary ? wrap_reflect(v, hz, L[2]) : wrap_periodic(v, hz, L[2]);
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            double v = q[a];
+            if (a < 2 || zbox) {
+                v = v * (double)L[a] - (double)ha[a];
+                v = v - (double)L[a] * floor((v + hd[a]) / (double)L[a]);
+            }
+            xyz32[3 * i + a] = (float)v;
+        }
     }
 }
 

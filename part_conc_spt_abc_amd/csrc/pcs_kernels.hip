@@ -260,16 +260,30 @@ __device__ __forceinline__ bool track_record(uint32_t k, float de, float four_dt
 
 // ------------------------------------------------------------------- walk
 
+// Positions along periodic axes are kept in box units of a frame whose origin
+// is the observation window's lower edge: q in [0, 1), inside the window iff
+// q <= 2 ha / L, a step is q = fract(q + (sigma / L) n) (the single periodic
+// wrap of the reference for any step shorter than the box). x and y are always
+// periodic; z is too unless walls reflect it or detection in z is Gaussian,
+// and then z stays in um, centred on the box.
+// v_fract_f32(v) = min(v - floor(v), 1 - 2^-24), bit for bit on every v in
+// [-0.125, 1.125) (checked exhaustively on the GPU), which the oracle restates.
+__device__ __forceinline__ float fract_box(float v) { return __builtin_amdgcn_fractf(v); }
+
 // de is the track's sum of squared lateral steps in units of sigma^2 (the sum
 // of nx^2 + ny^2 over its unit normals); track_de gives it in um^2.
 struct Particle {
-    float x, y, z, de, sigma, haz;
+    float x, y, z, de;
+    float sx, sy, sz;  // step per unit normal: sigma / L in box units, sigma for z in um
+    float wz;          // z window: 2 haz / Lz in box units, haz for z in um
+    float sig2;        // sigma^2
     uint32_t k;
     uint32_t id;
     bool active;
+    bool prev;         // inside the window in the previous frame
 };
 
-__device__ __forceinline__ float track_de(const Particle& q) { return (q.sigma * q.sigma) * q.de; }
+__device__ __forceinline__ float track_de(const Particle& q) { return q.sig2 * q.de; }
 
 // TRACKS = true: the track-list form of src/cpu/generate_experiment.jl -
 // every track with k >= kmin is emitted as (k, de / ((k-1) 4 dt)) without
@@ -283,13 +297,16 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                                           uint32_t s0, uint32_t s1)
 {
     if (p0 >= N) return;  // empty tile (a request without particles)
+    constexpr bool ZBOX = !GAUSS && !ZREF;  // z in box units
     const uint32_t sim = pr.sim_id;
     const uint32_t k0 = s0 ^ pr.key0, k1 = s1 ^ pr.key1;
 
-    const float hx = sy.half_l[0], hy = sy.half_l[1], hz = sy.half_l[2];
-    const float Lx = 2.0f * hx, Ly = 2.0f * hy, Lz = 2.0f * hz;
+    const float hz = sy.half_l[2];
+    const float Lx = 2.0f * sy.half_l[0], Ly = 2.0f * sy.half_l[1], Lz = 2.0f * hz;
     const float hax = sy.half_a[0], hay = sy.half_a[1];
-    const float sig2 = sy.detect_sigma * sy.detect_sigma;
+    const float ox = hax / Lx - 0.5f, oy = hay / Ly - 0.5f;  // window origin offsets
+    const float wx = (hax + hax) / Lx, wy = (hay + hay) / Ly;  // window widths
+    const float sig2d = sy.detect_sigma * sy.detect_sigma;
     const uint32_t kmin = sy.kmin, n_k = sy.n_k, n_de = sy.n_de, C = sy.n_species;
     const float four_dt = sy.four_dt, dde = sy.de_bin_width;
     // Philox blocks per 4-step chunk: 2 for the 12 displacements (+1 with
@@ -303,9 +320,8 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
         q.id = p0 + (uint32_t)(i * kWave) + lane_id();
         q.active = q.id < N;
         const U4 w = philox10(0u, q.id, video, sim, k0, k1);
-        q.x = hx * (2.0f * u24(w.x) - 1.0f);
-        q.y = hy * (2.0f * u24(w.y) - 1.0f);
-        q.z = hz * (2.0f * u24(w.z) - 1.0f);
+        q.x = fract_box(u24(w.x) + ox);
+        q.y = fract_box(u24(w.y) + oy);
         const float us = u24(w.w);
         uint32_t s = 0u;
 #pragma unroll
@@ -316,17 +332,29 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
             sg = (s == c) ? pr.sigma[c] : sg;
             az = (s == c) ? pr.half_az[c] : az;
         }
-        q.sigma = sg;
-        q.haz = az;
+        q.sx = sg / Lx;
+        q.sy = sg / Ly;
+        q.sig2 = sg * sg;
+        if (ZBOX) {
+            q.sz = sg / Lz;
+            q.wz = (az + az) / Lz;
+            q.z = fract_box(u24(w.z) + (az / Lz - 0.5f));
+        } else {
+            q.sz = sg;
+            q.wz = az;
+            q.z = hz * (2.0f * u24(w.z) - 1.0f);
+        }
         bool det;
         if (GAUSS) {
             const U4 wd = philox10(0xffffffffu, q.id, video, sim, k0, k1);
-            det = __builtin_fabsf(q.x) <= hax && __builtin_fabsf(q.y) <= hay &&
-                  (q.z * q.z) <= sig2 * neg2log_u24(wd.x);
+            det = q.x <= wx && q.y <= wy && (q.z * q.z) <= sig2d * neg2log_u24(wd.x);
+        } else if (ZBOX) {
+            det = q.x <= wx && q.y <= wy && q.z <= q.wz;
         } else {
-            det = __builtin_fabsf(q.z) <= q.haz && __builtin_fabsf(q.x) <= hax && __builtin_fabsf(q.y) <= hay;
+            det = q.x <= wx && q.y <= wy && __builtin_fabsf(q.z) <= q.wz;
         }
         q.k = det ? 1u : 0u;
+        q.prev = det;
         q.de = 0.0f;
     }
 
@@ -370,12 +398,17 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                 Particle& q = pt[i];
                 // Unit normals of this step; the particle moves by sigma * n.
                 const float nx = d[i][3 * s], ny = d[i][3 * s + 1], nz = d[i][3 * s + 2];
-                q.x = wrap_periodic(__builtin_fmaf(q.sigma, nx, q.x), hx, Lx);
-                q.y = wrap_periodic(__builtin_fmaf(q.sigma, ny, q.y), hy, Ly);
-                const float zn = __builtin_fmaf(q.sigma, nz, q.z);
-                q.z = ZREF ? wrap_reflect(zn, hz, Lz) : wrap_periodic(zn, hz, Lz);
-                const bool cx = __builtin_fabsf(q.x) <= hax, cy = __builtin_fabsf(q.y) <= hay;
-                const bool cz = GAUSS ? (q.z * q.z) <= sig2 * neg2log_u24(wdet[i][s]) : __builtin_fabsf(q.z) <= q.haz;
+                q.x = fract_box(__builtin_fmaf(q.sx, nx, q.x));
+                q.y = fract_box(__builtin_fmaf(q.sy, ny, q.y));
+                if (ZBOX) {
+                    q.z = fract_box(__builtin_fmaf(q.sz, nz, q.z));
+                } else {
+                    const float zn = __builtin_fmaf(q.sz, nz, q.z);
+                    q.z = ZREF ? wrap_reflect(zn, hz, Lz) : wrap_periodic(zn, hz, Lz);
+                }
+                const bool cx = q.x <= wx, cy = q.y <= wy;
+                const bool cz = GAUSS ? (q.z * q.z) <= sig2d * neg2log_u24(wdet[i][s])
+                                      : (ZBOX ? q.z <= q.wz : __builtin_fabsf(q.z) <= q.wz);
                 const bool det = cx && cy && cz;
                 // Lanes closing a track: the wave mask is formed from the masks of
                 // the single comparisons (no per-lane boolean is materialised).
@@ -394,8 +427,9 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                     }
                 }
                 const float de_n = __builtin_fmaf(ny, ny, __builtin_fmaf(nx, nx, q.de));
-                q.de = (det && q.k >= 1u) ? de_n : 0.0f;
+                q.de = (det && q.prev) ? de_n : 0.0f;
                 q.k = det ? q.k + 1u : 0u;
+                q.prev = det;
             }
         }
     }
