@@ -220,30 +220,6 @@ __device__ __forceinline__ void stage_flush(Stage& st)
     st.cnt = 0u;
 }
 
-__device__ __forceinline__ void stage_push(Stage& st, bool ok, uint32_t v)
-{
-    const uint64_t m = ballot(ok);
-    if (m == 0ull) return;
-    if (st.cnt + (uint32_t)kWave > st.cap) stage_flush(st);
-    const uint32_t pos = st.cnt + mbcnt64(m);
-    if (ok) st.buf[pos] = v;
-    st.cnt += (uint32_t)__builtin_popcountll(m);
-}
-
-// Two-word records (k, DE bits) of the track-list form.
-__device__ __forceinline__ void stage_push2(Stage& st, bool ok, uint32_t a, uint32_t b)
-{
-    const uint64_t m = ballot(ok);
-    if (m == 0ull) return;
-    if (st.cnt + (uint32_t)(2 * kWave) > st.cap) stage_flush(st);
-    const uint32_t pos = st.cnt + 2u * mbcnt64(m);
-    if (ok) {
-        st.buf[pos] = a;
-        st.buf[pos + 1u] = b;
-    }
-    st.cnt += 2u * (uint32_t)__builtin_popcountll(m);
-}
-
 // Track record: de / ((k-1) 4 dt) then bin = trunc(value / dDE), dropped when
 // the bin or the length is outside the matrix
 // (src/gpu/cusim/kernel-simulate-unroll.inl:187-190, src/gpu/cusim/histogram2d.inl:98-106).
@@ -256,6 +232,56 @@ __device__ __forceinline__ bool track_record(uint32_t k, float de, float four_dt
     const bool ok = (k <= n_k) && (tb < (float)n_de);
     out = ok ? (((k - 1u) << 16) | (uint32_t)tb) : 0u;
     return ok;
+}
+
+struct BinSpec {
+    float four_dt, dde;
+    uint32_t n_k, n_de;
+};
+
+// The stage holds raw (k, de) pairs; they are binned here, with every lane of
+// the wave busy, rather than when a track closes (where a whole wave would run
+// the two divisions for one or two closing lanes). The kept records are
+// compacted in place: batch i0 reads words [2 i0, 2 i0 + 128) and writes below
+// word i0 + 64, so it never overwrites a pair a later batch still reads.
+__device__ __forceinline__ void stage_flush_binned(Stage& st, const BinSpec& bs)
+{
+    if (st.cnt == 0u) return;
+    wave_lds_sync();
+    const uint32_t n = st.cnt >> 1;
+    uint32_t kept = 0u;
+    for (uint32_t i0 = 0u; i0 < n; i0 += (uint32_t)kWave) {
+        const uint32_t i = i0 + lane_id();
+        uint32_t rv = 0u;
+        bool ok = false;
+        if (i < n) ok = track_record(st.buf[2u * i], __uint_as_float(st.buf[2u * i + 1u]), bs.four_dt, bs.dde,
+                                     bs.n_k, bs.n_de, rv);
+        const uint64_t m = ballot(ok);
+        wave_lds_sync();
+        if (ok) st.buf[kept + mbcnt64(m)] = rv;
+        kept += (uint32_t)__builtin_popcountll(m);
+    }
+    st.cnt = kept;
+    stage_flush(st);
+}
+
+// Two-word records: (k, DE bits) of the track-list form (BIN = false) or the
+// raw (k, de bits) pairs that stage_flush_binned bins (BIN = true).
+template <bool BIN>
+__device__ __forceinline__ void stage_push2(Stage& st, bool ok, uint32_t a, uint32_t b, const BinSpec& bs)
+{
+    const uint64_t m = ballot(ok);
+    if (m == 0ull) return;
+    if (st.cnt + (uint32_t)(2 * kWave) > st.cap) {
+        if (BIN) stage_flush_binned(st, bs);
+        else stage_flush(st);
+    }
+    const uint32_t pos = st.cnt + 2u * mbcnt64(m);
+    if (ok) {
+        st.buf[pos] = a;
+        st.buf[pos + 1u] = b;
+    }
+    st.cnt += 2u * (uint32_t)__builtin_popcountll(m);
 }
 
 // ------------------------------------------------------------------- walk
@@ -307,8 +333,9 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
     const float ox = hax / Lx - 0.5f, oy = hay / Ly - 0.5f;  // window origin offsets
     const float wx = (hax + hax) / Lx, wy = (hay + hay) / Ly;  // window widths
     const float sig2d = sy.detect_sigma * sy.detect_sigma;
-    const uint32_t kmin = sy.kmin, n_k = sy.n_k, n_de = sy.n_de, C = sy.n_species;
-    const float four_dt = sy.four_dt, dde = sy.de_bin_width;
+    const uint32_t kmin = sy.kmin, C = sy.n_species;
+    const float four_dt = sy.four_dt;
+    const BinSpec bs{four_dt, sy.de_bin_width, sy.n_k, sy.n_de};
     // Philox blocks per 4-step chunk: 2 for the 12 displacements (+1 with
     // Gaussian detection for its 4 uniforms).
     constexpr uint32_t B = GAUSS ? 3u : 2u;
@@ -415,16 +442,9 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                 const uint64_t m_closing = m_active[i] & ~(ballot(cx) & ballot(cy) & ballot(cz)) & ballot(q.k >= kmin);
                 if (m_closing != 0ull) {
                     const bool closing = ((m_closing >> lane_id()) & 1ull) != 0ull;
-                    if (TRACKS) {
-                        float v = 0.0f;
-                        if (closing) v = track_de(q) / ((float)(q.k - 1u) * four_dt);
-                        stage_push2(st, closing, q.k, __float_as_uint(v));
-                    } else {
-                        uint32_t rv = 0u;
-                        bool ok = false;
-                        if (closing) ok = track_record(q.k, track_de(q), four_dt, dde, n_k, n_de, rv);
-                        stage_push(st, ok, rv);
-                    }
+                    float v = track_de(q);
+                    if (TRACKS && closing) v = v / ((float)(q.k - 1u) * four_dt);
+                    stage_push2<!TRACKS>(st, closing, q.k, __float_as_uint(v), bs);
                 }
                 const float de_n = __builtin_fmaf(ny, ny, __builtin_fmaf(nx, nx, q.de));
                 q.de = (det && q.prev) ? de_n : 0.0f;
@@ -438,19 +458,13 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
     for (int i = 0; i < PPL; ++i) {
         const bool closing = pt[i].active && pt[i].k >= kmin;
         if (ballot(closing) != 0ull) {
-            if (TRACKS) {
-                float v = 0.0f;
-                if (closing) v = track_de(pt[i]) / ((float)(pt[i].k - 1u) * four_dt);
-                stage_push2(st, closing, pt[i].k, __float_as_uint(v));
-            } else {
-                uint32_t rv = 0u;
-                bool ok = false;
-                if (closing) ok = track_record(pt[i].k, track_de(pt[i]), four_dt, dde, n_k, n_de, rv);
-                stage_push(st, ok, rv);
-            }
+            float v = track_de(pt[i]);
+            if (TRACKS && closing) v = v / ((float)(pt[i].k - 1u) * four_dt);
+            stage_push2<!TRACKS>(st, closing, pt[i].k, __float_as_uint(v), bs);
         }
     }
-    stage_flush(st);
+    if (TRACKS) stage_flush(st);
+    else stage_flush_binned(st, bs);
 }
 
 // Tile t of a batch plan: the video owning it is the largest v with
