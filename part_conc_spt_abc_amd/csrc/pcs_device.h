@@ -55,9 +55,10 @@ __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
     uint32_t y3 = kPhiloxM0 * x0;                       // varying
     k0 += kPhiloxW0;
     k1 += kPhiloxW1;
-    // round 3: y2, y3 vary
-    c0 = xor3(mulhi(kPhiloxM1, y2), y1, k0);
-    c1 = kPhiloxM1 * y2;
+    // round 3: y2, y3 vary (both halves of M1 y2 from one 32x32->64 multiply)
+    const uint64_t p1 = (uint64_t)kPhiloxM1 * y2;
+    c0 = xor3((uint32_t)(p1 >> 32), y1, k0);
+    c1 = (uint32_t)p1;
     c2 = xor3(mulhi(kPhiloxM0, y0), y3, k1);
     c3 = kPhiloxM0 * y0;  // uniform
     k0 += kPhiloxW0;
