@@ -39,13 +39,15 @@ from part_conc_spt_abc_amd import _abi  # noqa: E402
 from part_conc_spt_abc_amd.gpu import Context, Server  # noqa: E402
 from part_conc_spt_abc_amd.model import Model, proposal_record, stokes_einstein  # noqa: E402
 
-# Algorithmic VALU lane-operations per particle-step of the walk (Philox
-# 0.75 blocks x 104, Box-Muller 12 counting log/sqrt/sin/cos as one each,
-# 6 uniform conversions, 27 walk/wrap/detect/track) and MI355X's FP32 vector
-# peak in lane-operations: 157.3 TFLOP/s / 2 flops per FMA = 78.6e12/s
-# (256 CU x 128 lanes/cycle x 2.4 GHz; measured issue rates of the walk
-# exceed the 64 lanes/cycle/CU of a 4-cycle wave64 issue, see profiles/).
-OPS_PER_STEP = 123
+# The walk is bound by VALU issue. Its roofline counts the VALU lane-
+# instructions the kernel issues per particle-step (rocprofv3 SQ_INSTS_VALU x 64
+# over the profiled launch's particle-steps, kept with the HBM bytes in
+# profiles/walk_kernel_pmc.json) against MI355X's FP32 vector peak in lane-
+# instructions: 157.3 TFLOP/s / 2 flops per FMA = 78.65e12/s (256 CU x 128
+# lanes/cycle x 2.4 GHz). A reference-style count of the work (0.75 Philox
+# blocks x 104 ops, polynomial log/sqrt/sin/cos: 123 lane-ops per step)
+# overstates what this walk needs.
+PMC_PROFILE = ROOT / "profiles" / "walk_kernel_pmc.json"
 VALU_PEAK = 157.3e12 / 2
 
 D175 = stokes_einstein(175.0)
@@ -360,16 +362,13 @@ def main():
             return None
         return sweep[1][key] if len(sweep) > 1 else sweep[0][key]
 
-    ach = steps_per_gen * OPS_PER_STEP / walk_s if walk_s == walk_s else None
-    # HBM bytes per launch: FETCH_SIZE/WRITE_SIZE per particle-step from the
-    # committed rocprofv3 runs (copy-calibrated), times this launch's steps.
-    traffic = None
-    pmc = ROOT / "profiles" / "walk_kernel_pmc.json"
-    if pmc.exists():
-        try:
-            traffic = json.loads(pmc.read_text())["hbm_bytes_per_particle_step"] * steps_per_gen
-        except Exception:
-            traffic = None
+    # Per-particle-step VALU lane-instructions and HBM bytes (FETCH_SIZE /
+    # WRITE_SIZE, copy-calibrated) from the committed rocprofv3 runs, times
+    # this launch's particle-steps.
+    prof = json.loads(PMC_PROFILE.read_text())
+    valu_step = prof["valu_lane_instr_per_particle_step"]
+    traffic = prof["hbm_bytes_per_particle_step"] * steps_per_gen
+    ach = steps_per_gen * valu_step / walk_s if walk_s == walk_s else None
     line = {
         "metric": "ABC sims/sec & particle-steps/s, batched vs plain; p50/p99 sim latency",
         "value": sims_s,
@@ -413,12 +412,12 @@ def main():
             "bound": "valu",
             "achieved": ach / 1e12 if ach else None,
             "peak": VALU_PEAK / 1e12,
-            "unit": "Tlane-op/s",
+            "unit": "Tlane-instr/s",
             "frac": (ach / VALU_PEAK) if ach else None,
             "traffic": traffic,
             "kernel": "walk_kernel",
             "kernel_ms": walk_s * 1e3,
-            "ops_per_particle_step": OPS_PER_STEP,
+            "valu_lane_instr_per_particle_step": valu_step,
         },
         "cpu_baseline": cpu,
     }
