@@ -521,6 +521,8 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
 // --------------------------------------------------------------- distance
 
 constexpr int kDistThreads = 256;
+// Rows with at most this many records skip the row histogram and block scan.
+constexpr int kDistSparseRow = 32;
 
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total)
 {
@@ -624,7 +626,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     __threadfence_block();
     __syncthreads();
 
-    const uint32_t last = max(*L.smax, sy.maxrow_obs);
+    const uint32_t last = max(*L.smax, sy.maxrow_obs);  // < n_k
     const uint32_t j0 = tid * SEG;
     uint32_t Ccol[SEG];
 #pragma unroll
@@ -632,50 +634,79 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     uint64_t acc = 0ull;
     const bool l1 = sy.dist_kind == PCS_DIST_L1;
     const uint32_t* R = sy.obs_sat;
-
     const uint32_t maxrow_obs = sy.maxrow_obs;
+    const uint32_t lane = tid % kWave;
+
+    // Observed rows are loaded PF rows ahead of their use into a ring of
+    // register slots (slot d holds row i0 + d); a refill asks for row
+    // min(i + PF, last), which exists, so no load leaves the table.
+    constexpr int PF = SEG >= 8 ? 2 : (SEG >= 4 ? 4 : 8);
+    uint32_t rb[PF][SEG];
+    auto load_row = [&](uint32_t (&dst)[SEG], uint32_t row) {
+        const uint32_t* Rrow = R + (uint64_t)row * n_de;
+#pragma unroll
+        for (int m = 0; m < SEG; ++m) dst[m] = (j0 + m < n_de) ? Rrow[j0 + m] : 0u;
+    };
+#pragma unroll
+    for (int d = 0; d < PF; ++d) load_row(rb[d], min((uint32_t)d, last));
+
     uint64_t racc_prev = 0ull;
-    for (uint32_t i = 0; i <= last; ++i) {
-        const uint32_t s0 = (i == 0u) ? 0u : rowc[i - 1u];
-        const uint32_t s1 = rowc[i];
-        // Past the deepest observed row R repeats its row; a row without
-        // records repeats S's row, so it contributes what the row above did.
-        if (s1 == s0 && i > maxrow_obs && i != last) {
-            acc += racc_prev;
-            continue;
-        }
-        if (s1 > s0) {
-            for (uint32_t t = s0 + tid; t < s1; t += kDistThreads) atomicAdd(&colh[srt[t]], 1u);
-            __syncthreads();
-            uint32_t pre[SEG];
-            uint32_t run = 0u;
+    for (uint32_t i0 = 0; i0 <= last; i0 += PF) {
 #pragma unroll
-            for (int m = 0; m < SEG; ++m) {
-                run += (j0 + m < n_de) ? colh[j0 + m] : 0u;
-                pre[m] = run;
+        for (int d = 0; d < PF; ++d) {
+            const uint32_t i = i0 + (uint32_t)d;
+            if (i > last) break;
+            const uint32_t s0 = (i == 0u) ? 0u : rowc[i - 1u];
+            const uint32_t s1 = rowc[i];
+            // Past the deepest observed row R repeats its row; a row without
+            // records repeats S's row, so it contributes what the row above did.
+            if (s1 == s0 && i > maxrow_obs && i != last) {
+                acc += racc_prev;
+            } else {
+                const uint32_t nr = s1 - s0;
+                if (nr > 0u && nr <= (uint32_t)kDistSparseRow) {
+                    // A few records: every wave reads them once (one per lane)
+                    // and each thread adds those at or left of its columns.
+                    const uint32_t mine = (lane < nr) ? srt[s0 + lane] : 0u;
+                    for (uint32_t q = 0u; q < nr; ++q) {
+                        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)q);
+#pragma unroll
+                        for (int m = 0; m < SEG; ++m) Ccol[m] += (j0 + (uint32_t)m >= b) ? 1u : 0u;
+                    }
+                } else if (nr > 0u) {
+                    for (uint32_t t = s0 + tid; t < s1; t += kDistThreads) atomicAdd(&colh[srt[t]], 1u);
+                    __syncthreads();
+                    uint32_t pre[SEG];
+                    uint32_t run = 0u;
+#pragma unroll
+                    for (int m = 0; m < SEG; ++m) {
+                        run += (j0 + m < n_de) ? colh[j0 + m] : 0u;
+                        pre[m] = run;
+                    }
+                    uint32_t tot;
+                    const uint32_t ex = block_excl_scan(run, wsum, tot);
+#pragma unroll
+                    for (int m = 0; m < SEG; ++m) Ccol[m] += ex + pre[m];
+#pragma unroll
+                    for (int m = 0; m < SEG; ++m)
+                        if (j0 + m < n_de) colh[j0 + m] = 0u;
+                    __syncthreads();
+                }
+                uint64_t racc = 0ull;
+#pragma unroll
+                for (int m = 0; m < SEG; ++m) {
+                    if (j0 + m < n_de) {
+                        const uint32_t rv = rb[d][m];
+                        const uint32_t dd = Ccol[m] > rv ? Ccol[m] - rv : rv - Ccol[m];
+                        racc += l1 ? (uint64_t)dd : (uint64_t)dd * dd;
+                    }
+                }
+                // Rows past `last` repeat row `last` in both tables.
+                acc += (i == last) ? racc * (uint64_t)(n_k - last) : racc;
+                racc_prev = racc;
             }
-            uint32_t tot;
-            const uint32_t ex = block_excl_scan(run, wsum, tot);
-#pragma unroll
-            for (int m = 0; m < SEG; ++m) Ccol[m] += ex + pre[m];
-#pragma unroll
-            for (int m = 0; m < SEG; ++m)
-                if (j0 + m < n_de) colh[j0 + m] = 0u;
-            __syncthreads();
+            load_row(rb[d], min(i + (uint32_t)PF, last));
         }
-        uint64_t racc = 0ull;
-        const uint32_t* Rrow = R + (uint64_t)i * n_de;
-#pragma unroll
-        for (int m = 0; m < SEG; ++m) {
-            if (j0 + m < n_de) {
-                const uint32_t rv = Rrow[j0 + m];
-                const uint32_t dd = Ccol[m] > rv ? Ccol[m] - rv : rv - Ccol[m];
-                racc += l1 ? (uint64_t)dd : (uint64_t)dd * dd;
-            }
-        }
-        // Rows past `last` repeat row `last` in both tables.
-        acc += (i == last) ? racc * (uint64_t)(n_k - last) : racc;
-        racc_prev = racc;
     }
 
     red[tid] = acc;
