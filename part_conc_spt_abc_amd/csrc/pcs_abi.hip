@@ -313,7 +313,7 @@ int enqueue_batched(pcs_batch* b, bool with_events)
     PCS_HIP(launch_walk(walk_args(b, 0, b->n_tiles), (int)t.particles_per_lane, (int)t.waves_per_group, true, grid,
                         gauss, zref, b->stream));
     if (with_events) PCS_HIP(hipEventRecord(b->ev[2], b->stream));
-    PCS_HIP(launch_distance(dist_args(b, 0, b->n_props), c->sys.n_de, b->stream));
+    PCS_HIP(launch_distance(dist_args(b, 0, b->n_props), c->sys.n_k, c->sys.n_de, b->stream));
     if (with_events) PCS_HIP(hipEventRecord(b->ev[3], b->stream));
     return PCS_OK;
 }
@@ -333,7 +333,7 @@ int enqueue_plain(pcs_batch* b, const std::vector<uint32_t>* subset)
         const uint32_t lo = b->vtile[p.first_video], hi = b->vtile[p.first_video + p.n_videos];
         PCS_HIP(launch_walk(walk_args(b, lo, hi), (int)t.particles_per_lane, (int)t.waves_per_group, false, 0, gauss,
                             zref, b->stream));
-        PCS_HIP(launch_distance(dist_args(b, P, 1), c->sys.n_de, b->stream));
+        PCS_HIP(launch_distance(dist_args(b, P, 1), c->sys.n_k, c->sys.n_de, b->stream));
     }
     return PCS_OK;
 }

@@ -72,6 +72,7 @@ struct DistArgs {
     uint64_t* exact;            // integer sum per proposal
     uint32_t* tracks;           // binned tracks per proposal
     int32_t* status;            // PCS_OK or PCS_EOVERFLOW per proposal
+    uint32_t colh_off;          // LDS word offset of the column histogram (set by launch_distance)
 };
 
 // ------------------------------------------------ device-resident request loop
@@ -153,7 +154,7 @@ struct LoopArgs {
 hipError_t launch_reset(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter, hipStream_t s);
 hipError_t launch_walk(const WalkArgs& a, int ppl, int waves_per_group, bool persistent,
                        uint32_t grid_groups, bool gauss, bool zreflect, hipStream_t s);
-hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s);
+hipError_t launch_distance(const DistArgs& a, uint32_t n_k_max, uint32_t n_de_max, hipStream_t s);
 // Track-list form (k, DE) of the walk, one wave per tile, tiles of 64 particles.
 hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s);
 hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, const double* w_old, uint32_t n_old,

@@ -523,6 +523,8 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
 constexpr int kDistThreads = 256;
 // Rows with at most this many records skip the row histogram and block scan.
 constexpr int kDistSparseRow = 32;
+// Record loads each thread keeps in flight while reading a slice.
+constexpr int kDistLoads = 8;
 
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total)
 {
@@ -591,12 +593,24 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     if (tid == 0) *L.smax = 0u;
     __syncthreads();
 
-    // Row histogram of the records and the deepest occupied row.
+    // Row histogram of the records and the deepest occupied row. The records
+    // come from HBM: each thread keeps kDistLoads loads in flight.
     uint32_t mr = 0u;
-    for (uint32_t t = tid; t < T; t += kDistThreads) {
-        const uint32_t r = rec[t] >> 16;
-        atomicAdd(&rowc[r], 1u);
-        mr = max(mr, r);
+    for (uint32_t t0 = tid; t0 < T; t0 += kDistLoads * kDistThreads) {
+        uint32_t v[kDistLoads];
+#pragma unroll
+        for (int u = 0; u < kDistLoads; ++u) {
+            const uint32_t t = t0 + (uint32_t)u * kDistThreads;
+            v[u] = (t < T) ? rec[t] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kDistLoads; ++u) {
+            if (t0 + (uint32_t)u * kDistThreads < T) {
+                const uint32_t r = v[u] >> 16;
+                atomicAdd(&rowc[r], 1u);
+                mr = max(mr, r);
+            }
+        }
     }
     atomicMax(L.smax, mr);
     __syncthreads();
@@ -618,10 +632,20 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     __syncthreads();
 
     // Scatter bins by row; afterwards rowc[r] is one past the last slot of row r.
-    for (uint32_t t = tid; t < T; t += kDistThreads) {
-        const uint32_t rv = rec[t];
-        const uint32_t pos = atomicAdd(&rowc[rv >> 16], 1u);
-        srt[pos] = rv & 0xffffu;
+    for (uint32_t t0 = tid; t0 < T; t0 += kDistLoads * kDistThreads) {
+        uint32_t v[kDistLoads];
+#pragma unroll
+        for (int u = 0; u < kDistLoads; ++u) {
+            const uint32_t t = t0 + (uint32_t)u * kDistThreads;
+            v[u] = (t < T) ? rec[t] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kDistLoads; ++u) {
+            if (t0 + (uint32_t)u * kDistThreads < T) {
+                const uint32_t pos = atomicAdd(&rowc[v[u] >> 16], 1u);
+                srt[pos] = v[u] & 0xffffu;
+            }
+        }
     }
     __threadfence_block();
     __syncthreads();
@@ -719,11 +743,15 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     return DistOut{red[0], T, PCS_OK};
 }
 
+// Dynamic LDS: row counts (n_k_max words, rounded up to 4) then one row's
+// column histogram (n_de_max words), sized to the models of the launch so
+// that more workgroups share a CU.
 template <int SEG>
 __global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
 {
-    __shared__ uint32_t rowc[PCS_MAX_NK];
-    __shared__ uint32_t colh[PCS_MAX_NDE];
+    extern __shared__ uint32_t dist_lds[];
+    uint32_t* rowc = dist_lds;
+    uint32_t* colh = dist_lds + a.colh_off;
     __shared__ uint32_t wsum[kDistThreads / kWave];
     __shared__ uint32_t smax;
     __shared__ uint64_t red[kDistThreads];
@@ -1283,16 +1311,21 @@ hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_distance(const DistArgs& a, uint32_t n_de_max, hipStream_t s)
+hipError_t launch_distance(const DistArgs& args, uint32_t n_k_max, uint32_t n_de_max, hipStream_t s)
 {
     (void)hipGetLastError();  // report only this launch's error
-    if (a.n_props == 0) return hipSuccess;
+    if (args.n_props == 0) return hipSuccess;
+    if (n_k_max == 0u || n_k_max > PCS_MAX_NK || n_de_max == 0u || n_de_max > PCS_MAX_NDE)
+        return hipErrorInvalidValue;
+    DistArgs a = args;
+    a.colh_off = (n_k_max + 3u) & ~3u;
+    const size_t shmem = ((size_t)a.colh_off + n_de_max) * sizeof(uint32_t);
     const dim3 g(a.n_props), b(kDistThreads);
-    if (n_de_max <= 256) hipLaunchKernelGGL(distance_kernel<1>, g, b, 0, s, a);
-    else if (n_de_max <= 512) hipLaunchKernelGGL(distance_kernel<2>, g, b, 0, s, a);
-    else if (n_de_max <= 1024) hipLaunchKernelGGL(distance_kernel<4>, g, b, 0, s, a);
-    else if (n_de_max <= 2048) hipLaunchKernelGGL(distance_kernel<8>, g, b, 0, s, a);
-    else hipLaunchKernelGGL(distance_kernel<16>, g, b, 0, s, a);
+    if (n_de_max <= 256) hipLaunchKernelGGL(distance_kernel<1>, g, b, shmem, s, a);
+    else if (n_de_max <= 512) hipLaunchKernelGGL(distance_kernel<2>, g, b, shmem, s, a);
+    else if (n_de_max <= 1024) hipLaunchKernelGGL(distance_kernel<4>, g, b, shmem, s, a);
+    else if (n_de_max <= 2048) hipLaunchKernelGGL(distance_kernel<8>, g, b, shmem, s, a);
+    else hipLaunchKernelGGL(distance_kernel<16>, g, b, shmem, s, a);
     return hipGetLastError();
 }
 

@@ -255,7 +255,7 @@ int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vect
     g.submit_us.resize(n);
     g.arena_off.assign(offs.begin(), offs.end());
     g.arena_n.resize(n);
-    uint32_t n_de_max = 1;
+    uint32_t n_de_max = 1, n_k_max = 1;
     uint32_t classes = 0;  // bit (gauss*2 + zref) present
     for (uint32_t i = 0; i < n; ++i) {
         const Pending& p = batch[i];
@@ -270,6 +270,7 @@ int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vect
         g.submit_us[i] = p.submit_us;
         g.arena_n[i] = p.cap;
         n_de_max = std::max(n_de_max, sy.n_de);
+        n_k_max = std::max(n_k_max, sy.n_k);
         classes |= 1u << ((sy.detect_model == PCS_DETECT_GAUSS_Z ? 2 : 0) + (sy.z_boundary == PCS_Z_REFLECT ? 1 : 0));
     }
     g.n = n;
@@ -322,7 +323,7 @@ int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vect
         d.exact = g.d_exact;
         d.tracks = g.h_tracks;
         d.status = g.h_status;
-        e = launch_distance(d, n_de_max, st);
+        e = launch_distance(d, n_k_max, n_de_max, st);
     }
     if (e == hipSuccess) e = hipEventRecord(g.done, st);
     if (e != hipSuccess) return hip_error(e, "serving launch");
