@@ -82,6 +82,27 @@ def test_headline_single_simulation(gpu_lib, mode):
     check_against_oracle(model, obs, [p], f, n, 1, dist, tracks, H, exact)
 
 
+@pytest.mark.parametrize("mode", [_abi.PCS_MODE_PLAIN, _abi.PCS_MODE_BATCHED])
+def test_maximum_histogram_shape(gpu_lib, mode):
+    """The ABI's largest histogram, n_k = n_de = 4096 (the distance's widest
+    column segments and its largest LDS), with rows dense and sparse in
+    records, and a proposal whose particles never leave the window (slab
+    deeper than the box, full lateral window), so every track outlasts n_k
+    and is dropped; against the oracle."""
+    model = cases.headline_model(n_k=_abi.PCS_MAX_NK, n_de=_abi.PCS_MAX_NDE, ax=200.0, ay=200.0)
+    obs = observed(model, particles=800, frames=[2000])
+    m, c = [cases.D175, cases.D500], [5e8, 5e8]
+    props = [cases.proposal_record(model, m, c, [2.0], 1, 0, 1),
+             cases.proposal_record(model, m, c, [2.0], 2, 1, 2),
+             cases.proposal_record(model, m, c, [60.0], 3, 3, 1)]
+    f = np.array([4100, 1500, 300, 4100], np.uint32)
+    n = np.array([2500, 3000, 500, 40], np.uint32)
+    with Context(model, obs) as ctx:
+        dist, tracks, H, exact, _ = run_gpu(ctx, props, f, n, 17, mode)
+    assert tracks[2] == 0 and not H[2].any()
+    check_against_oracle(model, obs, props, f, n, 17, dist, tracks, H, exact)
+
+
 def test_batched_mixed_sizes(gpu_lib):
     model = cases.headline_model()  # H 1000 x 500
     obs = observed(model)
