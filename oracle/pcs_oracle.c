@@ -235,6 +235,36 @@ static inline float neg2log_i(uint32_t w)
 }
 
 float orc_neg2log_u24(uint32_t w) { return neg2log_i(w); }
+
+/* 2^f = 1 + f Q(f) on [0, 1) (fit_math_coeffs.py). */
+static const float EXP2_Q[6] = {
+    0x1.62e42cp-1f, 0x1.ebfd58p-3f, 0x1.c68912p-5f, 0x1.3d0b92p-7f, 0x1.46d64cp-10f, 0x1.c54176p-13f,
+};
+static const float LOG2E_F = 0x1.715476p0f;
+
+static inline float exp2_01(float f)
+{
+    float Q = EXP2_Q[5];
+    for (int i = 4; i >= 0; --i) Q = fmaf(Q, f, EXP2_Q[i]);
+    return fmaf(f, Q, 1.0f);
+}
+
+/* Gaussian detection in z (model extension): detected iff j <= 2^(24 - c z^2)
+ * for j = (w >> 8) + 1 in [1, 2^24] and c = log2(e) / (2 s^2), i.e. with
+ * probability exp(-z^2 / (2 s^2)); the exponent splits into floor n and
+ * fraction f (exact), 2^f by the polynomial, scaled by 2^n. */
+static inline float gauss_c_i(float sig2) { return LOG2E_F / (sig2 + sig2); }
+static inline int gauss_detect_i(float z, float c, uint32_t w)
+{
+    const float v = fmaf(-c, z * z, 24.0f);
+    const float n = floorf(v);
+    const float t = ldexpf(exp2_01(v - n), (int)n);
+    return (float)((w >> 8) + 1u) <= t;
+}
+
+float orc_exp2_01(float f) { return exp2_01(f); }
+float orc_gauss_c(float sig2) { return gauss_c_i(sig2); }
+int orc_gauss_detect(float z, float c, uint32_t w) { return gauss_detect_i(z, c, w); }
 static inline float bm_radius_i(uint32_t w) { return sqrtf(neg2log_i(w)); }
 float orc_bm_radius(uint32_t w) { return bm_radius_i(w); }
 
@@ -360,7 +390,7 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
     const float wx = (hax + hax) / Lx, wy = (hay + hay) / Ly;
     const int gauss = sys->detect_model == 1;
     const int zbox = !gauss && !sys->z_boundary;
-    const float sig2d = sys->detect_sigma * sys->detect_sigma;
+    const float cg = gauss_c_i(sys->detect_sigma * sys->detect_sigma);
     const uint32_t kmin = sys->kmin;
 
     for (uint32_t part = 0; part < N; ++part) {
@@ -387,7 +417,7 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
         if (gauss) {
             uint32_t wd[4];
             block(st, 0xffffffffu, part, video, wd);
-            det = x <= wx && y <= wy && (z * z) <= sig2d * neg2log_i(wd[0]);
+            det = x <= wx && y <= wy && gauss_detect_i(z, cg, wd[0]);
         } else {
             det = x <= wx && y <= wy && (zbox ? z <= wz : fabsf(z) <= wz);
         }
@@ -411,7 +441,7 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
             }
 
             if (gauss)
-                det = x <= wx && y <= wy && (z * z) <= sig2d * neg2log_i(wdet[sl]);
+                det = x <= wx && y <= wy && gauss_detect_i(z, cg, wdet[sl]);
             else
                 det = x <= wx && y <= wy && (zbox ? z <= wz : fabsf(z) <= wz);
 

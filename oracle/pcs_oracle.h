@@ -25,6 +25,9 @@
  *   orc_neg2log_u24       libm FP64 log, ulp bound, all 2^24 inputs
  *   orc_sincos16          libm FP64 sin/cos, error bound, all 2^16 codes; the
  *                         table against 50-digit values (fit_math_coeffs.py)
+ *   orc_exp2_01           numpy exp2, ulp bound on a 2^22-point grid; 2^0 = 1
+ *   orc_gauss_detect      the exact decision u <= exp(-z^2 / (2 s^2)) in FP64
+ *                         away from the boundary; detected fraction of a slab
  *   orc_chunk_normals_n   moments, KS against N(0,1), zero correlation between
  *                         the 12 normals of a chunk; orc_bm_angle16 byte layout
  *   orc_simulate          closed forms (sigma=0 tracks, chi-square DE law,
@@ -63,6 +66,12 @@ uint32_t orc_bm_angle16(uint32_t w_radius, uint32_t w_angle, uint32_t p);
  * out[(part * n_chunks + chunk) * 12 + i], i = 3 * step + axis. */
 void orc_chunk_normals_n(uint64_t seed, uint32_t sim_id, uint32_t video, uint32_t n_particles, uint32_t n_chunks,
                          int gauss, float* out);
+
+/* 2^f for f in [0, 1) and the Gaussian detection draw of the walk: c =
+ * log2(e) / (2 s^2) for s^2 = sig2; detected iff ((w >> 8) + 1) <= 2^(24 - c z^2). */
+float orc_exp2_01(float f);
+float orc_gauss_c(float sig2);
+int orc_gauss_detect(float z, float c, uint32_t w);
 
 /* Batched versions of the math helpers (for exhaustive tests). */
 void orc_bm_radius_n(const uint32_t* w, float* out, uint64_t n);

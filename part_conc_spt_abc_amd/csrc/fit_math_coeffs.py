@@ -7,6 +7,8 @@ hex floats.
   (weighted least squares on Chebyshev nodes, reweighted iteratively by the
   residual - Lawson's algorithm - to approach the minimax polynomial, then
   rounded to FP32);
+- 2^f = 1 + f Q(f), Q of degree 5, on [0, 1) (same method, weighted for the
+  relative error of 2^f; the constant 1 keeps 2^0 exact);
 - the two Box-Muller angle tables for the angle 2 pi (a + 1/2) / 2^16 of a
   16-bit code a = i << 8 | r, which is the sector centre 2 pi (i + 1/2) / 256
   plus beta = 2 pi (r - 127.5) / 2^16: SINCOS_TAB holds sin and cos of the
@@ -80,6 +82,10 @@ def main():
     A = np.stack([x ** p for p in range(8)], 1)
     logp = lawson(A, (np.log1p(x) - x) / x ** 2, np.abs(x) + 1e-12, 80).astype(np.float32)
     print("LOG_P", [hexf(v) for v in logp])
+    x = cheb(0.0, 1.0)
+    A = np.stack([x ** p for p in range(6)], 1)
+    exq = lawson(A, np.expm1(x * np.log(2.0)) / x, x / np.exp2(x), 80).astype(np.float32)
+    print("EXP2_Q", [hexf(v) for v in exq])
     print("SINCOS_TAB", [float.hex(v) for v in sincos_table()])
     print("BETA_TAB", [float.hex(v) for v in beta_table()])
 

@@ -1,6 +1,6 @@
-// pcs_device.h -- gfx950 device primitives of the walk kernel: Philox4x32-10,
-// the project's FP32 log / sqrt / sincos-in-turns, and the per-step walk,
-// wrap, detect and track logic.
+// pcs_device.h -- gfx950 device primitives of the walk kernel: Philox4x32-10
+// and the project's FP32 math: the Box-Muller log, sqrt and table sin/cos, and
+// the bound of the Gaussian detection draw.
 //
 // Every function here fixes its FP32 operation order: the library is built
 // with -ffp-contract=off and correctly rounded division/sqrt, fmaf() is the
@@ -131,6 +131,35 @@ __device__ __forceinline__ float neg2log_u24(uint32_t w)
     Q = __builtin_fmaf(Q, g, kLogQ0);
     const float l = __builtin_fmaf(g2, Q, g);
     return __builtin_fmaf(E, kNeg2Ln2Hi, __builtin_fmaf(E, kNeg2Ln2Lo, l));
+}
+
+// 2^f = 1 + f Q(f) on [0, 1); Q from fit_math_coeffs.py (within 1.07 ulp of 2^f,
+// and 2^0 = 1 exactly).
+constexpr float kExp2Q0 = 0x1.62e42cp-1f, kExp2Q1 = 0x1.ebfd58p-3f, kExp2Q2 = 0x1.c68912p-5f,
+                kExp2Q3 = 0x1.3d0b92p-7f, kExp2Q4 = 0x1.46d64cp-10f, kExp2Q5 = 0x1.c54176p-13f;
+constexpr float kLog2E = 0x1.715476p0f;
+
+// Gaussian detection in z (the model's extension of the slab): a particle at
+// z is detected with probability exp(-z^2 / (2 s^2)), drawn with the uniform
+// integer j = (w >> 8) + 1 in [1, 2^24] as j <= 2^24 exp(-z^2 / (2 s^2))
+// = 2^(24 - c z^2), c = log2(e) / (2 s^2) (gauss_c). 24 - c z^2 = n + f with
+// n = floor (so f in [0, 1) is exact), 2^f by the polynomial, scaled by 2^n;
+// at z = 0 the bound is 2^24 and detection is certain.
+__device__ __forceinline__ float gauss_c(float sig2) { return kLog2E / (sig2 + sig2); }
+
+__device__ __forceinline__ bool gauss_detect(float z, float c, uint32_t w)
+{
+    const float v = __builtin_fmaf(-c, z * z, 24.0f);
+    const float n = __builtin_floorf(v);
+    const float f = v - n;
+    float Q = kExp2Q5;
+    Q = __builtin_fmaf(Q, f, kExp2Q4);
+    Q = __builtin_fmaf(Q, f, kExp2Q3);
+    Q = __builtin_fmaf(Q, f, kExp2Q2);
+    Q = __builtin_fmaf(Q, f, kExp2Q1);
+    Q = __builtin_fmaf(Q, f, kExp2Q0);
+    const float t = __builtin_ldexpf(__builtin_fmaf(f, Q, 1.0f), (int)n);
+    return (float)((w >> 8) + 1u) <= t;
 }
 
 // Correctly rounded sqrt on the Box-Muller domain y = -2 ln u (about 2^-102,

@@ -332,7 +332,7 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
     const float hax = sy.half_a[0], hay = sy.half_a[1];
     const float ox = hax / Lx - 0.5f, oy = hay / Ly - 0.5f;  // window origin offsets
     const float wx = (hax + hax) / Lx, wy = (hay + hay) / Ly;  // window widths
-    const float sig2d = sy.detect_sigma * sy.detect_sigma;
+    const float cg = gauss_c(sy.detect_sigma * sy.detect_sigma);  // Gaussian detection scale
     const uint32_t kmin = sy.kmin, C = sy.n_species;
     const float four_dt = sy.four_dt;
     const BinSpec bs{four_dt, sy.de_bin_width, sy.n_k, sy.n_de};
@@ -374,7 +374,7 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
         bool det;
         if (GAUSS) {
             const U4 wd = philox10(0xffffffffu, q.id, video, sim, k0, k1);
-            det = q.x <= wx && q.y <= wy && (q.z * q.z) <= sig2d * neg2log_u24(wd.x);
+            det = q.x <= wx && q.y <= wy && gauss_detect(q.z, cg, wd.x);
         } else if (ZBOX) {
             det = q.x <= wx && q.y <= wy && q.z <= q.wz;
         } else {
@@ -434,7 +434,7 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                     q.z = ZREF ? wrap_reflect(zn, hz, Lz) : wrap_periodic(zn, hz, Lz);
                 }
                 const bool cx = q.x <= wx, cy = q.y <= wy;
-                const bool cz = GAUSS ? (q.z * q.z) <= sig2d * neg2log_u24(wdet[i][s])
+                const bool cz = GAUSS ? gauss_detect(q.z, cg, wdet[i][s])
                                       : (ZBOX ? q.z <= q.wz : __builtin_fabsf(q.z) <= q.wz);
                 const bool det = cx && cy && cz;
                 // Lanes closing a track: the wave mask is formed from the masks of

@@ -64,6 +64,12 @@ def lib():
         L.orc_philox4x32_10.argtypes = [vp, vp, vp]
         L.orc_neg2log_u24.argtypes = [C.c_uint32]
         L.orc_neg2log_u24.restype = C.c_float
+        L.orc_exp2_01.argtypes = [C.c_float]
+        L.orc_exp2_01.restype = C.c_float
+        L.orc_gauss_c.argtypes = [C.c_float]
+        L.orc_gauss_c.restype = C.c_float
+        L.orc_gauss_detect.argtypes = [C.c_float, C.c_float, C.c_uint32]
+        L.orc_gauss_detect.restype = C.c_int
         L.orc_bm_radius.argtypes = [C.c_uint32]
         L.orc_bm_radius.restype = C.c_float
         L.orc_bm_radius_n.argtypes = [vp, vp, C.c_uint64]

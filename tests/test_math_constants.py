@@ -29,6 +29,8 @@ def test_device_and_oracle_constants_match_fit():
     orc = (ROOT / "oracle/pcs_oracle.c").read_text()
     assert floats(re.search(r"constexpr float kLogP0.*?;", dev, re.S).group(0)) == v["LOG_P"]
     assert floats(re.search(r"LOG_P\[8\] = \{.*?\};", orc, re.S).group(0)) == v["LOG_P"]
+    assert floats(re.search(r"constexpr float kExp2Q0.*?;", dev, re.S).group(0)) == v["EXP2_Q"]
+    assert floats(re.search(r"EXP2_Q\[6\] = \{.*?\};", orc, re.S).group(0)) == v["EXP2_Q"]
     # the two 256-entry angle tables, bit for bit in both copies
     for name_dev, name_orc, key in (("kSinCosTab", "SINCOS_TAB", "SINCOS_TAB"), ("kBetaTab", "BETA_TAB", "BETA_TAB")):
         tab_dev = floats(re.search(name_dev + r"\[kSinCosTabFloats\] = \{.*?\};", ker, re.S).group(0))

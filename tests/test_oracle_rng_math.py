@@ -6,8 +6,11 @@
 - log / sqrt / sincos: every one of the 2^24 inputs the walk can feed them,
   against libm FP64 through numpy, with ulp bounds; plus identities a wrong
   quadrant, sign or swapped pair would break.
+- the Gaussian detection draw: its 2^f polynomial against numpy's exp2, and
+  its decision against the exact rule u <= exp(-z^2 / (2 s^2)) in FP64.
 """
 import json
+import math
 import pathlib
 
 import numpy as np
@@ -111,6 +114,43 @@ def test_angle_layout():
     assert [L.orc_bm_angle16(wr, wa, p) for p in range(4)] == [0xABAA, 0xABBB, 0xABCC, 0xABDD]
     assert L.orc_bm_angle16(0xFFFFFF00, 0xFFFFFFFF, 0) == 0x00FF  # the radius bits never leak in
     assert L.orc_bm_angle16(0x000000FF, 0, 1) == 0xFF00
+
+
+def test_exp2_polynomial():
+    """2^f on [0, 1) within 1.1 ulp of numpy's exp2 at 4096 points spread over
+    the interval, and 2^0 = 1 exactly (detection is certain at z = 0)."""
+    L = O.lib()
+    f = np.linspace(0.0, 1.0, 1 << 22, endpoint=False).astype(np.float32)
+    got = np.array([L.orc_exp2_01(float(x)) for x in f[:: 1 << 10]], np.float64)
+    want = np.exp2(f[:: 1 << 10].astype(np.float64))
+    assert (np.abs(got - want) / np.spacing(want.astype(np.float32))).max() <= 1.1
+    assert L.orc_exp2_01(0.0) == 1.0
+
+
+def test_gauss_detect_decision():
+    """The detection draw equals the exact decision u <= exp(-z^2 / (2 s^2)),
+    u = ((w >> 8) + 1) 2^-24, computed in FP64, for every (z, w) whose u is
+    not within 1e-6 relative of the bound; at z = 0 every w is detected, and
+    over uniform w the detected fraction is exp(-z^2 / (2 s^2))."""
+    L = O.lib()
+    rng = np.random.default_rng(3)
+    for sig in (0.5, 1.0, 2.5):
+        c = L.orc_gauss_c(np.float32(sig) * np.float32(sig))
+        z = rng.uniform(-4 * sig, 4 * sig, 4000).astype(np.float32)
+        w = rng.integers(0, 2 ** 32, 4000, dtype=np.uint64).astype(np.uint32)
+        got = np.array([L.orc_gauss_detect(float(a), c, int(b)) for a, b in zip(z, w)], bool)
+        u = ((w >> 8).astype(np.float64) + 1.0) * 2.0 ** -24
+        bound = np.exp(-z.astype(np.float64) ** 2 / (2.0 * np.float32(sig) ** 2))
+        clear = np.abs(u - bound) > 1e-6 * bound
+        assert np.array_equal(got[clear], (u <= bound)[clear])
+        assert clear.mean() > 0.99
+        assert all(L.orc_gauss_detect(0.0, c, int(x)) for x in (0, 0xFFFFFFFF, 0x12345678))
+    c = L.orc_gauss_c(1.0)
+    w = rng.integers(0, 2 ** 32, 20000, dtype=np.uint64).astype(np.uint32)
+    for zz in (0.5, 1.0, 2.0):
+        frac = np.mean([L.orc_gauss_detect(zz, c, int(b)) for b in w])
+        p = math.exp(-zz * zz / 2.0)
+        assert abs(frac - p) < 5 * math.sqrt(p * (1 - p) / len(w))
 
 
 @pytest.mark.parametrize("gauss", [False, True])
