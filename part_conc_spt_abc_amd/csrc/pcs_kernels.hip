@@ -4,16 +4,19 @@
 //                    1 lane = PPL particles with all state in VGPRs. Replaces
 //                    K_simulate_system_unr (src/gpu/cusim/kernel-simulate-unroll.inl:69-298),
 //                    where one 32-wide warp looped over all particles of a video.
-//                    Closed tracks are packed (row<<16 | bin) and staged in a
-//                    per-wave LDS buffer, then appended to the proposal's HBM
-//                    record slice with one atomic per flush (replaces the
-//                    per-track global atomicInc of src/gpu/cusim/histogram2d.inl:98-106).
+//                    Closed tracks are staged raw, (k, de), in a per-wave LDS
+//                    buffer, binned (row<<16 | bin) when the buffer is flushed,
+//                    and appended to the proposal's HBM record slice with one
+//                    atomic per flush (replaces the per-track global atomicInc
+//                    of src/gpu/cusim/histogram2d.inl:98-106).
 //   distance_kernel  one workgroup per proposal: counting sort of the records
-//                    by row in LDS, then a row wavefront over the 2D
-//                    cumulative histogram with the running column sums kept
-//                    in registers and each row's prefix formed by an LDS block
-//                    scan; exact integer sum of (S-R)^2 or |S-R| against the
-//                    observed summed-area table. Replaces K_distance
+//                    by row (LDS row counters, scratch slice in HBM), then a
+//                    row sweep over the 2D cumulative histogram with the
+//                    running column sums kept in registers: a row's records
+//                    enter through an LDS histogram and block scan, or, when
+//                    few, straight into each thread's columns; exact integer
+//                    sum of (S-R)^2 or |S-R| against the observed summed-area
+//                    table, whose rows are loaded ahead. Replaces K_distance
 //                    (src/gpu/cusim/kernel-distance.inl:81-241), a single 32x32
 //                    block that scanned the whole dense histogram in FP32.
 //   dense_hist       scatter of records into dense histograms (tests, export).
@@ -336,7 +339,7 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
     const uint32_t kmin = sy.kmin, C = sy.n_species;
     const float four_dt = sy.four_dt;
     const BinSpec bs{four_dt, sy.de_bin_width, sy.n_k, sy.n_de};
-    // Philox blocks per 4-step chunk: 2 for the 12 displacements (+1 with
+    // Philox blocks per 4-step chunk: 2 for the 12 unit normals (+1 with
     // Gaussian detection for its 4 uniforms).
     constexpr uint32_t B = GAUSS ? 3u : 2u;
 
