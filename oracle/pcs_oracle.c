@@ -7,7 +7,7 @@
  *
  * Random stream (the same on the GPU path):
  *   key     = (seed & 0xffffffff, seed >> 32)
- *   counter = (block, particle, video, sim_id)
+ *   counter = (video, particle, sim_id, block)
  *   block 0             : initial state; words 0,1,2 -> x,y,z uniforms, word 3 -> species
  *   block 0xffffffff    : initial detection uniform (word 0), gaussian detection only
  *   chunk c (4 steps)   : blocks B*c+1 .. B*c+B, B = 2 (hard detection) or 3 (gaussian)
@@ -307,7 +307,7 @@ typedef struct {
 
 static void block(const stream_t* st, uint32_t blk, uint32_t particle, uint32_t video, uint32_t out[4])
 {
-    uint32_t ctr[4] = { blk, particle, video, st->sim_id };
+    uint32_t ctr[4] = { video, particle, st->sim_id, blk };
     philox_i(ctr, st->key, out);
 }
 

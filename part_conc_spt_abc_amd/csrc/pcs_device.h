@@ -31,36 +31,43 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
-__device__ __forceinline__ uint32_t mulhi(uint32_t m, uint32_t x) { return (uint32_t)(((uint64_t)m * x) >> 32); }
-
-// Counter words block (c0), video (c2) and simulation (c3) and the key are
-// wave-uniform; only the particle word c1 varies per lane. Rounds 1-3 are
-// written out so that the uniform halves stay scalar (plain ^, s_mul) and
-// only lane-varying words use the fused v_bitop3 XOR; from round 4 on every
-// word varies per lane. Same function as the textbook round loop.
-__device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+// Philox4x32-10 of the counter (video, particle, sim, blk). video, sim and the
+// key are fixed for a tile and wave-uniform, the particle word varies per lane
+// and is fixed for a tile, blk is wave-uniform and changes from chunk to chunk.
+// Rounds 1-3 are written out: rounds 1 and 2 mix the lane word with tile
+// constants only (the compiler hoists them out of the chunk loop, so a chunk's
+// block pays for them with scalar work), round 3 joins the lane halves with the
+// block halves by XOR, and from round 4 on every word varies per lane. Uniform
+// XOR operands are grouped so they fold on the scalar unit. Same function as
+// the textbook round loop.
+__device__ __forceinline__ U4 philox10(uint32_t blk, uint32_t particle, uint32_t video, uint32_t sim,
                                        uint32_t k0, uint32_t k1)
 {
-    // round 1: c1 varies
-    uint32_t x0 = xor3(mulhi(kPhiloxM1, c2), c1, k0);  // varying
-    uint32_t x1 = kPhiloxM1 * c2;                       // uniform
-    uint32_t x2 = mulhi(kPhiloxM0, c0) ^ c3 ^ k1;       // uniform
-    uint32_t x3 = kPhiloxM0 * c0;                       // uniform
+    // round 1: counter (video, particle, sim, blk)
+    const uint64_t a0 = (uint64_t)kPhiloxM0 * video;  // uniform
+    const uint64_t a1 = (uint64_t)kPhiloxM1 * sim;    // uniform
+    const uint32_t x0 = particle ^ ((uint32_t)(a1 >> 32) ^ k0);  // lane, tile constant
+    const uint32_t x1 = (uint32_t)a1;                            // uniform
+    const uint32_t x2 = (uint32_t)(a0 >> 32) ^ blk ^ k1;         // uniform, per chunk
+    const uint32_t x3 = (uint32_t)a0;                            // uniform
     k0 += kPhiloxW0;
     k1 += kPhiloxW1;
-    // round 2: x0 varies
-    uint32_t y0 = mulhi(kPhiloxM1, x2) ^ x1 ^ k0;       // uniform
-    uint32_t y1 = kPhiloxM1 * x2;                       // uniform
-    uint32_t y2 = xor3(mulhi(kPhiloxM0, x0), x3, k1);   // varying
-    uint32_t y3 = kPhiloxM0 * x0;                       // varying
+    // round 2
+    const uint64_t b0 = (uint64_t)kPhiloxM0 * x0;                // lane, tile constant
+    const uint64_t b1 = (uint64_t)kPhiloxM1 * x2;                // uniform, per chunk
+    const uint32_t y0 = (uint32_t)(b1 >> 32) ^ x1 ^ k0;          // uniform, per chunk
+    const uint32_t y1 = (uint32_t)b1;                            // uniform, per chunk
+    const uint32_t y2 = (uint32_t)(b0 >> 32) ^ (x3 ^ k1);        // lane, tile constant
+    const uint32_t y3 = (uint32_t)b0;                            // lane, tile constant
     k0 += kPhiloxW0;
     k1 += kPhiloxW1;
-    // round 3: y2, y3 vary (both halves of M1 y2 from one 32x32->64 multiply)
-    const uint64_t p1 = (uint64_t)kPhiloxM1 * y2;
-    c0 = xor3((uint32_t)(p1 >> 32), y1, k0);
-    c1 = (uint32_t)p1;
-    c2 = xor3(mulhi(kPhiloxM0, y0), y3, k1);
-    c3 = kPhiloxM0 * y0;  // uniform
+    // round 3
+    const uint64_t c0p = (uint64_t)kPhiloxM0 * y0;               // uniform, per chunk
+    const uint64_t c1p = (uint64_t)kPhiloxM1 * y2;               // lane, tile constant
+    uint32_t c0 = (uint32_t)(c1p >> 32) ^ (y1 ^ k0);
+    uint32_t c1 = (uint32_t)c1p;
+    uint32_t c2 = y3 ^ ((uint32_t)(c0p >> 32) ^ k1);
+    uint32_t c3 = (uint32_t)c0p;                                 // uniform, per chunk
     k0 += kPhiloxW0;
     k1 += kPhiloxW1;
 #pragma unroll

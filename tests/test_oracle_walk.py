@@ -47,7 +47,7 @@ def test_sigma_zero_tracks_are_exact():
     inside = 0
     h = [np.float32(100.0), np.float32(100.0), np.float32(15.0)]
     for i in range(N):
-        w = O.philox((0, i, 0, 3), (9, 0))
+        w = O.philox((0, i, 3, 0), (9, 0))  # counter (video 0, particle i, sim 3, block 0)
         u = (w >> 8).astype(np.float32) * np.float32(2.0 ** -24)
         x, y, z = (h[a] * (np.float32(2.0) * u[a] - np.float32(1.0)) for a in range(3))
         s = 0 if u[3] < p.cum_prob[0] else 1
