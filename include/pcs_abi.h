@@ -192,7 +192,8 @@ int pcs_autotune(pcs_ctx* ctx, const pcs_proposal* props, uint32_t n, const uint
 int pcs_set_record_capacity_factor(pcs_ctx* ctx, double factor);
 
 /* Evaluates the walk kernel's FP32 math on the GPU for n 32-bit words:
- * radius[i] = sqrt(-2 ln(((w>>8)+1) 2^-24)), (s[i], c[i]) = sincos(2 pi (w>>8) 2^-24).
+ * radius[i] = sqrt(-2 ln u) for the walk's radius uniform u = (float)(w | 255) 2^-32,
+ * (s[i], c[i]) = sin, cos of the walk's angle 2 pi (a + 1/2) 2^-16, a = w >> 16.
  * Test hook for bit-exactness of the device math; no reference counterpart. */
 int pcs_selftest_math(int device, const uint32_t* w, float* radius, float* s, float* c, uint64_t n);
 

@@ -11,7 +11,8 @@
  *   block 0             : initial state; words 0,1,2 -> x,y,z uniforms, word 3 -> species
  *   block 0xffffffff    : initial detection uniform (word 0), gaussian detection only
  *   chunk c (4 steps)   : blocks B*c+1 .. B*c+B, B = 2 (hard detection) or 3 (gaussian)
- *     words 0..5 -> radius uniforms of six Box-Muller pairs (top 24 bits of word p),
+ *     words 0..5 -> radius uniforms of six Box-Muller pairs (top 24 bits of word p:
+ *     u = (float)(w | 255) 2^-32, see neg2log_i),
  *     angle of pair p: 16-bit code a = (low byte of word p) << 8 | byte (p % 4) of
  *     word 6 + p/4, theta = 2 pi (a + 1/2) 2^-16;
  *     pair p gives unit normals n[2p] = r*sin, n[2p+1] = r*cos, step s of the
@@ -65,9 +66,8 @@ static const float LOG_P[8] = {
     -0x1.fffff4p-2f, 0x1.5556e8p-2f, -0x1.0006c6p-2f, 0x1.98da1cp-3f,
     -0x1.52fb0cp-3f, 0x1.30d166p-3f, -0x1.277654p-3f, 0x1.6fccd4p-4f,
 };
-/* ln 2 split so that E * LN2_HI is exact for |E| < 256 (Cody & Waite). */
-static const float LN2_HI = 0x1.62e4p-1f;
-static const float LN2_LO = 0x1.7f7d1cp-20f;
+/* -2 ln 2 rounded to FP32 (2.8e-9 relative error). */
+static const float NEG2_LN2 = -0x1.62e43p+0f;
 static const float SQRT_HALF = 0x1.6a09e6p-1f;
 
 /* -2 ln(1+f) = g + g^2 Q(g) in g = -2f with Q(g) = -P(-g/2)/2: Q's coefficients
@@ -214,24 +214,24 @@ static const float BETA_TAB[512] = {
 static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
-/* -2 ln u for u = ((w >> 8) + 1) 2^-24, positive for every w: the series in
+/* -2 ln u for the uniform u = x 2^-32 of the top 24 bits of w, where
+ * x = (float)(w | 255) rounded to nearest even (256 J + 255 for J = w >> 8
+ * below 2^16, 256 (J + 1) above), positive for every w: the series in
  * g = -2 (m - 1) (exact), each Horner step the one in f = m - 1 scaled by a
  * power of two; g^2 carries an added 2^-100, so u = 1 gives about 2^-102
  * rather than 0 (and a g^2 exactly halfway between two floats rounds up). */
 static inline float neg2log_i(uint32_t w)
 {
-    uint32_t j = (w >> 8) + 1u;              /* 1 .. 2^24 */
-    float xf = (float)j;                      /* exact */
+    float xf = (float)(w | 0xffu);            /* 255 .. 2^32, rounded to nearest even */
     int e;
     float m = frexpf(xf, &e);                 /* xf = m * 2^e, m in [1/2, 1) */
     if (m < SQRT_HALF) { m = m * 2.0f; e = e - 1; }
     float g = fmaf(m, -2.0f, 2.0f);           /* -2 (m - 1), exact */
-    float E = (float)(e - 24);                /* u = xf * 2^-24 */
+    float E = (float)(e - 32);                /* u = xf * 2^-32 */
     float g2 = fmaf(g, g, 0x1p-100f);
     float Q = LOG_P[7] * LOG_QS[7];
     for (int i = 6; i >= 0; --i) Q = fmaf(Q, g, LOG_P[i] * LOG_QS[i]);
-    float l = fmaf(g2, Q, g);
-    return fmaf(E, -2.0f * LN2_HI, fmaf(E, -2.0f * LN2_LO, l));
+    return fmaf(E, NEG2_LN2, fmaf(g2, Q, g));
 }
 
 float orc_neg2log_u24(uint32_t w) { return neg2log_i(w); }
@@ -580,7 +580,7 @@ static void normals_f64(const stream_t* st, uint32_t c, uint32_t B, uint32_t par
     block(st, B * c + 1u, part, video, w);
     block(st, B * c + 2u, part, video, w + 4);
     for (int pr = 0; pr < 6; ++pr) {
-        double u1 = (double)((w[pr] >> 8) + 1u) * 0x1p-24;
+        double u1 = (double)(float)(w[pr] | 0xffu) * 0x1p-32; /* the FP32 walk's u */
         double u2 = ((double)bm_angle16(w[pr], w[6 + pr / 4], (uint32_t)pr) + 0.5) * 0x1p-16;
         double r = sqrt(-2.0 * log(u1));
         d[2 * pr] = sigma * r * sin(TWO_PI * u2);

@@ -50,7 +50,8 @@ extern "C" {
 /* Philox4x32-10 block function (Salmon et al., SC'11). */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
-/* y = -2 ln(u), u = ((w >> 8) + 1) * 2^-24, in the project's FP32 op order. */
+/* y = -2 ln(u), u = (float)(w | 255) * 2^-32 (the top 24 bits of w), in the
+ * project's FP32 op order. */
 float orc_neg2log_u24(uint32_t w);
 /* Box-Muller radius sqrt(-2 ln u) in FP32 (correctly rounded sqrt of the above). */
 float orc_bm_radius(uint32_t w);

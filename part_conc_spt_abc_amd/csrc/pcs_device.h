@@ -91,12 +91,12 @@ __device__ __forceinline__ U4 philox10(uint32_t blk, uint32_t particle, uint32_t
 // --------------------------------------------------------------- FP32 math
 // ln(1+f) = f + f^2 P(f) on [sqrt(1/2)-1, sqrt(2)-1]; coefficients from
 // fit_math_coeffs.py (reweighted least squares towards minimax), rounded to
-// FP32. ln 2 is split (Cody & Waite) so E * kLn2Hi is exact for |E| < 256.
+// FP32. -2 ln 2 rounded to FP32 is within 2.8e-9 relative of the exact value,
+// so E (-2 ln 2) for |E| <= 24 is within 0.03 ulp of any result it enters.
 constexpr float kLogP0 = -0x1.fffff4p-2f, kLogP1 = 0x1.5556e8p-2f, kLogP2 = -0x1.0006c6p-2f,
                 kLogP3 = 0x1.98da1cp-3f, kLogP4 = -0x1.52fb0cp-3f, kLogP5 = 0x1.30d166p-3f,
                 kLogP6 = -0x1.277654p-3f, kLogP7 = 0x1.6fccd4p-4f;
-constexpr float kLn2Hi = 0x1.62e4p-1f;
-constexpr float kLn2Lo = 0x1.7f7d1cp-20f;
+constexpr float kNeg2Ln2 = -0x1.62e43p+0f;
 constexpr float kSqrtHalf = 0x1.6a09e6p-1f;
 constexpr uint32_t kSqrtHalfBits = 0x3f3504f3u;  // bits of kSqrtHalf
 static_assert(__builtin_bit_cast(uint32_t, kSqrtHalf) == kSqrtHalfBits, "bits of sqrt(1/2)");
@@ -105,25 +105,26 @@ static_assert(__builtin_bit_cast(uint32_t, kSqrtHalf) == kSqrtHalfBits, "bits of
 constexpr float kLogQ0 = kLogP0 * -0.5f, kLogQ1 = kLogP1 * 0.25f, kLogQ2 = kLogP2 * -0.125f,
                 kLogQ3 = kLogP3 * 0.0625f, kLogQ4 = kLogP4 * -0.03125f, kLogQ5 = kLogP5 * 0.015625f,
                 kLogQ6 = kLogP6 * -0.0078125f, kLogQ7 = kLogP7 * 0.00390625f;
-constexpr float kNeg2Ln2Hi = -2.0f * kLn2Hi, kNeg2Ln2Lo = -2.0f * kLn2Lo;
 constexpr float kLogTiny = 0x1p-100f;
 
-// -2 ln(u) for u = ((w >> 8) + 1) 2^-24 in (0, 1]; positive for every w.
-// With m, E as below, g = -2 (m - 1) = fma(m, -2, 2) is exact and every Horner
-// step in g is the corresponding step in f = m - 1 scaled by a power of two, so
-// the result is -2 (E ln 2 + f + f^2 P(f)) with the same roundings. g^2 carries
-// an added 2^-100, which changes the result only where u = 1 (then 2^-100 Q(0),
-// about 2^-102, instead of 0, so the square root below never sees 0) and where
-// g^2 is exactly halfway between two floats (rounded up instead of to even).
+// -2 ln(u) for the uniform u = x 2^-32 of the top 24 bits of w, with
+// x = (float)(w | 255) (rounded to nearest even): x = 256 J + 255 exactly for
+// J = w >> 8 < 2^16 and x = 256 (J + 1) above, so u is in [255 2^-32, 1] and
+// positive for every w; the low byte of w does not enter. With m, E as below,
+// g = -2 (m - 1) = fma(m, -2, 2) is exact and every Horner step in g is the
+// corresponding step in f = m - 1 scaled by a power of two, so the result is
+// -2 (E ln 2 + f + f^2 P(f)) with the same roundings. g^2 carries an added
+// 2^-100, which changes the result only where u = 1 (then 2^-100 Q(0), about
+// 2^-102, instead of 0, so the square root below never sees 0) and where g^2
+// is exactly halfway between two floats (rounded up instead of to even).
 __device__ __forceinline__ float neg2log_u24(uint32_t w)
 {
-    const uint32_t j = (w >> 8) + 1u;
-    // x = (float)j = m 2^e with m in [sqrt(1/2), sqrt(2)), in integer steps on
-    // the bits of x: t = bits(x) - bits(sqrt(1/2)) - (24 << 23) carries e - 24
-    // in its top 9 bits (arithmetic shift) and m's bits are the low 23 bits of t
-    // plus bits(sqrt(1/2)) (a borrow from the mantissa lands m in [1, sqrt 2)).
+    // x = m 2^e with m in [sqrt(1/2), sqrt(2)), in integer steps on the bits
+    // of x: t = bits(x) - bits(sqrt(1/2)) - (32 << 23) carries e - 32 in its
+    // top 9 bits (arithmetic shift) and m's bits are the low 23 bits of t plus
+    // bits(sqrt(1/2)) (a borrow from the mantissa lands m in [1, sqrt 2)).
     // Same m and e as frexp followed by the sqrt(1/2) fold-up.
-    const int32_t t = (int32_t)(__float_as_uint((float)j) - (kSqrtHalfBits + (24u << 23)));
+    const int32_t t = (int32_t)(__float_as_uint((float)(w | 0xffu)) - (kSqrtHalfBits + (32u << 23)));
     const float m = __uint_as_float(((uint32_t)t & 0x7fffffu) + kSqrtHalfBits);
     const float g = __builtin_fmaf(m, -2.0f, 2.0f);  // exact (Sterbenz)
     const float E = (float)(t >> 23);
@@ -136,8 +137,7 @@ __device__ __forceinline__ float neg2log_u24(uint32_t w)
     Q = __builtin_fmaf(Q, g, kLogQ2);
     Q = __builtin_fmaf(Q, g, kLogQ1);
     Q = __builtin_fmaf(Q, g, kLogQ0);
-    const float l = __builtin_fmaf(g2, Q, g);
-    return __builtin_fmaf(E, kNeg2Ln2Hi, __builtin_fmaf(E, kNeg2Ln2Lo, l));
+    return __builtin_fmaf(E, kNeg2Ln2, __builtin_fmaf(g2, Q, g));
 }
 
 // 2^f = 1 + f Q(f) on [0, 1); Q from fit_math_coeffs.py (within 1.07 ulp of 2^f,
@@ -169,11 +169,13 @@ __device__ __forceinline__ bool gauss_detect(float z, float c, uint32_t w)
     return (float)((w >> 8) + 1u) <= t;
 }
 
-// Correctly rounded sqrt on the Box-Muller domain y = -2 ln u (about 2^-102,
-// or 1.19e-7 <= y <= 33.3; 2^24 values): s = y rsq(y) with one correction by
-// the residual y - s^2 (Markstein). It equals the correctly rounded square
-// root on every input of the domain (checked exhaustively against the CPU's
-// sqrtf, bit for bit); the hardware square root alone misses on 15 % of them.
+// Correctly rounded sqrt for the Box-Muller radius y = -2 ln u (about 2^-102,
+// or 1.19e-7 <= y <= 33.3): s = y rsq(y) with one correction by the residual
+// y - s^2 (Markstein). It equals the correctly rounded square root on every
+// float in [2^-102, 64) (checked exhaustively on the GPU, each result against
+// the midpoints to its neighbours squared exactly in FP64; below 2^-102 the
+// residual term is subnormal and it fails); the hardware square root alone
+// misses on 15 % of the radius inputs.
 __device__ __forceinline__ float sqrt_rn_domain(float y)
 {
     const float q = __builtin_amdgcn_rsqf(y);

@@ -327,8 +327,12 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
 {
     if (p0 >= N) return;  // empty tile (a request without particles)
     constexpr bool ZBOX = !GAUSS && !ZREF;  // z in box units
-    const uint32_t sim = pr.sim_id;
-    const uint32_t k0 = s0 ^ pr.key0, k1 = s1 ^ pr.key1;
+    // The counter words other than the particle and the key are the same on
+    // every lane: held in SGPRs, the Philox rounds that see only them (round 1,
+    // and half of rounds 2 and 3) run on the scalar unit.
+    video = uniform(video);
+    const uint32_t sim = uniform(pr.sim_id);
+    const uint32_t k0 = uniform(s0 ^ pr.key0), k1 = uniform(s1 ^ pr.key1);
 
     const float hz = sy.half_l[2];
     const float Lx = 2.0f * sy.half_l[0], Ly = 2.0f * sy.half_l[1], Lz = 2.0f * hz;

@@ -36,6 +36,15 @@ def test_device_and_oracle_constants_match_fit():
         tab_dev = floats(re.search(name_dev + r"\[kSinCosTabFloats\] = \{.*?\};", ker, re.S).group(0))
         tab_orc = floats(re.search(name_orc + r"\[512\] = \{.*?\};", orc, re.S).group(0))
         assert tab_dev == v[key] and tab_orc == v[key] and len(tab_dev) == 512
+    # -2 ln 2 in both copies: the FP32 value nearest the exact one
+    import math
+
+    import numpy as np
+
+    c = floats(re.search(r"constexpr float kNeg2Ln2 = .*?;", dev, re.S).group(0))
+    assert c == floats(re.search(r"NEG2_LN2 = .*?;", orc, re.S).group(0)) and len(c) == 1
+    x = -2.0 * math.log(2.0)
+    assert np.float32(c[0]) == c[0] and abs(c[0] - x) <= 0.5 * float(np.spacing(np.float32(abs(c[0]))))
 
 
 def _max_ulp(got, want):

@@ -54,8 +54,13 @@ def ulps(got, want):
 
 def test_radius_all_inputs(all_words):
     r = O.bm_radius(all_words)
-    j = np.arange(1 << 24, dtype=np.float64)
-    want = np.sqrt(-2.0 * np.log((j + 1.0) * 2.0 ** -24))
+    # u = (float)(w | 255) 2^-32 (numpy's uint32 -> float32 cast rounds to
+    # nearest even, as the walk's conversion does)
+    x = (all_words | np.uint32(0xFF)).astype(np.float32).astype(np.float64)
+    assert x.min() == 255.0 and x.max() == 2.0 ** 32
+    J = np.arange(1 << 24, dtype=np.float64)
+    assert np.array_equal(x, np.where(J < 2 ** 16, 256.0 * J + 255.0, 256.0 * (J + 1.0)))
+    want = np.sqrt(-2.0 * np.log(x * 2.0 ** -32))
     # u = 1: -2 ln u is 2^-100 Q(0) with Q(0) = -P0/2 (the tiny term that keeps
     # the square root's input positive), P0 = -0x1.fffff4p-2; its root, 4.4e-16
     y1 = np.float64(2.0 ** -100) * (-float.fromhex("-0x1.fffff4p-2") / 2.0)
