@@ -197,6 +197,13 @@ int pcs_set_record_capacity_factor(pcs_ctx* ctx, double factor);
  * Test hook for bit-exactness of the device math; no reference counterpart. */
 int pcs_selftest_math(int device, const uint32_t* w, float* radius, float* s, float* c, uint64_t n);
 
+/* Counts, in *n_wrong, the floats whose bit patterns lie in [lo_bits, hi_bits)
+ * (positive normal floats) on which the walk kernel's square root (the radius
+ * of its Box-Muller pairs) differs from the correctly rounded one; each result
+ * is checked on the GPU against the midpoints to its neighbours, squared
+ * exactly in FP64. Test hook; no reference counterpart. */
+int pcs_selftest_sqrt(int device, uint32_t lo_bits, uint32_t hi_bits, uint64_t* n_wrong);
+
 /* Copies `bytes` (a multiple of 16) between two fresh device buffers with a
  * plain streaming kernel and reports its time in *ms (nullable). Calibration
  * hook for hardware byte counters (known traffic: bytes read + bytes

@@ -138,6 +138,7 @@ PROTOTYPES = {
     "pcs_autotune": (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, C.POINTER(pcs_tuning), C.POINTER(C.c_float)]),
     "pcs_selftest_math": (C.c_int, [C.c_int, P, P, P, P, C.c_uint64]),
     "pcs_selftest_copy": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_float)]),
+    "pcs_selftest_sqrt": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     "pcs_generate_tracks": (C.c_int, [P, C.POINTER(pcs_proposal), P, P, C.c_uint32, C.c_uint64, P, P, C.c_uint64,
                                        C.POINTER(C.c_uint64)]),
     "pcs_pmc_weights": (C.c_int, [C.c_int, P, C.c_uint32, P, P, C.c_uint32, P, C.c_uint32, P]),

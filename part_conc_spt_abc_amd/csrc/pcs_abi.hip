@@ -845,6 +845,30 @@ int pcs_selftest_math(int device, const uint32_t* w, float* radius, float* s, fl
     return PCS_OK;
 }
 
+int pcs_selftest_sqrt(int device, uint32_t lo_bits, uint32_t hi_bits, uint64_t* n_wrong)
+{
+    if (!n_wrong) return set_error(PCS_EINVAL, "NULL argument");
+    if (lo_bits < 0x00800000u || hi_bits > 0x7f800000u || lo_bits > hi_bits)
+        return set_error(PCS_EINVAL, "range [%#x, %#x) is not within the positive normal floats", lo_bits, hi_bits);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PCS_ENODEV, "no GPU visible");
+    if (device < 0 || device >= ndev) return set_error(PCS_EINVAL, "device %d outside [0, %d)", device, ndev);
+    PCS_HIP(hipSetDevice(device));
+    *n_wrong = 0;
+    if (lo_bits == hi_bits) return PCS_OK;
+    unsigned long long* dbad = nullptr;
+    PCS_TRY(dev_alloc(&dbad, 1));
+    unsigned long long h = 0;
+    hipError_t e = hipMemset(dbad, 0, sizeof(h));
+    if (e == hipSuccess) e = launch_selftest_sqrt(lo_bits, hi_bits - lo_bits, dbad, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(&h, dbad, sizeof(h), hipMemcpyDeviceToHost);
+    hipFree(dbad);
+    if (e != hipSuccess) return hip_error(e, "pcs_selftest_sqrt");
+    *n_wrong = h;
+    return PCS_OK;
+}
+
 int pcs_selftest_copy(int device, uint64_t bytes, float* ms)
 {
     if (bytes == 0 || bytes % 16u) return set_error(PCS_EINVAL, "bytes must be a positive multiple of 16");

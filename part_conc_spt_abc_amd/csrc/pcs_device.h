@@ -169,13 +169,14 @@ __device__ __forceinline__ bool gauss_detect(float z, float c, uint32_t w)
     return (float)((w >> 8) + 1u) <= t;
 }
 
-// Correctly rounded sqrt for the Box-Muller radius y = -2 ln u (about 2^-102,
-// or 1.19e-7 <= y <= 33.3): s = y rsq(y) with one correction by the residual
-// y - s^2 (Markstein). It equals the correctly rounded square root on every
-// float in [2^-102, 64) (checked exhaustively on the GPU, each result against
-// the midpoints to its neighbours squared exactly in FP64; below 2^-102 the
-// residual term is subnormal and it fails); the hardware square root alone
-// misses on 15 % of the radius inputs.
+// Correctly rounded sqrt for the Box-Muller radius y = -2 ln u (2^-100 Q(0) =
+// 0x1.fffff4p-103 at u = 1, else 1.19e-7 <= y <= 33.3): s = y rsq(y) with one
+// correction by the residual y - s^2 (Markstein). It equals the correctly
+// rounded square root on every float in [2^-102, 64) and at 0x1.fffff4p-103
+// (checked exhaustively on the GPU, each result against the midpoints to its
+// neighbours squared exactly in FP64; it misses on one of the five floats in
+// between, and on many far below, where the residual term is subnormal); the
+// hardware square root alone misses on 15 % of the radius inputs.
 __device__ __forceinline__ float sqrt_rn_domain(float y)
 {
     const float q = __builtin_amdgcn_rsqf(y);

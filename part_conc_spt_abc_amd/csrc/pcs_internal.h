@@ -161,6 +161,7 @@ hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, c
                            uint32_t P, double* wsum, hipStream_t s);
 hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, float* c, uint64_t n, hipStream_t st);
 hipError_t launch_selftest_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st);
+hipError_t launch_selftest_sqrt(uint32_t lo, uint32_t n, unsigned long long* bad, hipStream_t st);
 // Request loop: `grid` groups of kLoopWaves waves; n_de_max selects the
 // distance segment width. Returns once launched.
 hipError_t launch_serve_loop(const LoopArgs& a, uint32_t grid, uint32_t n_de_max, hipStream_t s);

@@ -1185,6 +1185,30 @@ __global__ __launch_bounds__(256) void selftest_math_kernel(const uint32_t* w, f
     c[i] = cs;
 }
 
+// Counts the floats x (every bit pattern in [lo, lo + n)) on which the walk's
+// radius square root is not correctly rounded. s is the correctly rounded
+// root iff mlo^2 < x < mhi^2 for the midpoints mlo, mhi between s and its two
+// neighbours: they have at most 26 significant bits, so their squares and the
+// comparisons are exact in FP64.
+__global__ __launch_bounds__(256) void selftest_sqrt_kernel(uint32_t lo, uint32_t n, unsigned long long* bad)
+{
+    uint32_t mine = 0u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float x = __uint_as_float(lo + i);
+        const uint32_t sb = __float_as_uint(sqrt_rn_domain(x));
+        bool ok = sb > 0u && sb < 0x7f800000u;
+        if (ok) {
+            const double sd = (double)__uint_as_float(sb);
+            const double mlo = 0.5 * (sd + (double)__uint_as_float(sb - 1u));
+            const double mhi = 0.5 * (sd + (double)__uint_as_float(sb + 1u));
+            const double xd = (double)x;
+            ok = mlo * mlo < xd && xd < mhi * mhi;
+        }
+        mine += ok ? 0u : 1u;
+    }
+    if (mine != 0u) atomicAdd(bad, (unsigned long long)mine);
+}
+
 __global__ void selftest_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -1256,6 +1280,14 @@ hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, floa
     if (n == 0) return hipSuccess;
     const uint64_t nb = (n + 255u) / 256u;
     hipLaunchKernelGGL(selftest_math_kernel, dim3((uint32_t)nb), dim3(256), 0, st, w, radius, s, c, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_selftest_sqrt(uint32_t lo, uint32_t n, unsigned long long* bad, hipStream_t st)
+{
+    (void)hipGetLastError();  // report only this launch's error
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(selftest_sqrt_kernel, dim3(8192), dim3(256), 0, st, lo, n, bad);
     return hipGetLastError();
 }
 

@@ -172,6 +172,15 @@ def selftest_math(words: np.ndarray, device: int = 0):
     return r, s, c
 
 
+def selftest_sqrt(lo_bits: int, hi_bits: int, device: int = 0) -> int:
+    """Number of floats with bit patterns in [lo_bits, hi_bits) on which the
+    walk's square root is not correctly rounded (checked on the GPU)."""
+    lib = _abi.load()
+    n = C.c_uint64(0)
+    _abi.check(lib.pcs_selftest_sqrt(device, lo_bits, hi_bits, C.byref(n)), "pcs_selftest_sqrt")
+    return int(n.value)
+
+
 class Server:
     """Continuous-batching serving loop (pcs_server_*). device_loop=True
     selects the device-resident form (one persistent kernel fed from a ring in
