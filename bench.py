@@ -58,9 +58,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def headline_model() -> Model:
+def headline_model(z_boundary: str = "periodic") -> Model:
+    zb = _abi.PCS_Z_REFLECT if z_boundary == "reflect" else _abi.PCS_Z_PERIODIC
     return Model(Lx=200.0, Ly=200.0, Lz=30.0, ax=160.0, ay=120.0, deltat=0.05, kmin=3, n_k=1000, n_de=500,
-                 ub_de=20.0, n_species=2)
+                 ub_de=20.0, n_species=2, z_boundary=zb)
 
 
 def workload(model, n, rng, frames_choices):
@@ -260,6 +261,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--large-sims", type=int, default=64)
+    ap.add_argument("--z-boundary", choices=("periodic", "reflect"), default="periodic",
+                    help="z walls of the 200x200x30 um box (x and y are periodic); the batched leg also runs "
+                         "the other z boundary on the same proposals unless --no-other-z")
+    ap.add_argument("--no-other-z", action="store_true")
     ap.add_argument("--tuning", default="",
                     help="launch shape 'particles_per_lane,waves_per_group,groups_per_cu' instead of the autotuner "
                          "(profiling runs: no autotuner launches among the profiled ones)")
@@ -272,7 +277,7 @@ def main():
               f"ranges across GPUs by design but nothing here runs on more than one.", file=sys.stderr)
         sys.exit(2)
 
-    model = headline_model()
+    model = headline_model(args.z_boundary)
     ctx = Context(model)  # scratch context to make the data histogram
     obs = observed_hist(ctx, model)
     ctx.close()
@@ -322,6 +327,30 @@ def main():
         cms = C.c_float(0.0)
         _abi.check(_abi.load().pcs_selftest_copy(0, 256 << 20, C.byref(cms)), "pcs_selftest_copy")
         log(f"calibration: copied 268435456 bytes device to device in {cms.value:.3f} ms")
+
+    other_z = None
+    if not args.no_other_z:
+        # The same proposals with the other z boundary: the reference wraps z
+        # periodically (src/cpu/simulate.jl:93), the model can also reflect it.
+        oz = "reflect" if args.z_boundary == "periodic" else "periodic"
+        m2 = headline_model(oz)
+        c2 = Context(m2)
+        obs2 = observed_hist(c2, m2)
+        c2.close()
+        c2 = Context(m2, obs2)
+        if tuning:
+            c2.set_tuning(tuning["particles_per_lane"], tuning["waves_per_group"], tuning["groups_per_cu"],
+                          tuning["stage_records"])
+        b2 = c2.batch()
+        b2.load(props, frames, parts)
+        w2, _, _ = run_timed(b2, args.steps, args.warmup, _abi.PCS_MODE_BATCHED, 100)
+        _, _, wk2 = run_timed(b2, args.steps, 0, _abi.PCS_MODE_BATCHED_DIRECT, 100)
+        b2.close()
+        c2.close()
+        other_z = {"z_boundary": oz, "sims_per_s": args.batched_sims * args.steps / sum(w2),
+                   "particle_steps_per_s": steps_per_gen * args.steps / sum(w2),
+                   "walk_kernel_ms": float(np.mean(wk2))}
+        log(f"batched, z {oz}: {other_z}")
 
     plain = None
     if not args.no_plain:
@@ -384,8 +413,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"batched ABC generation: {args.batched_sims} proposals, c log-uniform 1e8-1e10/mL "
-                        f"(120-12,000 particles), frames U{{500,1000,2000}}, box 200x200x30 um periodic, "
+                        f"(120-12,000 particles), frames U{{500,1000,2000}}, box 200x200x30 um "
+                        f"{'periodic' if args.z_boundary == 'periodic' else 'periodic in x/y, reflecting in z'}, "
                         f"2 species 175/500 nm, H 1000x500 uint32, one hipGraph launch per generation",
+            "z_boundary": args.z_boundary,
+            "other_z_boundary": other_z,
             "particle_steps_per_generation": steps_per_gen,
             "particle_steps_per_s": psteps_s,
             "plain": plain,

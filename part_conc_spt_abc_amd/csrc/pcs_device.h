@@ -211,10 +211,13 @@ __device__ __forceinline__ float wrap_periodic(float v, float h, float L)
 {
     return (__builtin_fabsf(v) > h) ? v - __builtin_copysignf(L, v) : v;
 }
-// Reflecting wall: v' = +-L - v beyond +-L/2.
-__device__ __forceinline__ float wrap_reflect(float v, float h, float L)
+// Reflecting walls at +-h: v' = copysign(2h, v) - v beyond them (single
+// reflection), as 2 med3(v, -h, h) - v in one fma: v itself inside (2v - v is
+// exact) and +-2h - v, rounded once, outside - the same value as the
+// reflection from L = 2h.
+__device__ __forceinline__ float wrap_reflect(float v, float h)
 {
-    return (__builtin_fabsf(v) > h) ? __builtin_copysignf(L, v) - v : v;
+    return __builtin_fmaf(2.0f, __builtin_amdgcn_fmed3f(v, -h, h), -v);
 }
 
 }  // namespace pcs

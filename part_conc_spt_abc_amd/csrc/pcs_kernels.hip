@@ -438,7 +438,7 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                     q.z = fract_box(__builtin_fmaf(q.sz, nz, q.z));
                 } else {
                     const float zn = __builtin_fmaf(q.sz, nz, q.z);
-                    q.z = ZREF ? wrap_reflect(zn, hz, Lz) : wrap_periodic(zn, hz, Lz);
+                    q.z = ZREF ? wrap_reflect(zn, hz) : wrap_periodic(zn, hz, Lz);
                 }
                 const bool cx = q.x <= wx, cy = q.y <= wy;
                 const bool cz = GAUSS ? gauss_detect(q.z, cg, wdet[i][s])
