@@ -249,17 +249,18 @@ static inline float exp2_01(float f)
     return fmaf(f, Q, 1.0f);
 }
 
-/* Gaussian detection in z (model extension): detected iff j <= 2^(24 - c z^2)
- * for j = (w >> 8) + 1 in [1, 2^24] and c = log2(e) / (2 s^2), i.e. with
- * probability exp(-z^2 / (2 s^2)); the exponent splits into floor n and
- * fraction f (exact), 2^f by the polynomial, scaled by 2^n. */
+/* Gaussian detection in z (model extension): detected iff x <= 2^(32 - c z^2)
+ * for x = (float)(w | 255) (the uniform u = x 2^-32 of w's top 24 bits) and
+ * c = log2(e) / (2 s^2), i.e. with probability exp(-z^2 / (2 s^2)); the
+ * exponent, clamped below at -128 (2^-128 is under every x), splits into
+ * floor n and fraction f (exact), 2^f by the polynomial, scaled by 2^n. */
 static inline float gauss_c_i(float sig2) { return LOG2E_F / (sig2 + sig2); }
 static inline int gauss_detect_i(float z, float c, uint32_t w)
 {
-    const float v = fmaf(-c, z * z, 24.0f);
+    const float v = fmaxf(fmaf(-c, z * z, 32.0f), -128.0f);
     const float n = floorf(v);
     const float t = ldexpf(exp2_01(v - n), (int)n);
-    return (float)((w >> 8) + 1u) <= t;
+    return (float)(w | 0xffu) <= t;
 }
 
 float orc_exp2_01(float f) { return exp2_01(f); }
@@ -349,11 +350,6 @@ static inline float fract_box(float v) { return fminf(v - floorf(v), 0x1.fffffep
 
 /* Single periodic wrap (src/gpu/cusim/kernel-simulate-unroll.inl:24-32;
  * src/cpu/position_periodic.jl:1-9 in the centred frame). */
-static inline float wrap_periodic(float v, float h, float L)
-{
-    if (fabsf(v) > h) v = v - copysignf(L, v);
-    return v;
-}
 /* Reflecting wall (not in the reference; model extension). */
 static inline float wrap_reflect(float v, float h, float L)
 {
@@ -389,8 +385,11 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
     const float ox = hax / Lx - 0.5f, oy = hay / Ly - 0.5f;
     const float wx = (hax + hax) / Lx, wy = (hay + hay) / Ly;
     const int gauss = sys->detect_model == 1;
-    const int zbox = !gauss && !sys->z_boundary;
-    const float cg = gauss_c_i(sys->detect_sigma * sys->detect_sigma);
+    /* z in box units when periodic: from the window's lower edge for the slab,
+     * from the box floor for Gaussian detection (centred distance z - 1/2). */
+    const int zbox = !sys->z_boundary;
+    const float sdz = zbox ? sys->detect_sigma / Lz : sys->detect_sigma;
+    const float cg = gauss_c_i(sdz * sdz);
     const uint32_t kmin = sys->kmin;
 
     for (uint32_t part = 0; part < N; ++part) {
@@ -403,7 +402,11 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
         const float haz = p->half_az[s];
         const float sx = sigma / Lx, sy = sigma / Ly, sig2 = sigma * sigma;
         float z, sz, wz;
-        if (zbox) {
+        if (zbox && gauss) {
+            sz = sigma / Lz;
+            wz = 0.0f;
+            z = u24(w0[2]);
+        } else if (zbox) {
             sz = sigma / Lz;
             wz = (haz + haz) / Lz;
             z = fract_box(u24(w0[2]) + (haz / Lz - 0.5f));
@@ -417,7 +420,7 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
         if (gauss) {
             uint32_t wd[4];
             block(st, 0xffffffffu, part, video, wd);
-            det = x <= wx && y <= wy && gauss_detect_i(z, cg, wd[0]);
+            det = x <= wx && y <= wy && gauss_detect_i(zbox ? z - 0.5f : z, cg, wd[0]);
         } else {
             det = x <= wx && y <= wy && (zbox ? z <= wz : fabsf(z) <= wz);
         }
@@ -436,12 +439,11 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
             if (zbox) {
                 z = fract_box(fmaf(sz, nz, z));
             } else {
-                const float zn = fmaf(sz, nz, z);
-                z = sys->z_boundary ? wrap_reflect(zn, hz, Lz) : wrap_periodic(zn, hz, Lz);
+                z = wrap_reflect(fmaf(sz, nz, z), hz, Lz);
             }
 
             if (gauss)
-                det = x <= wx && y <= wy && gauss_detect_i(z, cg, wdet[sl]);
+                det = x <= wx && y <= wy && gauss_detect_i(zbox ? z - 0.5f : z, cg, wdet[sl]);
             else
                 det = x <= wx && y <= wy && (zbox ? z <= wz : fabsf(z) <= wz);
 
@@ -690,14 +692,16 @@ void orc_trace_particle(const orc_system* sys, const orc_proposal* p, const orc_
     const float hz = sys->half_l[2];
     const float L[3] = { 2.0f * sys->half_l[0], 2.0f * sys->half_l[1], 2.0f * hz };
     const int gauss = sys->detect_model == 1;
-    const int zbox = !gauss && !sys->z_boundary;
+    const int zbox = !sys->z_boundary;
     uint32_t wdet[4];
     uint32_t w0[4];
     block(&st, 0u, part, video, w0);
     unsigned s = species_of(u24(w0[3]), p->cum_prob, sys->n_species);
     const float sigma = p->sigma[s];
     const float haz = p->half_az[s];
-    const float ha[3] = { sys->half_a[0], sys->half_a[1], haz };
+    /* origin of a box-unit axis: the window's lower edge; the box floor for z
+     * under Gaussian detection (then hz / Lz - 1/2 = 0 and q starts at u) */
+    const float ha[3] = { sys->half_a[0], sys->half_a[1], gauss ? hz : haz };
     float q[3], sc[3];
     for (int a = 0; a < 3; ++a) {
         if (a < 2 || zbox) {
@@ -716,7 +720,7 @@ void orc_trace_particle(const orc_system* sys, const orc_proposal* p, const orc_
             for (int a = 0; a < 3; ++a) {
                 const float v = fmaf(sc[a], d[3 * sl + a], q[a]);
                 if (a < 2 || zbox) q[a] = fract_box(v);
-                else q[a] = sys->z_boundary ? wrap_reflect(v, hz, L[2]) : wrap_periodic(v, hz, L[2]);
+                else q[a] = wrap_reflect(v, hz, L[2]);
             }
         }
         for (int a = 0; a < 3; ++a) {

@@ -69,7 +69,8 @@ void orc_chunk_normals_n(uint64_t seed, uint32_t sim_id, uint32_t video, uint32_
                          int gauss, float* out);
 
 /* 2^f for f in [0, 1) and the Gaussian detection draw of the walk: c =
- * log2(e) / (2 s^2) for s^2 = sig2; detected iff ((w >> 8) + 1) <= 2^(24 - c z^2). */
+ * log2(e) / (2 s^2) for s^2 = sig2; detected iff (float)(w | 255) <= 2^(32 - c z^2)
+ * (the exponent clamped below at -128). */
 float orc_exp2_01(float f);
 float orc_gauss_c(float sig2);
 int orc_gauss_detect(float z, float c, uint32_t w);

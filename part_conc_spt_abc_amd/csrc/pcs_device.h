@@ -147,16 +147,19 @@ constexpr float kExp2Q0 = 0x1.62e42cp-1f, kExp2Q1 = 0x1.ebfd58p-3f, kExp2Q2 = 0x
 constexpr float kLog2E = 0x1.715476p0f;
 
 // Gaussian detection in z (the model's extension of the slab): a particle at
-// z is detected with probability exp(-z^2 / (2 s^2)), drawn with the uniform
-// integer j = (w >> 8) + 1 in [1, 2^24] as j <= 2^24 exp(-z^2 / (2 s^2))
-// = 2^(24 - c z^2), c = log2(e) / (2 s^2) (gauss_c). 24 - c z^2 = n + f with
-// n = floor (so f in [0, 1) is exact), 2^f by the polynomial, scaled by 2^n;
-// at z = 0 the bound is 2^24 and detection is certain.
+// z (from the centre, in the unit of s) is detected with probability
+// exp(-z^2 / (2 s^2)), drawn with the uniform u = x 2^-32 of the top 24 bits
+// of w, x = (float)(w | 255) (as for the Box-Muller radius), as
+// x <= 2^32 exp(-z^2 / (2 s^2)) = 2^(32 - c z^2), c = log2(e) / (2 s^2)
+// (gauss_c). 32 - c z^2 = n + f with n = floor (so f in [0, 1) is exact), 2^f
+// by the polynomial, scaled by 2^n; at z = 0 the bound is 2^32 and detection
+// is certain. Below -128 the exponent is clamped: 2^-128 is already under
+// every x, and n stays in the range of the conversion to int.
 __device__ __forceinline__ float gauss_c(float sig2) { return kLog2E / (sig2 + sig2); }
 
 __device__ __forceinline__ bool gauss_detect(float z, float c, uint32_t w)
 {
-    const float v = __builtin_fmaf(-c, z * z, 24.0f);
+    const float v = __builtin_fmaxf(__builtin_fmaf(-c, z * z, 32.0f), -128.0f);
     const float n = __builtin_floorf(v);
     const float f = v - n;
     float Q = kExp2Q5;
@@ -166,7 +169,7 @@ __device__ __forceinline__ bool gauss_detect(float z, float c, uint32_t w)
     Q = __builtin_fmaf(Q, f, kExp2Q1);
     Q = __builtin_fmaf(Q, f, kExp2Q0);
     const float t = __builtin_ldexpf(__builtin_fmaf(f, Q, 1.0f), (int)n);
-    return (float)((w >> 8) + 1u) <= t;
+    return (float)(w | 0xffu) <= t;
 }
 
 // Correctly rounded sqrt for the Box-Muller radius y = -2 ln u (2^-100 Q(0) =
@@ -205,12 +208,6 @@ __device__ __forceinline__ void sincos_tab(const float2* tab, uint32_t i, uint32
 
 __device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
 
-// Single periodic wrap of a centred coordinate (the reference's
-// periodic_bounds_: src/gpu/cusim/kernel-simulate-unroll.inl:24-32).
-__device__ __forceinline__ float wrap_periodic(float v, float h, float L)
-{
-    return (__builtin_fabsf(v) > h) ? v - __builtin_copysignf(L, v) : v;
-}
 // Reflecting walls at +-h: v' = copysign(2h, v) - v beyond them (single
 // reflection), as 2 med3(v, -h, h) - v in one fma: v itself inside (2v - v is
 // exact) and +-2h - v, rounded once, outside - the same value as the

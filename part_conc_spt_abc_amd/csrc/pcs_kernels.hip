@@ -326,7 +326,10 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                                           uint32_t s0, uint32_t s1)
 {
     if (p0 >= N) return;  // empty tile (a request without particles)
-    constexpr bool ZBOX = !GAUSS && !ZREF;  // z in box units
+    // z in box units when periodic: measured from the window's lower edge for
+    // the slab, from the box floor for Gaussian detection (centred distance
+    // q - 1/2); in um, centred, between reflecting walls.
+    constexpr bool ZBOX = !ZREF;
     // The counter words other than the particle and the key are the same on
     // every lane: held in SGPRs, the Philox rounds that see only them (round 1,
     // and half of rounds 2 and 3) run on the scalar unit.
@@ -339,7 +342,8 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
     const float hax = sy.half_a[0], hay = sy.half_a[1];
     const float ox = hax / Lx - 0.5f, oy = hay / Ly - 0.5f;  // window origin offsets
     const float wx = (hax + hax) / Lx, wy = (hay + hay) / Ly;  // window widths
-    const float cg = gauss_c(sy.detect_sigma * sy.detect_sigma);  // Gaussian detection scale
+    const float sdz = ZBOX ? sy.detect_sigma / Lz : sy.detect_sigma;  // Gaussian detection sigma, z's unit
+    const float cg = gauss_c(sdz * sdz);
     const uint32_t kmin = sy.kmin, C = sy.n_species;
     const float four_dt = sy.four_dt;
     const BinSpec bs{four_dt, sy.de_bin_width, sy.n_k, sy.n_de};
@@ -369,7 +373,11 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
         q.sx = sg / Lx;
         q.sy = sg / Ly;
         q.sig2 = sg * sg;
-        if (ZBOX) {
+        if (ZBOX && GAUSS) {
+            q.sz = sg / Lz;
+            q.wz = 0.0f;
+            q.z = u24(w.z);
+        } else if (ZBOX) {
             q.sz = sg / Lz;
             q.wz = (az + az) / Lz;
             q.z = fract_box(u24(w.z) + (az / Lz - 0.5f));
@@ -381,7 +389,7 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
         bool det;
         if (GAUSS) {
             const U4 wd = philox10(0xffffffffu, q.id, video, sim, k0, k1);
-            det = q.x <= wx && q.y <= wy && gauss_detect(q.z, cg, wd.x);
+            det = q.x <= wx && q.y <= wy && gauss_detect(ZBOX ? q.z - 0.5f : q.z, cg, wd.x);
         } else if (ZBOX) {
             det = q.x <= wx && q.y <= wy && q.z <= q.wz;
         } else {
@@ -438,10 +446,10 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                     q.z = fract_box(__builtin_fmaf(q.sz, nz, q.z));
                 } else {
                     const float zn = __builtin_fmaf(q.sz, nz, q.z);
-                    q.z = ZREF ? wrap_reflect(zn, hz) : wrap_periodic(zn, hz, Lz);
+                    q.z = wrap_reflect(zn, hz);
                 }
                 const bool cx = q.x <= wx, cy = q.y <= wy;
-                const bool cz = GAUSS ? gauss_detect(q.z, cg, wdet[i][s])
+                const bool cz = GAUSS ? gauss_detect(ZBOX ? q.z - 0.5f : q.z, cg, wdet[i][s])
                                       : (ZBOX ? q.z <= q.wz : __builtin_fabsf(q.z) <= q.wz);
                 const bool det = cx && cy && cz;
                 // Lanes closing a track: the wave mask is formed from the masks of

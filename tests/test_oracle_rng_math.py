@@ -134,9 +134,10 @@ def test_exp2_polynomial():
 
 def test_gauss_detect_decision():
     """The detection draw equals the exact decision u <= exp(-z^2 / (2 s^2)),
-    u = ((w >> 8) + 1) 2^-24, computed in FP64, for every (z, w) whose u is
-    not within 1e-6 relative of the bound; at z = 0 every w is detected, and
-    over uniform w the detected fraction is exp(-z^2 / (2 s^2))."""
+    u = (float)(w | 255) 2^-32, computed in FP64, for every (z, w) whose u is
+    not within 1e-6 relative of the bound; at z = 0 every w is detected, far
+    out (c z^2 beyond the int range of the exponent) none is, and over
+    uniform w the detected fraction is exp(-z^2 / (2 s^2))."""
     L = O.lib()
     rng = np.random.default_rng(3)
     for sig in (0.5, 1.0, 2.5):
@@ -144,12 +145,15 @@ def test_gauss_detect_decision():
         z = rng.uniform(-4 * sig, 4 * sig, 4000).astype(np.float32)
         w = rng.integers(0, 2 ** 32, 4000, dtype=np.uint64).astype(np.uint32)
         got = np.array([L.orc_gauss_detect(float(a), c, int(b)) for a, b in zip(z, w)], bool)
-        u = ((w >> 8).astype(np.float64) + 1.0) * 2.0 ** -24
+        u = (w | np.uint32(0xFF)).astype(np.float32).astype(np.float64) * 2.0 ** -32
         bound = np.exp(-z.astype(np.float64) ** 2 / (2.0 * np.float32(sig) ** 2))
         clear = np.abs(u - bound) > 1e-6 * bound
         assert np.array_equal(got[clear], (u <= bound)[clear])
         assert clear.mean() > 0.99
         assert all(L.orc_gauss_detect(0.0, c, int(x)) for x in (0, 0xFFFFFFFF, 0x12345678))
+    for tiny in (1e-3, 1e-6, 1e-18):  # c z^2 up to 1e38: the clamped exponent, never detected
+        c = L.orc_gauss_c(np.float32(tiny) * np.float32(tiny))
+        assert not any(L.orc_gauss_detect(30.0, c, int(x)) for x in (0, 0xFF, 0x12345678))
     c = L.orc_gauss_c(1.0)
     w = rng.integers(0, 2 ** 32, 20000, dtype=np.uint64).astype(np.uint32)
     for zz in (0.5, 1.0, 2.0):
