@@ -59,12 +59,12 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* ------------------------------------------------------------ FP32 math */
 
-/* ln(1+f) = f + f^2 * P(f) on [sqrt(1/2)-1, sqrt(2)-1]; P fitted by
- * iteratively reweighted least squares towards minimax
+/* ln(1+f) = f + f^2 * P(f) on [sqrt(1/2)-1, sqrt(2)-1], P of degree 6 fitted
+ * by iteratively reweighted least squares towards minimax
  * (part_conc_spt_abc_amd/csrc/fit_math_coeffs.py). */
-static const float LOG_P[8] = {
-    -0x1.fffff4p-2f, 0x1.5556e8p-2f, -0x1.0006c6p-2f, 0x1.98da1cp-3f,
-    -0x1.52fb0cp-3f, 0x1.30d166p-3f, -0x1.277654p-3f, 0x1.6fccd4p-4f,
+static const float LOG_P[7] = {
+    -0x1.000060p-1f, 0x1.5559f0p-2f, -0x1.ff7476p-3f, 0x1.976dd6p-3f,
+    -0x1.6049bep-3f, 0x1.4d64c0p-3f, -0x1.a7da34p-4f,
 };
 /* -2 ln 2 rounded to FP32 (2.8e-9 relative error). */
 static const float NEG2_LN2 = -0x1.62e43p+0f;
@@ -72,7 +72,7 @@ static const float SQRT_HALF = 0x1.6a09e6p-1f;
 
 /* -2 ln(1+f) = g + g^2 Q(g) in g = -2f with Q(g) = -P(-g/2)/2: Q's coefficients
  * are LOG_P[k] * LOG_QS[k], exact power-of-two scalings. */
-static const float LOG_QS[8] = { -0.5f, 0.25f, -0.125f, 0.0625f, -0.03125f, 0.015625f, -0.0078125f, 0.00390625f };
+static const float LOG_QS[7] = { -0.5f, 0.25f, -0.125f, 0.0625f, -0.03125f, 0.015625f, -0.0078125f };
 
 /* Box-Muller angle theta = 2 pi (a + 1/2) 2^-16 for the 16-bit code a = i << 8 | r:
  * the sector centre 2 pi (i + 1/2) / 256 plus beta = 2 pi (r - 127.5) 2^-16.
@@ -229,8 +229,8 @@ static inline float neg2log_i(uint32_t w)
     float g = fmaf(m, -2.0f, 2.0f);           /* -2 (m - 1), exact */
     float E = (float)(e - 32);                /* u = xf * 2^-32 */
     float g2 = fmaf(g, g, 0x1p-100f);
-    float Q = LOG_P[7] * LOG_QS[7];
-    for (int i = 6; i >= 0; --i) Q = fmaf(Q, g, LOG_P[i] * LOG_QS[i]);
+    float Q = LOG_P[6] * LOG_QS[6];
+    for (int i = 5; i >= 0; --i) Q = fmaf(Q, g, LOG_P[i] * LOG_QS[i]);
     return fmaf(E, NEG2_LN2, fmaf(g2, Q, g));
 }
 

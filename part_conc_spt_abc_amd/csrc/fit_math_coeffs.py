@@ -3,7 +3,7 @@
 (pcs_device.h; the oracle carries the same constants) and prints them as
 hex floats.
 
-- ln(1+f) = f + f^2 P(f), P of degree 7, on [sqrt(1/2)-1, sqrt(2)-1]
+- ln(1+f) = f + f^2 P(f), P of degree 6, on [sqrt(1/2)-1, sqrt(2)-1]
   (weighted least squares on Chebyshev nodes, reweighted iteratively by the
   residual - Lawson's algorithm - to approach the minimax polynomial, then
   rounded to FP32);
@@ -79,7 +79,7 @@ def beta_table():
 def main():
     a, b = np.sqrt(0.5) - 1, np.sqrt(2) - 1
     x = cheb(a, b)
-    A = np.stack([x ** p for p in range(8)], 1)
+    A = np.stack([x ** p for p in range(7)], 1)
     logp = lawson(A, (np.log1p(x) - x) / x ** 2, np.abs(x) + 1e-12, 80).astype(np.float32)
     print("LOG_P", [hexf(v) for v in logp])
     x = cheb(0.0, 1.0)

@@ -93,9 +93,9 @@ __device__ __forceinline__ U4 philox10(uint32_t blk, uint32_t particle, uint32_t
 // fit_math_coeffs.py (reweighted least squares towards minimax), rounded to
 // FP32. -2 ln 2 rounded to FP32 is within 2.8e-9 relative of the exact value,
 // so E (-2 ln 2) for |E| <= 24 is within 0.03 ulp of any result it enters.
-constexpr float kLogP0 = -0x1.fffff4p-2f, kLogP1 = 0x1.5556e8p-2f, kLogP2 = -0x1.0006c6p-2f,
-                kLogP3 = 0x1.98da1cp-3f, kLogP4 = -0x1.52fb0cp-3f, kLogP5 = 0x1.30d166p-3f,
-                kLogP6 = -0x1.277654p-3f, kLogP7 = 0x1.6fccd4p-4f;
+constexpr float kLogP0 = -0x1.000060p-1f, kLogP1 = 0x1.5559f0p-2f, kLogP2 = -0x1.ff7476p-3f,
+                kLogP3 = 0x1.976dd6p-3f, kLogP4 = -0x1.6049bep-3f, kLogP5 = 0x1.4d64c0p-3f,
+                kLogP6 = -0x1.a7da34p-4f;
 constexpr float kNeg2Ln2 = -0x1.62e43p+0f;
 constexpr float kSqrtHalf = 0x1.6a09e6p-1f;
 constexpr uint32_t kSqrtHalfBits = 0x3f3504f3u;  // bits of kSqrtHalf
@@ -104,7 +104,7 @@ static_assert(__builtin_bit_cast(uint32_t, kSqrtHalf) == kSqrtHalfBits, "bits of
 // whose coefficients are P's times powers of two (exact in FP32).
 constexpr float kLogQ0 = kLogP0 * -0.5f, kLogQ1 = kLogP1 * 0.25f, kLogQ2 = kLogP2 * -0.125f,
                 kLogQ3 = kLogP3 * 0.0625f, kLogQ4 = kLogP4 * -0.03125f, kLogQ5 = kLogP5 * 0.015625f,
-                kLogQ6 = kLogP6 * -0.0078125f, kLogQ7 = kLogP7 * 0.00390625f;
+                kLogQ6 = kLogP6 * -0.0078125f;
 constexpr float kLogTiny = 0x1p-100f;
 
 // -2 ln(u) for the uniform u = x 2^-32 of the top 24 bits of w, with
@@ -113,10 +113,11 @@ constexpr float kLogTiny = 0x1p-100f;
 // positive for every w; the low byte of w does not enter. With m, E as below,
 // g = -2 (m - 1) = fma(m, -2, 2) is exact and every Horner step in g is the
 // corresponding step in f = m - 1 scaled by a power of two, so the result is
-// -2 (E ln 2 + f + f^2 P(f)) with the same roundings. g^2 carries an added
-// 2^-100, which changes the result only where u = 1 (then 2^-100 Q(0), about
-// 2^-102, instead of 0, so the square root below never sees 0) and where g^2
-// is exactly halfway between two floats (rounded up instead of to even).
+// -2 (E ln 2 + f + f^2 P(f)) with the same roundings (within 2.4 ulp of the
+// exact value; the radius too). g^2 carries an added 2^-100, which changes the
+// result only where u = 1 (then 2^-100 Q(0), about 2^-102, instead of 0, so
+// the square root below never sees 0) and where g^2 is exactly halfway
+// between two floats (rounded up instead of to even).
 __device__ __forceinline__ float neg2log_u24(uint32_t w)
 {
     // x = m 2^e with m in [sqrt(1/2), sqrt(2)), in integer steps on the bits
@@ -129,8 +130,7 @@ __device__ __forceinline__ float neg2log_u24(uint32_t w)
     const float g = __builtin_fmaf(m, -2.0f, 2.0f);  // exact (Sterbenz)
     const float E = (float)(t >> 23);
     const float g2 = __builtin_fmaf(g, g, kLogTiny);
-    float Q = kLogQ7;
-    Q = __builtin_fmaf(Q, g, kLogQ6);
+    float Q = kLogQ6;
     Q = __builtin_fmaf(Q, g, kLogQ5);
     Q = __builtin_fmaf(Q, g, kLogQ4);
     Q = __builtin_fmaf(Q, g, kLogQ3);
@@ -173,12 +173,11 @@ __device__ __forceinline__ bool gauss_detect(float z, float c, uint32_t w)
 }
 
 // Correctly rounded sqrt for the Box-Muller radius y = -2 ln u (2^-100 Q(0) =
-// 0x1.fffff4p-103 at u = 1, else 1.19e-7 <= y <= 33.3): s = y rsq(y) with one
+// 0x1.00006p-102 at u = 1, else 1.19e-7 <= y <= 33.3): s = y rsq(y) with one
 // correction by the residual y - s^2 (Markstein). It equals the correctly
-// rounded square root on every float in [2^-102, 64) and at 0x1.fffff4p-103
-// (checked exhaustively on the GPU, each result against the midpoints to its
-// neighbours squared exactly in FP64; it misses on one of the five floats in
-// between, and on many far below, where the residual term is subnormal); the
+// rounded square root on every float in [2^-102, 64) (checked exhaustively on
+// the GPU, each result against the midpoints to its neighbours squared exactly
+// in FP64; far below, where the residual term is subnormal, it misses); the
 // hardware square root alone misses on 15 % of the radius inputs.
 __device__ __forceinline__ float sqrt_rn_domain(float y)
 {

@@ -61,19 +61,15 @@ def test_device_math_bit_exact_all_inputs(gpu_lib):
 
 def test_device_sqrt_correctly_rounded_on_its_range(gpu_lib):
     """The walk's square root (rsq + one Markstein correction) is correctly
-    rounded on every float in [2^-102, 64) (the largest input is 33.3) and on
-    its one input below 2^-102, -2 ln u at u = 1 (the tiny term
-    2^-100 Q(0) = 0x1.fffff4p-103), so the oracle's sqrtf matches it on
-    whatever the log gives it. Below 2^-102 it is not correctly rounded on
-    every float: one of the five floats between that input and 2^-102 (never
-    a radius input) misses, and far below, where the correction's residual
-    term is subnormal, many do."""
+    rounded on every float in [2^-102, 64), which holds every input the log
+    gives it: the smallest, -2 ln u at u = 1, is the tiny term
+    2^-100 Q(0) = 0x1.00006p-102, the largest 33.3. So the oracle's sqrtf
+    matches it. Far below 2^-102, where the correction's residual term is
+    subnormal, it misses."""
     bits = lambda v: int(np.array([v], np.float32).view(np.uint32)[0])
-    y1 = float.fromhex("0x1.fffff4p-103")
-    assert np.float32(y1) == y1 and bits(2.0 ** -102) - bits(y1) == 6
+    y1 = 2.0 ** -100 * (-float.fromhex("-0x1.000060p-1") / 2.0)
+    assert np.float32(y1) == y1 and 2.0 ** -102 <= y1
     assert selftest_sqrt(bits(2.0 ** -102), bits(64.0)) == 0
-    assert selftest_sqrt(bits(y1), bits(y1) + 1) == 0
-    assert selftest_sqrt(bits(y1) + 1, bits(2.0 ** -102)) == 1
     assert selftest_sqrt(bits(2.0 ** -126), bits(2.0 ** -118)) > 0
 
 

@@ -62,11 +62,12 @@ def test_radius_all_inputs(all_words):
     assert np.array_equal(x, np.where(J < 2 ** 16, 256.0 * J + 255.0, 256.0 * (J + 1.0)))
     want = np.sqrt(-2.0 * np.log(x * 2.0 ** -32))
     # u = 1: -2 ln u is 2^-100 Q(0) with Q(0) = -P0/2 (the tiny term that keeps
-    # the square root's input positive), P0 = -0x1.fffff4p-2; its root, 4.4e-16
-    y1 = np.float64(2.0 ** -100) * (-float.fromhex("-0x1.fffff4p-2") / 2.0)
+    # the square root's input positive), P0 = -0x1.000060p-1; its root, 4.4e-16
+    y1 = np.float64(2.0 ** -100) * (-float.fromhex("-0x1.000060p-1") / 2.0)
     assert r[-1] == np.float32(np.sqrt(y1)) and 0.0 < r[-1] < 1e-15
     e = ulps(r[:-1], want[:-1])
-    assert e.max() <= 1.5, e.max()
+    assert e.max() <= 2.5, e.max()  # a degree-6 P: 2.3 ulp at most, 0.45 on average
+    assert e.mean() <= 0.5, e.mean()
     # radius is non-increasing in u
     assert np.all(np.diff(r.astype(np.float64)) <= 0)
 
