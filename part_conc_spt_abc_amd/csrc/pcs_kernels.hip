@@ -300,16 +300,20 @@ __device__ __forceinline__ void stage_push2(Stage& st, bool ok, uint32_t a, uint
 __device__ __forceinline__ float fract_box(float v) { return __builtin_amdgcn_fractf(v); }
 
 // de is the track's sum of squared lateral steps in units of sigma^2 (the sum
-// of nx^2 + ny^2 over its unit normals); track_de gives it in um^2.
+// of nx^2 + ny^2 over its unit normals); track_de gives it in um^2. The open
+// track's length is kept as the frame it started in, ts, counted from the
+// first frame before the current 4-step chunk (rebased after each chunk): in
+// that frame numbering the track ending at frame f has k = f - ts + 1, and
+// the per-step update is one select instead of an add and a select.
 struct Particle {
     float x, y, z, de;
     float sx, sy, sz;  // step per unit normal: sigma / L in box units, sigma for z in um
     float wz;          // z window: 2 haz / Lz in box units, haz for z in um
     float sig2;        // sigma^2
-    uint32_t k;
+    int32_t ts;        // first frame of the open track (meaningful while in the window)
     uint32_t id;
     bool active;
-    bool prev;         // inside the window in the previous frame
+    bool det0;         // inside the window in the first frame
 };
 
 __device__ __forceinline__ float track_de(const Particle& q) { return q.sig2 * q.de; }
@@ -395,14 +399,20 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
         } else {
             det = q.x <= wx && q.y <= wy && __builtin_fabsf(q.z) <= q.wz;
         }
-        q.k = det ? 1u : 0u;
-        q.prev = det;
+        q.ts = 0;
+        q.det0 = det;
         q.de = 0.0f;
     }
 
-    uint64_t m_active[PPL];  // lanes holding a particle, per particle slot
+    // Per particle slot, wave masks of the lanes holding a particle and of the
+    // lanes inside the window in the previous frame. Kept as masks (SGPRs), the
+    // per-lane selects read them directly (inverse ballot).
+    uint64_t m_active[PPL], m_prev[PPL];
 #pragma unroll
-    for (int i = 0; i < PPL; ++i) m_active[i] = ballot(pt[i].active);
+    for (int i = 0; i < PPL; ++i) {
+        m_active[i] = ballot(pt[i].active);
+        m_prev[i] = ballot(pt[i].det0);
+    }
 
     const uint32_t steps = F > 0u ? F - 1u : 0u;
     for (uint32_t c = 0; 4u * c < steps; ++c) {
@@ -451,31 +461,41 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                 const bool cx = q.x <= wx, cy = q.y <= wy;
                 const bool cz = GAUSS ? gauss_detect(ZBOX ? q.z - 0.5f : q.z, cg, wdet[i][s])
                                       : (ZBOX ? q.z <= q.wz : __builtin_fabsf(q.z) <= q.wz);
-                const bool det = cx && cy && cz;
-                // Lanes closing a track: the wave mask is formed from the masks of
-                // the single comparisons (no per-lane boolean is materialised).
-                const uint64_t m_closing = m_active[i] & ~(ballot(cx) & ballot(cy) & ballot(cz)) & ballot(q.k >= kmin);
+                // Lanes inside the window and lanes closing a track: wave masks
+                // formed from the masks of the single comparisons (no per-lane
+                // boolean is materialised). This step reaches frame s + 1; the
+                // open track ends at frame s with k = s + 1 - ts, at least kmin
+                // iff ts <= s + 1 - kmin.
+                const int32_t f1 = (int32_t)s + 1;
+                const uint64_t m_det = ballot(cx) & ballot(cy) & ballot(cz);
+                const uint64_t m_closing = m_active[i] & m_prev[i] & ~m_det & ballot(q.ts <= f1 - (int32_t)kmin);
                 if (m_closing != 0ull) {
-                    const bool closing = ((m_closing >> lane_id()) & 1ull) != 0ull;
+                    const bool closing = __builtin_amdgcn_inverse_ballot_w64(m_closing);
+                    const uint32_t k = (uint32_t)(f1 - q.ts);
                     float v = track_de(q);
-                    if (TRACKS && closing) v = v / ((float)(q.k - 1u) * four_dt);
-                    stage_push2<!TRACKS>(st, closing, q.k, __float_as_uint(v), bs);
+                    if (TRACKS && closing) v = v / ((float)(k - 1u) * four_dt);
+                    stage_push2<!TRACKS>(st, closing, k, __float_as_uint(v), bs);
                 }
                 const float de_n = __builtin_fmaf(ny, ny, __builtin_fmaf(nx, nx, q.de));
-                q.de = (det && q.prev) ? de_n : 0.0f;
-                q.k = det ? q.k + 1u : 0u;
-                q.prev = det;
+                q.de = __builtin_amdgcn_inverse_ballot_w64(m_det & m_prev[i]) ? de_n : 0.0f;
+                q.ts = __builtin_amdgcn_inverse_ballot_w64(m_det & ~m_prev[i]) ? f1 : q.ts;
+                m_prev[i] = m_det;
             }
         }
+#pragma unroll
+        for (int i = 0; i < PPL; ++i) pt[i].ts -= (int32_t)nst;  // the chunk's last frame becomes frame 0
     }
-    // Tracks still open in the last frame (src/cpu/simulate.jl:114-121).
+    // Tracks still open in the last frame (src/cpu/simulate.jl:114-121), which
+    // is frame 0 after the rebase: k = 1 - ts.
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
-        const bool closing = pt[i].active && pt[i].k >= kmin;
-        if (ballot(closing) != 0ull) {
+        const uint64_t m_closing = m_active[i] & m_prev[i] & ballot(pt[i].ts <= 1 - (int32_t)kmin);
+        if (m_closing != 0ull) {
+            const bool closing = __builtin_amdgcn_inverse_ballot_w64(m_closing);
+            const uint32_t k = (uint32_t)(1 - pt[i].ts);
             float v = track_de(pt[i]);
-            if (TRACKS && closing) v = v / ((float)(pt[i].k - 1u) * four_dt);
-            stage_push2<!TRACKS>(st, closing, pt[i].k, __float_as_uint(v), bs);
+            if (TRACKS && closing) v = v / ((float)(k - 1u) * four_dt);
+            stage_push2<!TRACKS>(st, closing, k, __float_as_uint(v), bs);
         }
     }
     if (TRACKS) stage_flush(st);
