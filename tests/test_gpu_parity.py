@@ -203,6 +203,21 @@ def test_graph_replays_with_eager_work_between(gpu_lib):
         b.close()
 
 
+@pytest.mark.parametrize("kmin", [2, 7, 40])
+def test_kmin_values(gpu_lib, kmin):
+    """Shortest recorded track from the smallest allowed (2: one step, one
+    squared displacement) to one far above the typical track, over videos
+    whose step counts fill the last 4-step chunk (96) or leave it partial (499)."""
+    model = cases.headline_model(n_k=500, n_de=200, kmin=kmin)
+    obs = observed(model, frames=[500])
+    rng = np.random.default_rng(40 + kmin)
+    props, f, n = cases.draw_proposals(model, 6, rng, frames_choices=(97, 500), c_range=(1e8, 1e9))
+    with Context(model, obs) as ctx:
+        for mode in (_abi.PCS_MODE_PLAIN, _abi.PCS_MODE_BATCHED):
+            dist, tracks, H, exact, _ = run_gpu(ctx, props, f, n, 31, mode)
+            check_against_oracle(model, obs, props, f, n, 31, dist, tracks, H, exact)
+
+
 def test_empty_and_tiny_inputs(gpu_lib):
     model = cases.cpu_model()
     obs = observed(model)
@@ -313,8 +328,9 @@ def test_generate_tracks_matches_oracle(gpu_lib):
     multiset as the oracle, DE bit for bit."""
     import ctypes as C
 
-    for variant in ("slab", "gauss"):
-        kw = dict(detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0) if variant == "gauss" else {}
+    for variant in ("slab", "gauss", "kmin2"):
+        kw = {"slab": {}, "gauss": dict(detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0),
+              "kmin2": dict(kmin=2)}[variant]  # kmin 2: tracks of one step, divisor (k - 1) 4 dt = 4 dt
         model = cases.headline_model(**kw)
         p, f, n = cases.fixed_proposal(model, [900, 0, 130], [1000, 500, 2000], sim_id=41)
         with Context(model) as ctx:
