@@ -15,8 +15,11 @@
  *     u = (float)(w | 255) 2^-32, see neg2log_i),
  *     angle of pair p: 16-bit code a = (low byte of word p) << 8 | byte (p % 4) of
  *     word 6 + p/4, theta = 2 pi (a + 1/2) 2^-16;
- *     pair p gives unit normals n[2p] = r*sin, n[2p+1] = r*cos, step s of the
- *     chunk moves each coordinate by fma(sigma, n, x) with n = (n[3s], n[3s+1], n[3s+2]);
+ *     pair p gives unit normals n[2p] = r*sin, n[2p+1] = r*cos; step s of the
+ *     chunk moves each coordinate by fma(sigma, n, x) with n = (n[2s], n[2s+1], n[8+s]):
+ *     pairs 0..3 give steps 0..3 their x and y, pairs 4 and 5 the four z steps, and
+ *     the track's squared lateral step nx^2 + ny^2 is taken as pair s's -2 ln u (r^2
+ *     before the square root);
  *     words 8..11 (gaussian only) -> detection uniform of step s = word 8+s.
  * A video of F frames has F-1 steps (frames 2..F in src/cpu/simulate.jl:82).
  */
@@ -318,17 +321,20 @@ static inline float u24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
  * Gaussian detection uniforms. */
 static inline uint32_t chunk_blocks(int gauss) { return gauss ? 3u : 2u; }
 
-/* Unit normals of chunk c in FP32 (twelve, three per step); the detection
+/* Unit normals of chunk c in FP32 (twelve: pair p in d[2p], d[2p+1]) and the
+ * squared radii r^2 = -2 ln u of pairs 0..3 (the lateral steps); the detection
  * words of the chunk go to wdet when gauss. */
 static void chunk_f32(const stream_t* st, uint32_t c, int gauss, uint32_t part, uint32_t video,
-                      float d[12], uint32_t wdet[4])
+                      float d[12], float r2[4], uint32_t wdet[4])
 {
     const uint32_t B = chunk_blocks(gauss);
     uint32_t w[8];
     block(st, B * c + 1u, part, video, w);
     block(st, B * c + 2u, part, video, w + 4);
     for (int pr = 0; pr < 6; ++pr) {
-        float r = bm_radius_i(w[pr]);
+        const float y = neg2log_i(w[pr]);
+        float r = sqrtf(y);
+        if (pr < 4) r2[pr] = y;
         float sn, cs;
         const uint32_t a = bm_angle16(w[pr], w[6 + pr / 4], (uint32_t)pr);
         sincos_i(a >> 8, a & 0xffu, &sn, &cs);
@@ -428,12 +434,12 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
         float de = 0.0f;
 
         const uint32_t steps = F > 0 ? F - 1u : 0u;
-        float d[12];
+        float d[12], r2[4];
         uint32_t wdet[4] = {0, 0, 0, 0};
         for (uint32_t st_i = 0; st_i < steps; ++st_i) {
             uint32_t c = st_i >> 2, sl = st_i & 3u;
-            if (sl == 0) chunk_f32(st, c, gauss, part, video, d, wdet);
-            const float nx = d[3 * sl], ny = d[3 * sl + 1], nz = d[3 * sl + 2];
+            if (sl == 0) chunk_f32(st, c, gauss, part, video, d, r2, wdet);
+            const float nx = d[2 * sl], ny = d[2 * sl + 1], nz = d[8 + sl];
             x = fract_box(fmaf(sx, nx, x));
             y = fract_box(fmaf(sy, ny, y));
             if (zbox) {
@@ -447,12 +453,10 @@ static void walk_video_f32(const stream_t* st, const orc_proposal* p, uint32_t v
             else
                 det = x <= wx && y <= wy && (zbox ? z <= wz : fabsf(z) <= wz);
 
-            /* de: squared lateral steps in units of sigma^2, scaled on record. */
+            /* de: squared lateral steps in units of sigma^2 (nx^2 + ny^2 as the
+             * pair's r^2), scaled on record. */
             if (det) {
-                if (k >= 1u) {
-                    de = fmaf(nx, nx, de);
-                    de = fmaf(ny, ny, de);
-                }
+                if (k >= 1u) de = de + r2[sl];
                 k = k + 1u;
             } else {
                 if (k >= kmin) rec(rctx, k, sig2 * de);
@@ -546,7 +550,16 @@ void orc_chunk_normals_n(uint64_t seed, uint32_t sim_id, uint32_t video, uint32_
     uint32_t wdet[4];
     for (uint32_t part = 0; part < n_particles; ++part)
         for (uint32_t c = 0; c < n_chunks; ++c)
-            chunk_f32(&st, c, gauss, part, video, out + ((uint64_t)part * n_chunks + c) * 12u, wdet);
+        {
+            float d[12], r2[4];
+            chunk_f32(&st, c, gauss, part, video, d, r2, wdet);
+            float* o = out + ((uint64_t)part * n_chunks + c) * 12u;
+            for (int s = 0; s < 4; ++s) {
+                o[3 * s] = d[2 * s];
+                o[3 * s + 1] = d[2 * s + 1];
+                o[3 * s + 2] = d[8 + s];
+            }
+        }
 }
 
 /* ------------------------------------------------------------ walk (FP64) */
@@ -641,7 +654,7 @@ static void walk_particle_f64(const stream_t* st, const orc_proposal* p, const o
             normals_f64(st, c, B, part, video, sigma, d);
             if (gauss) block(st, B * c + 3u, part, video, wdet);
         }
-        const double dx = d[3 * sl], dy = d[3 * sl + 1], dz = d[3 * sl + 2];
+        const double dx = d[2 * sl], dy = d[2 * sl + 1], dz = d[8 + sl];
         x = wrap64(x + dx, hx, 2.0 * hx, 0);
         y = wrap64(y + dy, hy, 2.0 * hy, 0);
         z = wrap64(z + dz, hz, 2.0 * hz, sys->z_boundary != 0);
@@ -712,13 +725,14 @@ void orc_trace_particle(const orc_system* sys, const orc_proposal* p, const orc_
             sc[a] = sigma;
         }
     }
-    float d[12];
+    float d[12], r2[4];
     for (uint32_t i = 0; i < F; ++i) {
         if (i > 0) {
             uint32_t c = (i - 1) >> 2, sl = (i - 1) & 3u;
-            if (sl == 0) chunk_f32(&st, c, gauss, part, video, d, wdet);
+            if (sl == 0) chunk_f32(&st, c, gauss, part, video, d, r2, wdet);
+            const float n[3] = { d[2 * sl], d[2 * sl + 1], d[8 + sl] };
             for (int a = 0; a < 3; ++a) {
-                const float v = fmaf(sc[a], d[3 * sl + a], q[a]);
+                const float v = fmaf(sc[a], n[a], q[a]);
                 if (a < 2 || zbox) q[a] = fract_box(v);
                 else q[a] = wrap_reflect(v, hz, L[2]);
             }

@@ -417,7 +417,8 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
     const uint32_t steps = F > 0u ? F - 1u : 0u;
     for (uint32_t c = 0; 4u * c < steps; ++c) {
         const uint32_t nst = min(4u, steps - 4u * c);
-        float d[PPL][12];
+        float d[PPL][12];   // pair p: d[2p] = r sin, d[2p + 1] = r cos
+        float r2[PPL][4];   // r^2 = -2 ln u of pairs 0..3
         uint32_t wdet[PPL][4];
 #pragma unroll
         for (int i = 0; i < PPL; ++i) {
@@ -428,7 +429,9 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
             const uint32_t w[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
             for (int p = 0; p < 6; ++p) {
-                const float r = bm_radius(w[p]);
+                const float y = neg2log_u24(w[p]);
+                const float r = sqrt_rn_domain(y);
+                if (p < 4) r2[i][p] = y;
                 float sn, cs;
                 sincos_tab(tab, w[p] & 0xffu, (w[6 + p / 4] >> (8 * (p & 3))) & 0xffu, sn, cs);
                 d[i][2 * p] = r * sn;
@@ -448,8 +451,9 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
 #pragma unroll
             for (int i = 0; i < PPL; ++i) {
                 Particle& q = pt[i];
-                // Unit normals of this step; the particle moves by sigma * n.
-                const float nx = d[i][3 * s], ny = d[i][3 * s + 1], nz = d[i][3 * s + 2];
+                // Unit normals of this step (x and y from pair s, z from pairs 4
+                // and 5); the particle moves by sigma * n.
+                const float nx = d[i][2 * s], ny = d[i][2 * s + 1], nz = d[i][8 + s];
                 q.x = fract_box(__builtin_fmaf(q.sx, nx, q.x));
                 q.y = fract_box(__builtin_fmaf(q.sy, ny, q.y));
                 if (ZBOX) {
@@ -476,7 +480,8 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                     if (TRACKS && closing) v = v / ((float)(k - 1u) * four_dt);
                     stage_push2<!TRACKS>(st, closing, k, __float_as_uint(v), bs);
                 }
-                const float de_n = __builtin_fmaf(ny, ny, __builtin_fmaf(nx, nx, q.de));
+                // nx^2 + ny^2 of this step is pair s's r^2, the log's result
+                const float de_n = q.de + r2[i][s];
                 q.de = __builtin_amdgcn_inverse_ballot_w64(m_det & m_prev[i]) ? de_n : 0.0f;
                 q.ts = __builtin_amdgcn_inverse_ballot_w64(m_det & ~m_prev[i]) ? f1 : q.ts;
                 m_prev[i] = m_det;
