@@ -142,7 +142,8 @@ def test_tuning_does_not_change_results(gpu_lib, tune):
     check_against_oracle(model, obs, props, f, n, 21, dist, tracks, H, exact)
 
 
-@pytest.mark.parametrize("variant", ["reflect", "gauss", "five_species_l1", "variable_depth"])
+@pytest.mark.parametrize("variant", ["reflect", "gauss", "five_species_l1", "variable_depth", "non_square",
+                                     "non_square_gauss_reflect"])
 def test_model_variants(gpu_lib, variant):
     kw = {}
     m = az = None
@@ -150,6 +151,11 @@ def test_model_variants(gpu_lib, variant):
         kw = dict(z_boundary=_abi.PCS_Z_REFLECT)
     elif variant == "gauss":
         kw = dict(detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0)
+    elif variant.startswith("non_square"):
+        # every axis its own length and window, so a swapped axis shows
+        kw = dict(Lx=230.0, Ly=140.0, Lz=26.0, ax=150.0, ay=70.0)
+        if variant.endswith("gauss_reflect"):
+            kw.update(detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.5, z_boundary=_abi.PCS_Z_REFLECT)
     elif variant == "five_species_l1":
         kw = dict(n_species=5, dist_kind=_abi.PCS_DIST_L1)
         m = FIVE_CLASSES

@@ -125,8 +125,12 @@ def test_tracks_rederived_from_trajectory(sim):
     assert np.allclose([v for _, v in want], DE, rtol=2e-5, atol=1e-6)
 
 
-def test_fp64_twin_and_drift():
-    model = cases.headline_model()
+@pytest.mark.parametrize("box", [(200.0, 200.0, 30.0, 160.0, 120.0), (230.0, 140.0, 26.0, 150.0, 70.0)])
+def test_fp64_twin_and_drift(box):
+    """The FP32 walk (box units) against its FP64 twin (um, libm), in the
+    headline box and in one where every axis has its own length and window."""
+    Lx, Ly, Lz, ax, ay = box
+    model = cases.headline_model(Lx=Lx, Ly=Ly, Lz=Lz, ax=ax, ay=ay)
     p, f, n = cases.fixed_proposal(model, [1200], [1000], sim_id=11)
     osys = O.system_from_abi(model.to_abi())
     op = O.OracleProposal(p, f, n)
@@ -144,7 +148,7 @@ def test_fp64_twin_and_drift():
     for part in range(0, 1200, 60):
         x32, x64 = O.trace_particle(osys, op, pd, 1, 0, part)
         dlt = np.abs(x32.astype(np.float64) - x64)
-        L = np.array([200.0, 200.0, 30.0])
+        L = np.array([Lx, Ly, Lz])
         dlt = np.minimum(dlt, L - dlt)
         worst = max(worst, dlt.max())
     assert worst < 5e-3, worst
