@@ -19,9 +19,17 @@
 //                    table, whose rows are loaded ahead. Replaces K_distance
 //                    (src/gpu/cusim/kernel-distance.inl:81-241), a single 32x32
 //                    block that scanned the whole dense histogram in FP32.
+//   serve_loop_kernel  device-resident request loop: one admitter wave copies
+//                    requests from a ring in pinned host memory into HBM slots,
+//                    the other groups take their tiles, walk them and run the
+//                    distance of a request once its last tile is done.
+//   pmc_sums_kernel  ABC-PMC importance-weight sums in FP64
+//                    (src/gpu/detail/simulation_impl.inl:832-871).
 //   dense_hist       scatter of records into dense histograms (tests, export).
 //   reset_kernel     zeroes per-proposal counters (a kernel rather than a
 //                    memset so it can sit inside a replayed hipGraph).
+//   selftest_*       exhaustive checks of the device math and a copy kernel
+//                    that calibrates the byte counters (test hooks).
 #include "pcs_device.h"
 #include "pcs_internal.h"
 
