@@ -130,9 +130,12 @@ def test_batched_mixed_sizes(gpu_lib):
 
 
 @pytest.mark.parametrize("tune", [dict(particles_per_lane=2), dict(waves_per_group=1, stage_records=128),
-                                  dict(waves_per_group=2, groups_per_cu=2, stage_records=512)])
+                                  dict(waves_per_group=2, groups_per_cu=2, stage_records=512),
+                                  dict(particles_per_lane=2, gauss=True)])
 def test_tuning_does_not_change_results(gpu_lib, tune):
-    model = cases.headline_model(n_k=600, n_de=300)
+    tune = dict(tune)
+    kw = dict(detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0) if tune.pop("gauss", False) else {}
+    model = cases.headline_model(n_k=600, n_de=300, **kw)
     obs = observed(model, frames=[600])
     rng = np.random.default_rng(5)
     props, f, n = cases.draw_proposals(model, 10, rng, frames_choices=(300, 600), videos=3, c_range=(1e8, 1e9))
