@@ -34,12 +34,12 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
 // Philox4x32-10 of the counter (video, particle, sim, blk). video, sim and the
 // key are fixed for a tile and wave-uniform, the particle word varies per lane
 // and is fixed for a tile, blk is wave-uniform and changes from chunk to chunk.
-// Rounds 1-3 are written out: rounds 1 and 2 mix the lane word with tile
+// Rounds 1-4 are written out: rounds 1 and 2 mix the lane word with tile
 // constants only (the compiler hoists them out of the chunk loop, so a chunk's
 // block pays for them with scalar work), round 3 joins the lane halves with the
-// block halves by XOR, and from round 4 on every word varies per lane. Uniform
-// XOR operands are grouped so they fold on the scalar unit. Same function as
-// the textbook round loop.
+// block halves by XOR, round 4 still takes one uniform word, and from round 5
+// on every word varies per lane. Uniform XOR operands are grouped so they fold
+// on the scalar unit. Same function as the textbook round loop.
 __device__ __forceinline__ U4 philox10(uint32_t blk, uint32_t particle, uint32_t video, uint32_t sim,
                                        uint32_t k0, uint32_t k1)
 {
@@ -70,8 +70,21 @@ __device__ __forceinline__ U4 philox10(uint32_t blk, uint32_t particle, uint32_t
     uint32_t c3 = (uint32_t)c0p;                                 // uniform, per chunk
     k0 += kPhiloxW0;
     k1 += kPhiloxW1;
+    // round 4: c3 is still uniform, so its XOR with the key is scalar work
+    {
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ (c3 ^ k1);
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
 #pragma unroll
-    for (int r = 3; r < 10; ++r) {
+    for (int r = 4; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
         const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
         const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
