@@ -203,7 +203,9 @@ def large_model(n_sims, seed=9, device_loop=False):
     lat = np.array(lat)
     return {"sims": n_sims, "particles_per_sim": steps / n_sims / 1999.0, "sims_per_s": n_sims / wall,
             "particle_steps_per_s": steps / wall, "p50_ms": float(np.percentile(lat, 50) / 1e3),
-            "p99_ms": float(np.percentile(lat, 99) / 1e3)}
+            "p99_ms": float(np.percentile(lat, 99) / 1e3),
+            # every walk launched, the warm-up request included (for profiles)
+            "particle_steps_launched": steps + float(reqs[0][2][0]) * 1999.0}
 
 
 def cpu_baseline(model, props, frames, parts, obs, n_sample):
