@@ -129,6 +129,48 @@ def test_batched_mixed_sizes(gpu_lib):
             check_against_oracle(model, obs, props, f, n, 11, dist, tracks, H, exact)
 
 
+def test_full_generation_properties(gpu_lib):
+    """One generation at its full size: 100,000 proposals, total c log-uniform
+    on 1e8..1e10 /mL (120-12,000 particles), frames 500/1,000/2,000, H 1000 x
+    500. Graph replay, direct launches and a repeat agree bit for bit; every
+    distance is log10 of its exact sum; proposals taken out of the generation
+    and simulated alone in plain launches reproduce their tracks and exact
+    distances, and their histograms equal the oracle's."""
+    model = cases.headline_model()
+    obs = observed(model)
+    rng = np.random.default_rng(2024)
+    props, f, n = cases.draw_proposals(model, 100_000, rng, frames_choices=(500, 1000, 2000))
+    steps = n.astype(np.int64) * f
+    pick = [int(np.argmin(steps)), int(np.argmax(steps))] + [int(i) for i in rng.choice(len(props), 6, replace=False)]
+    print("sample", pick, "particles", n[pick].tolist(), "frames", f[pick].tolist(), flush=True)
+    with Context(model, obs) as ctx:
+        b = ctx.batch()
+        b.load(props, f, n)
+        runs = []
+        for mode in (_abi.PCS_MODE_BATCHED, _abi.PCS_MODE_BATCHED_DIRECT, _abi.PCS_MODE_BATCHED):
+            b.run(31, mode)
+            dist, tracks, _ = b.read()
+            runs.append((dist, tracks, b.read_exact()))
+        b.close()
+        dist, tracks, exact = runs[0]
+        for d2, t2, e2 in runs[1:]:
+            assert np.array_equal(exact, e2) and np.array_equal(tracks, t2) and np.array_equal(dist, d2)
+        pos = exact > 0
+        assert pos.mean() > 0.99
+        np.testing.assert_allclose(dist[pos], np.log10(exact[pos].astype(np.float64)), rtol=0, atol=1e-12)
+        assert tracks[int(np.argmax(steps))] > tracks[int(np.argmin(steps))]
+
+        sub = []
+        for j, i in enumerate(pick):
+            q = _abi.pcs_proposal.from_buffer_copy(props[i])
+            q.first_video = j
+            sub.append(q)
+        fs, ns = f[pick], n[pick]
+        sd, st, sH, se, _ = run_gpu(ctx, sub, fs, ns, 31, _abi.PCS_MODE_PLAIN)
+    assert np.array_equal(se, exact[pick]) and np.array_equal(st, tracks[pick])
+    check_against_oracle(model, obs, sub, fs, ns, 31, sd, st, sH, se)
+
+
 @pytest.mark.parametrize("tune", [dict(particles_per_lane=2), dict(waves_per_group=1, stage_records=128),
                                   dict(waves_per_group=2, groups_per_cu=2, stage_records=512),
                                   dict(particles_per_lane=2, gauss=True)])
