@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batched-sims", type=int, default=100_000)
     ap.add_argument("--plain-sims", type=int, default=10_000)
-    ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--serve-requests", type=int, default=10_000)
     ap.add_argument("--serve-rates", default="2000,5000,10000,20000,50000")
     ap.add_argument("--cpu-sample", type=int, default=384)
     ap.add_argument("--no-cpu", action="store_true")
