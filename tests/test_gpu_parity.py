@@ -374,6 +374,29 @@ def test_large_polydisperse_model(gpu_lib):
     assert worst < 1e-2
 
 
+def test_large_polydisperse_model_full_size(gpu_lib):
+    """The same configuration at its full size: c = 1e10 /mL in the 400x400x60
+    um box, a Poisson(96,000) particle count (95,727 particles x 2000 frames),
+    bit-equal to the oracle in the batched and plain forms (the oracle takes
+    10-20 s)."""
+    model = cases.headline_model(n_k=2000, n_de=1000, Lx=400.0, Ly=400.0, Lz=60.0, n_species=5,
+                                 detect_model=_abi.PCS_DETECT_GAUSS_Z, detect_sigma=1.0)
+    rng = np.random.default_rng(51)
+    w = rng.dirichlet(np.ones(5))
+    n = int(rng.poisson(1e10 * model.volume_factor))
+    assert n == 95727
+    p = cases.proposal_record(model, FIVE_CLASSES, list(w * 1e10), [2.0], 78, 0, 1)
+    f, n = np.array([2000], np.uint32), np.array([n], np.uint32)
+    obs = observed(model, particles=3000, frames=[2000])
+    with Context(model, obs) as ctx:
+        runs = [run_gpu(ctx, [p], f, n, 8, mode) for mode in (_abi.PCS_MODE_BATCHED, _abi.PCS_MODE_PLAIN)]
+    for dist, tracks, H, exact, _ in runs[1:]:
+        assert np.array_equal(H, runs[0][2]) and np.array_equal(exact, runs[0][3])
+    dist, tracks, H, exact, _ = runs[0]
+    print("tracks", int(tracks[0]), flush=True)
+    check_against_oracle(model, obs, [p], f, n, 8, dist, tracks, H, exact)
+
+
 def test_generate_tracks_matches_oracle(gpu_lib):
     """Track-list form (generate_experiment.jl) on the GPU: the same (K, DE)
     multiset as the oracle, DE bit for bit."""
