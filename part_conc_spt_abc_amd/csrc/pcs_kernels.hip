@@ -277,18 +277,18 @@ __device__ __forceinline__ void stage_flush_binned(Stage& st, const BinSpec& bs)
 }
 
 // Two-word records: (k, DE bits) of the track-list form (BIN = false) or the
-// raw (k, de bits) pairs that stage_flush_binned bins (BIN = true).
+// raw (k, de bits) pairs that stage_flush_binned bins (BIN = true), one per
+// lane of the non-zero wave mask m. The mask is the store predicate as it is,
+// so no per-lane flag is rebuilt from it.
 template <bool BIN>
-__device__ __forceinline__ void stage_push2(Stage& st, bool ok, uint32_t a, uint32_t b, const BinSpec& bs)
+__device__ __forceinline__ void stage_push2(Stage& st, uint64_t m, uint32_t a, uint32_t b, const BinSpec& bs)
 {
-    const uint64_t m = ballot(ok);
-    if (m == 0ull) return;
     if (st.cnt + (uint32_t)(2 * kWave) > st.cap) {
         if (BIN) stage_flush_binned(st, bs);
         else stage_flush(st);
     }
     const uint32_t pos = st.cnt + 2u * mbcnt64(m);
-    if (ok) {
+    if (__builtin_amdgcn_inverse_ballot_w64(m)) {
         st.buf[pos] = a;
         st.buf[pos + 1u] = b;
     }
@@ -486,7 +486,7 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                     const uint32_t k = (uint32_t)(f1 - q.ts);
                     float v = track_de(q);
                     if (TRACKS && closing) v = v / ((float)(k - 1u) * four_dt);
-                    stage_push2<!TRACKS>(st, closing, k, __float_as_uint(v), bs);
+                    stage_push2<!TRACKS>(st, m_closing, k, __float_as_uint(v), bs);
                 }
                 // nx^2 + ny^2 of this step is pair s's r^2, the log's result
                 const float de_n = q.de + r2[i][s];
@@ -508,7 +508,7 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
             const uint32_t k = (uint32_t)(1 - pt[i].ts);
             float v = track_de(pt[i]);
             if (TRACKS && closing) v = v / ((float)(k - 1u) * four_dt);
-            stage_push2<!TRACKS>(st, closing, k, __float_as_uint(v), bs);
+            stage_push2<!TRACKS>(st, m_closing, k, __float_as_uint(v), bs);
         }
     }
     if (TRACKS) stage_flush(st);
