@@ -477,16 +477,20 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                 // formed from the masks of the single comparisons (no per-lane
                 // boolean is materialised). This step reaches frame s + 1; the
                 // open track ends at frame s with k = s + 1 - ts, at least kmin
-                // iff ts <= s + 1 - kmin.
+                // iff ts <= s + 1 - kmin. The length test runs only in steps
+                // where some lane's track ends (about a quarter of them).
                 const int32_t f1 = (int32_t)s + 1;
                 const uint64_t m_det = ballot(cx) & ballot(cy) & ballot(cz);
-                const uint64_t m_closing = m_active[i] & m_prev[i] & ~m_det & ballot(q.ts <= f1 - (int32_t)kmin);
-                if (m_closing != 0ull) {
-                    const bool closing = __builtin_amdgcn_inverse_ballot_w64(m_closing);
-                    const uint32_t k = (uint32_t)(f1 - q.ts);
-                    float v = track_de(q);
-                    if (TRACKS && closing) v = v / ((float)(k - 1u) * four_dt);
-                    stage_push2<!TRACKS>(st, m_closing, k, __float_as_uint(v), bs);
+                const uint64_t m_end = m_active[i] & m_prev[i] & ~m_det;
+                if (m_end != 0ull) {
+                    const uint64_t m_closing = m_end & ballot(q.ts <= f1 - (int32_t)kmin);
+                    if (m_closing != 0ull) {
+                        const bool closing = __builtin_amdgcn_inverse_ballot_w64(m_closing);
+                        const uint32_t k = (uint32_t)(f1 - q.ts);
+                        float v = track_de(q);
+                        if (TRACKS && closing) v = v / ((float)(k - 1u) * four_dt);
+                        stage_push2<!TRACKS>(st, m_closing, k, __float_as_uint(v), bs);
+                    }
                 }
                 // nx^2 + ny^2 of this step is pair s's r^2, the log's result
                 const float de_n = q.de + r2[i][s];
