@@ -268,8 +268,8 @@ def main():
                          "the other z boundary on the same proposals unless --no-other-z")
     ap.add_argument("--no-other-z", action="store_true")
     ap.add_argument("--tuning", default="",
-                    help="launch shape 'particles_per_lane,waves_per_group,groups_per_cu' instead of the autotuner "
-                         "(profiling runs: no autotuner launches among the profiled ones)")
+                    help="launch shape 'particles_per_lane,waves_per_group,groups_per_cu[,stage_records]' instead "
+                         "of the autotuner (profiling runs: no autotuner launches among the profiled ones)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the batched leg, copy 256 MiB device to device (pcs_selftest_copy): its known "
                          "traffic calibrates rocprofv3's FETCH_SIZE/WRITE_SIZE counters")
@@ -293,8 +293,9 @@ def main():
     steps_per_gen = particle_steps(frames, parts)
     tuning = None
     if args.tuning:
-        ppl, wpg, gpc = (int(x) for x in args.tuning.split(","))
-        ctx.set_tuning(ppl, wpg, gpc, 256)
+        shape = [int(x) for x in args.tuning.split(",")]
+        ppl, wpg, gpc = shape[:3]
+        ctx.set_tuning(ppl, wpg, gpc, shape[3] if len(shape) > 3 else 256)
         tuning = ctx.get_tuning()
         log(f"tuning: {tuning}")
     elif not args.no_autotune:

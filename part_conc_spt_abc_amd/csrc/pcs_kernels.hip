@@ -280,10 +280,13 @@ __device__ __forceinline__ void stage_flush_binned(Stage& st, const BinSpec& bs)
 // raw (k, de bits) pairs that stage_flush_binned bins (BIN = true), one per
 // lane of the non-zero wave mask m. The mask is the store predicate as it is,
 // so no per-lane flag is rebuilt from it.
+// The stage is flushed only when these records do not fit, so it fills up
+// whatever its size (at least 2 x 64 words).
 template <bool BIN>
 __device__ __forceinline__ void stage_push2(Stage& st, uint64_t m, uint32_t a, uint32_t b, const BinSpec& bs)
 {
-    if (st.cnt + (uint32_t)(2 * kWave) > st.cap) {
+    const uint32_t need = 2u * (uint32_t)__builtin_popcountll(m);
+    if (st.cnt + need > st.cap) {
         if (BIN) stage_flush_binned(st, bs);
         else stage_flush(st);
     }
@@ -292,7 +295,7 @@ __device__ __forceinline__ void stage_push2(Stage& st, uint64_t m, uint32_t a, u
         st.buf[pos] = a;
         st.buf[pos + 1u] = b;
     }
-    st.cnt += 2u * (uint32_t)__builtin_popcountll(m);
+    st.cnt += need;
 }
 
 // ------------------------------------------------------------------- walk
