@@ -70,6 +70,12 @@ extern "C" {
 #define PCS_MODE_BATCHED 1  /* one persistent launch per batch in a hipGraph */
 #define PCS_MODE_BATCHED_DIRECT 2 /* the batched kernels launched without the graph,
                                      with events between them (kernel timing) */
+#define PCS_MODE_PLAIN_STREAMS 3  /* the plain form's launch pairs spread round-robin
+                                     over PCS_PLAIN_STREAMS streams of one priority:
+                                     the reference's job pool keeps one pair in flight
+                                     per queue (src/gpu/main.cpp:45 "1/30",
+                                     src/gpu/detail/simulation_impl.inl:686-710, 889-901) */
+#define PCS_PLAIN_STREAMS 4       /* hardware queues a process gets by default */
 
 /* Model shared by all simulations of one ABC run
  * (SimulateSetup, src/gpu/cusim/kernel-simulate.cuh:13-38, filled at
@@ -272,6 +278,7 @@ typedef struct pcs_completion {
     int32_t status;          /* PCS_OK or error */
     double submit_us;        /* host clock at submit (us since server start) */
     double done_us;          /* host clock when the result was observed */
+    uint64_t exact;          /* exact integer distance sum (saturated at 2^64-1) */
 } pcs_completion;
 
 int pcs_server_create(int device, const pcs_server_config* cfg, pcs_server** out);
@@ -283,6 +290,16 @@ int pcs_server_submit(pcs_server* s, const pcs_request* req);
 int pcs_server_poll(pcs_server* s, pcs_completion* out, uint32_t max, uint32_t* n_out, uint32_t wait_us);
 /* Requests submitted and not yet returned by poll. */
 int pcs_server_pending(pcs_server* s, uint32_t* n);
+/* Scale of the record slice estimate of requests submitted from now on
+ * (default 1.0; see pcs_set_record_capacity_factor). A request whose slice
+ * overflows is rerun with a slice of exactly its size before it completes. */
+int pcs_server_set_record_capacity_factor(pcs_server* s, double factor);
+/* Requests rerun so far after a record slice overflow (informational). */
+int pcs_server_overflow_reruns(pcs_server* s, uint64_t* n);
+/* The server's clock (us since server start; CLOCK_MONOTONIC), the clock of
+ * submit_us and done_us: lets a load generator stamp its own scheduled
+ * arrival times in the same frame. */
+int pcs_server_now_us(pcs_server* s, double* us);
 int pcs_server_destroy(pcs_server* s);
 
 #ifdef __cplusplus

@@ -11,6 +11,7 @@ from ._abi import (  # noqa: F401
     PCS_MODE_BATCHED,
     PCS_MODE_BATCHED_DIRECT,
     PCS_MODE_PLAIN,
+    PCS_MODE_PLAIN_STREAMS,
     PCS_Z_PERIODIC,
     PCS_Z_REFLECT,
     PcsError,

@@ -31,6 +31,8 @@ PCS_DIST_L1 = 1
 PCS_MODE_PLAIN = 0
 PCS_MODE_BATCHED = 1
 PCS_MODE_BATCHED_DIRECT = 2
+PCS_MODE_PLAIN_STREAMS = 3
+PCS_PLAIN_STREAMS = 4
 PCS_SERVE_DEVICE_LOOP = 1
 
 F8 = C.c_float * PCS_MAX_SPECIES
@@ -104,6 +106,7 @@ class pcs_completion(C.Structure):
         ("status", C.c_int32),
         ("submit_us", C.c_double),
         ("done_us", C.c_double),
+        ("exact", C.c_uint64),
     ]
 
 
@@ -147,6 +150,9 @@ PROTOTYPES = {
     "pcs_server_submit": (C.c_int, [P, C.POINTER(pcs_request)]),
     "pcs_server_poll": (C.c_int, [P, C.POINTER(pcs_completion), C.c_uint32, U32P, C.c_uint32]),
     "pcs_server_pending": (C.c_int, [P, U32P]),
+    "pcs_server_now_us": (C.c_int, [P, F64P]),
+    "pcs_server_set_record_capacity_factor": (C.c_int, [P, C.c_double]),
+    "pcs_server_overflow_reruns": (C.c_int, [P, U64P]),
     "pcs_server_destroy": (C.c_int, [P]),
 }
 

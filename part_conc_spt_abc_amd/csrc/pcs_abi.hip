@@ -114,31 +114,10 @@ DevSys make_devsys(const pcs_system& s, uint32_t maxrow, const uint32_t* dsat)
     return d;
 }
 
-// Record slice size for one simulation: every recorded track spans at least
-// kmin detected frames, so tracks <= detected particle-frames / kmin. The
-// expected detected fraction is the detection volume over the box volume;
-// `factor` scales that bound (calibrated, see tests) and a small floor
-// absorbs fluctuations. A slice that still overflows is rerun exactly.
 uint32_t record_capacity(const pcs_system& s, const pcs_proposal& p, const uint32_t* frames,
                          const uint32_t* particles, double factor)
 {
-    double fxy = std::min(1.0, (double)s.half_a[0] / s.half_l[0]) * std::min(1.0, (double)s.half_a[1] / s.half_l[1]);
-    double fz = 0.0;
-    if (s.detect_model == PCS_DETECT_GAUSS_Z) {
-        fz = std::min(1.0, (double)s.detect_sigma * std::sqrt(2.0 * M_PI) / (2.0 * s.half_l[2]));
-    } else {
-        for (uint32_t c = 0; c < s.n_species; ++c) fz = std::max(fz, std::min(1.0, (double)p.half_az[c] / s.half_l[2]));
-    }
-    double pf = 0.0, maxbound = 0.0;
-    for (uint32_t v = 0; v < p.n_videos; ++v) {
-        pf += (double)particles[v] * (double)frames[v];
-        maxbound += (double)particles[v] * (double)((frames[v] + 1u) / (s.kmin + 1u));
-    }
-    const double e = pf * fxy * fz / (double)s.kmin;
-    double cap = factor * e + 4.0 * std::sqrt(e) + 64.0;
-    cap = std::min(cap, maxbound + 1.0);
-    cap = std::min(cap, 4.0e9);
-    return (uint32_t)std::ceil(cap);
+    return slice_capacity(s, p.half_az, p.n_videos, frames, particles, factor);
 }
 
 }  // namespace pcs
@@ -187,6 +166,10 @@ struct pcs_batch {
     hipGraphExec_t gexec = nullptr;
     pcs_tuning graph_tune{};
     bool graph_events = false;
+    // plain form over several streams (PCS_MODE_PLAIN_STREAMS), made on first use
+    hipStream_t pstream[PCS_PLAIN_STREAMS] = {};
+    hipEvent_t pfork = nullptr;
+    hipEvent_t pjoin[PCS_PLAIN_STREAMS] = {};
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int last_mode = -1;
@@ -318,22 +301,50 @@ int enqueue_batched(pcs_batch* b, bool with_events)
     return PCS_OK;
 }
 
+// Streams (same priority as the batch stream) and fork/join events of the
+// multi-stream plain form.
+int plain_streams(pcs_batch* b)
+{
+    if (b->pfork) return PCS_OK;
+    for (int k = 0; k < PCS_PLAIN_STREAMS; ++k) {
+        PCS_HIP(hipStreamCreateWithFlags(&b->pstream[k], hipStreamNonBlocking));
+        PCS_HIP(hipEventCreateWithFlags(&b->pjoin[k], hipEventDisableTiming));
+    }
+    PCS_HIP(hipEventCreateWithFlags(&b->pfork, hipEventDisableTiming));
+    return PCS_OK;
+}
+
 // Plain form: per simulation, one walk launch over its tiles and one
-// distance launch (the reference's per-job pair, src/gpu/detail/simulation_impl.inl:1064-1097).
-int enqueue_plain(pcs_batch* b, const std::vector<uint32_t>* subset)
+// distance launch (the reference's per-job pair, src/gpu/detail/simulation_impl.inl:1064-1097),
+// on the batch stream or, with `streams`, round-robin over PCS_PLAIN_STREAMS
+// streams that fork from the batch stream and join it again (the reference's
+// job pool keeps one pair in flight per queue, simulation_impl.inl:686-710).
+int enqueue_plain(pcs_batch* b, const std::vector<uint32_t>* subset, bool streams = false)
 {
     pcs_ctx* c = b->ctx;
     const pcs_tuning& t = c->tune;
     const bool gauss = c->sys.detect_model == PCS_DETECT_GAUSS_Z;
     const bool zref = c->sys.z_boundary == PCS_Z_REFLECT;
     const uint32_t n = subset ? (uint32_t)subset->size() : b->n_props;
+    if (streams) {
+        PCS_TRY(plain_streams(b));
+        PCS_HIP(hipEventRecord(b->pfork, b->stream));
+        for (int k = 0; k < PCS_PLAIN_STREAMS; ++k) PCS_HIP(hipStreamWaitEvent(b->pstream[k], b->pfork, 0));
+    }
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t P = subset ? (*subset)[i] : i;
         const pcs_proposal& p = b->props[P];
         const uint32_t lo = b->vtile[p.first_video], hi = b->vtile[p.first_video + p.n_videos];
+        hipStream_t s = streams ? b->pstream[i % PCS_PLAIN_STREAMS] : b->stream;
         PCS_HIP(launch_walk(walk_args(b, lo, hi), (int)t.particles_per_lane, (int)t.waves_per_group, false, 0, gauss,
-                            zref, b->stream));
-        PCS_HIP(launch_distance(dist_args(b, P, 1), c->sys.n_k, c->sys.n_de, b->stream));
+                            zref, s));
+        PCS_HIP(launch_distance(dist_args(b, P, 1), c->sys.n_k, c->sys.n_de, s));
+    }
+    if (streams) {
+        for (int k = 0; k < PCS_PLAIN_STREAMS; ++k) {
+            PCS_HIP(hipEventRecord(b->pjoin[k], b->pstream[k]));
+            PCS_HIP(hipStreamWaitEvent(b->stream, b->pjoin[k], 0));
+        }
     }
     return PCS_OK;
 }
@@ -574,6 +585,8 @@ int pcs_batch_destroy(pcs_batch* b)
     if (!b) return PCS_OK;
     hipSetDevice(b->ctx->device);
     if (b->stream) hipStreamSynchronize(b->stream);
+    for (hipStream_t ps : b->pstream)
+        if (ps) hipStreamSynchronize(ps);
     destroy_graph(b);
     void* ptrs[] = {b->d_props, b->d_vframes, b->d_vparticles, b->d_vprop, b->d_vtile, b->d_rec, b->d_srt,
                     b->d_rec2, b->d_srt2, b->d_count, b->d_tracks, b->d_tile_counter, b->d_dist, b->d_exact, b->d_status, b->d_seed};
@@ -582,6 +595,11 @@ int pcs_batch_destroy(pcs_batch* b)
     if (b->h_seed) hipHostFree(b->h_seed);
     for (hipEvent_t& e : b->ev)
         if (e) hipEventDestroy(e);
+    for (int k = 0; k < PCS_PLAIN_STREAMS; ++k) {
+        if (b->pstream[k]) hipStreamDestroy(b->pstream[k]);
+        if (b->pjoin[k]) hipEventDestroy(b->pjoin[k]);
+    }
+    if (b->pfork) hipEventDestroy(b->pfork);
     if (b->stream) hipStreamDestroy(b->stream);
     pcs_ctx* c = b->ctx;
     delete b;
@@ -686,7 +704,8 @@ int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const ui
 int pcs_batch_run(pcs_batch* b, uint64_t seed, int mode)
 {
     if (!b) return set_error(PCS_EINVAL, "batch is NULL");
-    if (mode != PCS_MODE_PLAIN && mode != PCS_MODE_BATCHED && mode != PCS_MODE_BATCHED_DIRECT)
+    if (mode != PCS_MODE_PLAIN && mode != PCS_MODE_BATCHED && mode != PCS_MODE_BATCHED_DIRECT &&
+        mode != PCS_MODE_PLAIN_STREAMS)
         return set_error(PCS_EINVAL, "mode %d unknown", mode);
     pcs_ctx* c = b->ctx;
     PCS_HIP(hipSetDevice(c->device));
@@ -712,7 +731,7 @@ int pcs_batch_run(pcs_batch* b, uint64_t seed, int mode)
     } else {
         PCS_HIP(hipEventRecord(b->ev[0], b->stream));
         PCS_HIP(launch_reset(b->d_count, b->n_props, b->d_tile_counter, b->stream));
-        PCS_TRY(enqueue_plain(b, nullptr));
+        PCS_TRY(enqueue_plain(b, nullptr, mode == PCS_MODE_PLAIN_STREAMS));
         PCS_HIP(hipEventRecord(b->ev[3], b->stream));
     }
     b->last_mode = mode;

@@ -44,6 +44,41 @@ struct DevProp {
 };
 static_assert(sizeof(DevProp) == 144, "DevProp layout");
 
+// Record slice size for one simulation (host and device): every recorded
+// track spans at least kmin detected frames, so tracks <= detected
+// particle-frames / kmin. The expected detected fraction is the detection
+// volume over the box volume; `factor` scales that bound and a small floor
+// absorbs fluctuations. A slice that still overflows is rerun exactly.
+// Sys is pcs_system or DevSys (same field names).
+template <typename Sys>
+__host__ __device__ inline uint32_t slice_capacity(const Sys& s, const float* half_az, uint32_t n_videos,
+                                                   const uint32_t* frames, const uint32_t* particles, double factor)
+{
+    const double rx = (double)s.half_a[0] / s.half_l[0], ry = (double)s.half_a[1] / s.half_l[1];
+    const double fxy = (rx < 1.0 ? rx : 1.0) * (ry < 1.0 ? ry : 1.0);
+    double fz = 0.0;
+    if (s.detect_model == PCS_DETECT_GAUSS_Z) {
+        const double g = (double)s.detect_sigma * 2.5066282746310002 / (2.0 * s.half_l[2]);  // sigma sqrt(2 pi) / Lz
+        fz = g < 1.0 ? g : 1.0;
+    } else {
+        for (uint32_t c = 0; c < s.n_species; ++c) {
+            const double r = (double)half_az[c] / s.half_l[2];
+            const double rz = r < 1.0 ? r : 1.0;
+            fz = rz > fz ? rz : fz;
+        }
+    }
+    double pf = 0.0, maxbound = 0.0;
+    for (uint32_t v = 0; v < n_videos; ++v) {
+        pf += (double)particles[v] * (double)frames[v];
+        maxbound += (double)particles[v] * (double)((frames[v] + 1u) / (s.kmin + 1u));
+    }
+    const double e = pf * fxy * fz / (double)s.kmin;
+    double cap = factor * e + 4.0 * sqrt(e) + 64.0;
+    cap = cap < maxbound + 1.0 ? cap : maxbound + 1.0;
+    cap = cap < 4.0e9 ? cap : 4.0e9;
+    return (uint32_t)ceil(cap);
+}
+
 // Arguments of the walk kernel. Videos are the unit of the work plan: video v
 // belongs to proposal vprop[v], has vframes[v] frames and vparticles[v]
 // particles, and owns tiles [vtile[v], vtile[v+1]) of 64 * PPL particles.

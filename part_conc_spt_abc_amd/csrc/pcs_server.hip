@@ -25,6 +25,7 @@
 // result is the same whatever it shares the GPU with.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -94,26 +95,37 @@ private:
     std::map<uint64_t, uint64_t> free_;
 };
 
+// Model classes that need their own walk kernel: bit 1 = Gaussian detection,
+// bit 0 = reflecting z.
+constexpr int kClasses = 4;
+
+uint32_t model_class(const pcs_system& sy)
+{
+    return (sy.detect_model == PCS_DETECT_GAUSS_Z ? 2u : 0u) + (sy.z_boundary == PCS_Z_REFLECT ? 1u : 0u);
+}
+
 struct Group {
     bool busy = false;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    // device tables of this group
+    // device tables of this group; one tile table per model class, so the
+    // walks of every class present are issued back to back
     DevProp* d_props = nullptr;
-    uint32_t *d_vframes = nullptr, *d_vparticles = nullptr, *d_vprop = nullptr, *d_vtile = nullptr;
+    uint32_t *d_vframes = nullptr, *d_vparticles = nullptr, *d_vprop = nullptr;
+    uint32_t* d_vtile[kClasses] = {};
     uint32_t* d_count = nullptr;
-    uint64_t* d_exact = nullptr;
     // pinned host staging (uploads) and results (written by the distance kernel)
     DevProp* h_props = nullptr;
-    uint32_t *h_vframes = nullptr, *h_vparticles = nullptr, *h_vprop = nullptr, *h_vtile = nullptr;
+    uint32_t *h_vframes = nullptr, *h_vparticles = nullptr, *h_vprop = nullptr;
+    uint32_t* h_vtile[kClasses] = {};
     double* h_dist = nullptr;
+    uint64_t* h_exact = nullptr;
     uint32_t* h_tracks = nullptr;
     int32_t* h_status = nullptr;
-    // bookkeeping
-    std::vector<uint64_t> tags;
-    std::vector<double> submit_us;
+    // bookkeeping: the requests of the group (kept for exact reruns of
+    // overflowed record slices) and their arena ranges
+    std::vector<Pending> reqs;
     std::vector<uint64_t> arena_off;
-    std::vector<uint32_t> arena_n;
     uint32_t n = 0;
 };
 
@@ -139,9 +151,8 @@ struct LoopHost {
     int n_cu = 0;
     struct Book {
         bool busy = false;
-        uint64_t r = 0, tag = 0, off = 0;
-        uint32_t n = 0;
-        double submit_us = 0.0;
+        uint64_t r = 0, off = 0;
+        Pending req;  // kept for an exact rerun of an overflowed record slice
     };
     std::vector<Book> book;         // per slot
     std::vector<uint32_t> live;     // slots in flight
@@ -163,6 +174,8 @@ struct pcs_server {
     LoopHost loop;
     uint32_t inflight = 0;
     uint32_t max_queue = 0;
+    double cap_factor = 1.0;  // record slice estimate scale (pcs_server_set_record_capacity_factor)
+    uint64_t reruns = 0;      // requests rerun after a record slice overflow
 
     std::mutex mu;
     std::condition_variable cv;
@@ -195,15 +208,15 @@ hipError_t group_alloc(Group& g, uint32_t max_props, uint32_t max_videos)
     if (e == hipSuccess) e = dmalloc(&g.d_vframes, max_videos);
     if (e == hipSuccess) e = dmalloc(&g.d_vparticles, max_videos);
     if (e == hipSuccess) e = dmalloc(&g.d_vprop, max_videos);
-    if (e == hipSuccess) e = dmalloc(&g.d_vtile, max_videos + 1);
+    for (int k = 0; k < kClasses && e == hipSuccess; ++k) e = dmalloc(&g.d_vtile[k], max_videos + 1);
     if (e == hipSuccess) e = dmalloc(&g.d_count, max_props);
-    if (e == hipSuccess) e = dmalloc(&g.d_exact, max_props);
     if (e == hipSuccess) e = hmalloc(&g.h_props, max_props);
     if (e == hipSuccess) e = hmalloc(&g.h_vframes, max_videos);
     if (e == hipSuccess) e = hmalloc(&g.h_vparticles, max_videos);
     if (e == hipSuccess) e = hmalloc(&g.h_vprop, max_videos);
-    if (e == hipSuccess) e = hmalloc(&g.h_vtile, max_videos + 1);
+    for (int k = 0; k < kClasses && e == hipSuccess; ++k) e = hmalloc(&g.h_vtile[k], max_videos + 1);
     if (e == hipSuccess) e = hmalloc(&g.h_dist, max_props);
+    if (e == hipSuccess) e = hmalloc(&g.h_exact, max_props);
     if (e == hipSuccess) e = hmalloc(&g.h_tracks, max_props);
     if (e == hipSuccess) e = hmalloc(&g.h_status, max_props);
     return e;
@@ -212,12 +225,16 @@ hipError_t group_alloc(Group& g, uint32_t max_props, uint32_t max_videos)
 void group_free(Group& g)
 {
     if (g.stream) hipStreamSynchronize(g.stream);
-    void* dev[] = {g.d_props, g.d_vframes, g.d_vparticles, g.d_vprop, g.d_vtile, g.d_count, g.d_exact};
+    void* dev[] = {g.d_props, g.d_vframes, g.d_vparticles, g.d_vprop, g.d_count};
     for (void* p : dev)
         if (p) hipFree(p);
-    void* host[] = {g.h_props, g.h_vframes, g.h_vparticles, g.h_vprop, g.h_vtile, g.h_dist, g.h_tracks, g.h_status};
+    void* host[] = {g.h_props, g.h_vframes, g.h_vparticles, g.h_vprop, g.h_dist, g.h_exact, g.h_tracks, g.h_status};
     for (void* p : host)
         if (p) hipHostFree(p);
+    for (int k = 0; k < kClasses; ++k) {
+        if (g.d_vtile[k]) hipFree(g.d_vtile[k]);
+        if (g.h_vtile[k]) hipHostFree(g.h_vtile[k]);
+    }
     if (g.done) hipEventDestroy(g.done);
     if (g.stream) hipStreamDestroy(g.stream);
     g = Group{};
@@ -246,33 +263,36 @@ DevProp make_prop(const pcs_server* s, const Pending& p, uint32_t first_video, u
     return d;
 }
 
-// Issue one launch group on g from the admitted requests.
+// Issue one launch group on g from the admitted requests (moved into g).
 int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vector<uint64_t>& offs)
 {
     const uint32_t n = (uint32_t)batch.size();
     uint32_t nv = 0;
-    g.tags.resize(n);
-    g.submit_us.resize(n);
     g.arena_off.assign(offs.begin(), offs.end());
-    g.arena_n.resize(n);
     uint32_t n_de_max = 1, n_k_max = 1;
-    uint32_t classes = 0;  // bit (gauss*2 + zref) present
+    uint32_t classes = 0;  // bit model_class() present
+    uint32_t tiles[kClasses] = {0, 0, 0, 0};
     for (uint32_t i = 0; i < n; ++i) {
         const Pending& p = batch[i];
         const pcs_system& sy = s->systems[p.req.sys_id].sys;
+        const uint32_t cls = model_class(sy);
         g.h_props[i] = make_prop(s, p, nv, offs[i]);
         for (uint32_t v = 0; v < p.req.n_videos; ++v, ++nv) {
             g.h_vframes[nv] = p.frames[v];
             g.h_vparticles[nv] = p.particles[v];
             g.h_vprop[nv] = i;
+            // tile tables: the videos of other classes have no tiles in this one
+            for (uint32_t k = 0; k < (uint32_t)kClasses; ++k) {
+                g.h_vtile[k][nv] = tiles[k];
+                if (k == cls) tiles[k] += (p.particles[v] + 63u) / 64u;
+            }
         }
-        g.tags[i] = p.req.tag;
-        g.submit_us[i] = p.submit_us;
-        g.arena_n[i] = p.cap;
         n_de_max = std::max(n_de_max, sy.n_de);
         n_k_max = std::max(n_k_max, sy.n_k);
-        classes |= 1u << ((sy.detect_model == PCS_DETECT_GAUSS_Z ? 2 : 0) + (sy.z_boundary == PCS_Z_REFLECT ? 1 : 0));
+        classes |= 1u << cls;
     }
+    g.reqs.clear();
+    for (Pending& p : batch) g.reqs.push_back(std::move(p));
     g.n = n;
     hipStream_t st = g.stream;
     hipError_t e = hipMemcpyAsync(g.d_props, g.h_props, n * sizeof(DevProp), hipMemcpyHostToDevice, st);
@@ -280,19 +300,12 @@ int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vect
     if (e == hipSuccess) e = hipMemcpyAsync(g.d_vparticles, g.h_vparticles, nv * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g.d_vprop, g.h_vprop, nv * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_reset(g.d_count, n, nullptr, st);
-    // One walk launch per (detection, boundary) class present: tiles of the
-    // other classes are planned empty for it (zero-tile videos).
-    for (uint32_t cls = 0; cls < 4 && e == hipSuccess; ++cls) {
+    // One walk launch per (detection, boundary) class present, each over its
+    // own tile table.
+    for (uint32_t cls = 0; cls < (uint32_t)kClasses && e == hipSuccess; ++cls) {
         if (!(classes & (1u << cls))) continue;
-        uint32_t acc = 0;
-        for (uint32_t v = 0; v < nv; ++v) {
-            g.h_vtile[v] = acc;
-            const pcs_system& sy = s->systems[g.h_props[g.h_vprop[v]].sys_id].sys;
-            const uint32_t vc = (sy.detect_model == PCS_DETECT_GAUSS_Z ? 2u : 0u) + (sy.z_boundary == PCS_Z_REFLECT ? 1u : 0u);
-            if (vc == cls) acc += (g.h_vparticles[v] + 63u) / 64u;
-        }
-        g.h_vtile[nv] = acc;
-        e = hipMemcpyAsync(g.d_vtile, g.h_vtile, (nv + 1) * 4, hipMemcpyHostToDevice, st);
+        g.h_vtile[cls][nv] = tiles[cls];
+        e = hipMemcpyAsync(g.d_vtile[cls], g.h_vtile[cls], (nv + 1) * 4, hipMemcpyHostToDevice, st);
         if (e != hipSuccess) break;
         WalkArgs a{};
         a.sys = s->d_sys;
@@ -300,17 +313,15 @@ int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vect
         a.vframes = g.d_vframes;
         a.vparticles = g.d_vparticles;
         a.vprop = g.d_vprop;
-        a.vtile = g.d_vtile;
+        a.vtile = g.d_vtile[cls];
         a.n_videos = nv;
         a.tile_lo = 0;
-        a.tile_hi = acc;
+        a.tile_hi = tiles[cls];
         a.stage_cap = 256;
         a.rec_count = g.d_count;
         a.seed = s->d_zero_seed;
         a.tile_counter = nullptr;
         e = launch_walk(a, 1, 4, false, 0, (cls & 2u) != 0, (cls & 1u) != 0, st);
-        // the tile table is overwritten by the next class: wait for this walk
-        if (e == hipSuccess && (classes >> (cls + 1))) e = hipStreamSynchronize(st);
     }
     if (e == hipSuccess) {
         DistArgs d{};
@@ -320,7 +331,7 @@ int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vect
         d.n_props = n;
         d.rec_count = g.d_count;
         d.dist = g.h_dist;
-        d.exact = g.d_exact;
+        d.exact = g.h_exact;
         d.tracks = g.h_tracks;
         d.status = g.h_status;
         e = launch_distance(d, n_k_max, n_de_max, st);
@@ -331,23 +342,54 @@ int launch_group(pcs_server* s, Group& g, std::vector<Pending>& batch, std::vect
     return PCS_OK;
 }
 
+// A request whose record slice overflowed (status PCS_EOVERFLOW, tracks =
+// the exact number of records it produced) goes back to the front of the
+// queue with a slice of exactly that size: the random stream is keyed by the
+// request, so the rerun reproduces the same tracks. Returns false when the
+// exact slice cannot fit the arena (the overflow is then reported).
+bool requeue_overflow(pcs_server* s, Pending& p, uint32_t count, std::vector<Pending>& again)
+{
+    if (count == 0u || count > s->arena_records || count <= p.cap) return false;
+    p.cap = count;
+    again.push_back(std::move(p));
+    return true;
+}
+
+void push_back_reruns(pcs_server* s, std::vector<Pending>& again)
+{
+    if (again.empty()) return;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto it = again.rbegin(); it != again.rend(); ++it) s->queue.push_front(std::move(*it));
+    s->reruns += again.size();
+    again.clear();
+    s->cv.notify_all();
+}
+
 void retire_group(pcs_server* s, Group& g)
 {
     const double t = s->now_us();
-    std::vector<pcs_completion> out(g.n);
+    std::vector<pcs_completion> out;
+    std::vector<Pending> again;
+    out.reserve(g.n);
     for (uint32_t i = 0; i < g.n; ++i) {
-        pcs_completion& c = out[i];
-        c.tag = g.tags[i];
+        Pending& p = g.reqs[i];
+        s->arena.release(g.arena_off[i], p.cap);
+        if (g.h_status[i] == PCS_EOVERFLOW && requeue_overflow(s, p, g.h_tracks[i], again)) continue;
+        pcs_completion c{};
+        c.tag = p.req.tag;
         c.dist = g.h_dist[i];
+        c.exact = g.h_exact[i];
         c.tracks = g.h_tracks[i];
         c.status = g.h_status[i];
-        c.submit_us = g.submit_us[i];
+        c.submit_us = p.submit_us;
         c.done_us = t;
-        s->arena.release(g.arena_off[i], g.arena_n[i]);
+        out.push_back(c);
     }
     s->inflight -= g.n;
     g.busy = false;
     g.n = 0;
+    g.reqs.clear();
+    push_back_reruns(s, again);
     std::lock_guard<std::mutex> lk(s->mu);
     for (auto& c : out) s->done.push_back(c);
     s->cv.notify_all();
@@ -464,6 +506,7 @@ void loop_publish(pcs_server* s, uint32_t slot, uint64_t r, const Pending& p, ui
 uint32_t loop_collect(pcs_server* s, std::vector<pcs_completion>& out)
 {
     LoopHost& L = s->loop;
+    std::vector<Pending> again;
     out.clear();
     for (size_t i = 0; i < L.live.size();) {
         const uint32_t slot = L.live[i];
@@ -473,25 +516,29 @@ uint32_t loop_collect(pcs_server* s, std::vector<pcs_completion>& out)
             ++i;
             continue;
         }
-        pcs_completion c{};
-        c.tag = b.tag;
-        c.dist = d.dist;
-        c.tracks = d.tracks;
-        c.status = d.status;
-        c.submit_us = b.submit_us;
-        c.done_us = s->now_us();
-        out.push_back(c);
-        s->arena.release(b.off, b.n);
+        s->arena.release(b.off, b.req.cap);
         b.busy = false;
         L.live[i] = L.live.back();
         L.live.pop_back();
+        if (d.status == PCS_EOVERFLOW && requeue_overflow(s, b.req, d.tracks, again)) continue;
+        pcs_completion c{};
+        c.tag = b.req.req.tag;
+        c.dist = d.dist;
+        c.exact = d.exact;
+        c.tracks = d.tracks;
+        c.status = d.status;
+        c.submit_us = b.req.submit_us;
+        c.done_us = s->now_us();
+        out.push_back(c);
     }
+    const uint32_t n_again = (uint32_t)again.size();
+    push_back_reruns(s, again);
     if (!out.empty()) {
         std::lock_guard<std::mutex> lk(s->mu);
         for (auto& c : out) s->done.push_back(c);
         s->cv.notify_all();
     }
-    return (uint32_t)out.size();
+    return (uint32_t)out.size() + n_again;
 }
 
 // Launch the loop kernel for the first nsys systems (LDS and distance width
@@ -592,11 +639,9 @@ void scheduler_loop(pcs_server* s)
                 LoopHost::Book& b = L.book[slot];
                 b.busy = true;
                 b.r = L.next_r;
-                b.tag = p.req.tag;
                 b.off = off;
-                b.n = p.cap;
-                b.submit_us = p.submit_us;
                 loop_publish(s, slot, L.next_r, p, off);
+                b.req = std::move(p);
                 L.live.push_back(slot);
                 ++L.next_r;
                 s->queue.pop_front();
@@ -779,7 +824,7 @@ int pcs_server_submit(pcs_server* s, const pcs_request* req)
     if (req->prop.reserved != 0) return set_error(PCS_EINVAL, "reserved field must be 0");
     int r = validate_proposal(sy, req->prop);
     if (r != PCS_OK) return r;
-    p.cap = record_capacity(sy, req->prop, p.frames.data(), p.particles.data(), 1.0);
+    p.cap = record_capacity(sy, req->prop, p.frames.data(), p.particles.data(), s->cap_factor);
     if (s->loop.on && request_tiles(p) > kLoopMaxTiles)
         return set_error(PCS_EINVAL, "request needs %u tiles of 64 particles, the loop takes %u", request_tiles(p),
                          kLoopMaxTiles);
@@ -816,6 +861,29 @@ int pcs_server_pending(pcs_server* s, uint32_t* n)
     if (!s || !n) return set_error(PCS_EINVAL, "NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     *n = s->outstanding;
+    return PCS_OK;
+}
+
+int pcs_server_set_record_capacity_factor(pcs_server* s, double factor)
+{
+    if (!s || !(factor > 0.0) || !std::isfinite(factor)) return set_error(PCS_EINVAL, "factor must be positive");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cap_factor = factor;
+    return PCS_OK;
+}
+
+int pcs_server_overflow_reruns(pcs_server* s, uint64_t* n)
+{
+    if (!s || !n) return set_error(PCS_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *n = s->reruns;
+    return PCS_OK;
+}
+
+int pcs_server_now_us(pcs_server* s, double* us)
+{
+    if (!s || !us) return set_error(PCS_EINVAL, "NULL argument");
+    *us = s->now_us();
     return PCS_OK;
 }
 

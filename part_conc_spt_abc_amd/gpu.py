@@ -236,6 +236,20 @@ class Server:
         _abi.check(self.lib.pcs_server_pending(self.h, C.byref(n)), "pcs_server_pending")
         return int(n.value)
 
+    def set_record_capacity_factor(self, f: float):
+        _abi.check(self.lib.pcs_server_set_record_capacity_factor(self.h, f), "pcs_server_set_record_capacity_factor")
+
+    def overflow_reruns(self) -> int:
+        n = C.c_uint64(0)
+        _abi.check(self.lib.pcs_server_overflow_reruns(self.h, C.byref(n)), "pcs_server_overflow_reruns")
+        return int(n.value)
+
+    def now_us(self) -> float:
+        """The server's clock (us since creation), the clock of submit_us/done_us."""
+        t = C.c_double(0.0)
+        _abi.check(self.lib.pcs_server_now_us(self.h, C.byref(t)), "pcs_server_now_us")
+        return float(t.value)
+
     def close(self):
         if self.h:
             self.lib.pcs_server_destroy(self.h)

@@ -4,10 +4,20 @@
 (orc_distance, restating src/cpu/distance.jl). Test infrastructure: it is
 the CPU side of the posterior comparison, never part of the product.
 
+Where the reference's two estimators differ this follows the Julia one: a
+fixed depth is C identical component rows, each displaced member keeps them
+equal, and the pmc-standard weight multiplies the depth kernel once per row
+(estimate.jl:60-61, 143-144, 199-203; the C++ driver multiplies it once,
+src/gpu/detail/simulation_impl.inl:849-852); the random member is the first
+whose cumulative weight reaches u (rand_weighted_index.jl); displacement
+reflects into the open interval (displace.jl:5).
+
 Differences from the Julia code, all deliberate: the observed and simulated
 histograms bin with trunc (as the GPU reference) instead of ceil; the
 adaptive-gamma pass simulates 10^log10c (estimate.jl:87-100 passes log10c);
-the random streams are numpy's and the oracle's Philox.
+the random streams are numpy's and the oracle's Philox; the simulation is
+the oracle's FP64 walk (orc_simulate_f64) rather than simulate.jl's own
+stream.
 """
 from __future__ import annotations
 
@@ -128,11 +138,10 @@ class CpuAbc:
                     for i in range(C):
                         term *= normpdf(m_s[i, j] - m[i], 0.0, tm[i])
                         term *= normpdf(lc_s[i, j] - lc[i], 0.0, tc[i])
-                    if self.fixed:
-                        term *= normpdf(az_s[0, j] - az[0], 0.0, ta[0])
-                    else:
-                        for i in range(C):
-                            term *= normpdf(az_s[i, j] - az[i], 0.0, ta[i])
+                    # one depth kernel per component row, also for a fixed
+                    # depth, whose C rows are equal (estimate.jl:199-203)
+                    for i in range(C):
+                        term *= normpdf(az_s[i, j] - az[i], 0.0, ta[i])
                     ws[j] = 1.0 / (w * term).sum()
                 w = ws / ws.sum()
             else:

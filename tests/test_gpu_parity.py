@@ -85,7 +85,8 @@ def observed(model, seed=0, particles=1200, frames=None):
     return H
 
 
-@pytest.mark.parametrize("mode", [_abi.PCS_MODE_PLAIN, _abi.PCS_MODE_BATCHED, _abi.PCS_MODE_BATCHED_DIRECT])
+@pytest.mark.parametrize("mode", [_abi.PCS_MODE_PLAIN, _abi.PCS_MODE_BATCHED, _abi.PCS_MODE_BATCHED_DIRECT,
+                                  _abi.PCS_MODE_PLAIN_STREAMS])
 def test_headline_single_simulation(gpu_lib, mode):
     model = cases.cpu_model()  # 500 frames, H 500 x 200
     obs = observed(model)
@@ -124,7 +125,7 @@ def test_batched_mixed_sizes(gpu_lib):
     props, f, n = cases.draw_proposals(model, 24, rng, frames_choices=(200, 500, 1000, 2000), c_range=(1e8, 2e9))
     print("particles", n.tolist(), "frames", f.tolist(), flush=True)
     with Context(model, obs) as ctx:
-        for mode in (_abi.PCS_MODE_BATCHED, _abi.PCS_MODE_PLAIN):
+        for mode in (_abi.PCS_MODE_BATCHED, _abi.PCS_MODE_PLAIN, _abi.PCS_MODE_PLAIN_STREAMS):
             dist, tracks, H, exact, _ = run_gpu(ctx, props, f, n, 11, mode)
             check_against_oracle(model, obs, props, f, n, 11, dist, tracks, H, exact)
 

@@ -6,6 +6,11 @@ synthetic data and the same fixed tolerance schedule (gamma 9.95 - 0.05 g,
 available, so this CPU restatement stands in for the reference run. Means
 must agree within Monte-Carlo error (4 standard errors from the weights'
 effective sample sizes) and the 10-90 % credible intervals must overlap.
+The two weight the fixed depth differently, each as its reference does: the
+C++ driver multiplies one depth kernel into a pmc-standard weight
+(src/gpu/detail/simulation_impl.inl:849-852), the Julia restatement one per
+component row (src/cpu/estimate.jl:199-203); the test holds them to the
+same Monte-Carlo bound regardless.
 """
 import math
 import pathlib
