@@ -146,6 +146,28 @@ int pcs_simulate(pcs_ctx* ctx, const pcs_proposal* props, uint32_t n,
                  uint64_t seed, int mode, double* dist_out, uint32_t* hist_out,
                  uint32_t* tracks_out);
 
+/* Allocation-free variant of pcs_simulate (batched form, direct launches).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc). It
+ * calls no hipMalloc/hipFree: proposal records, tables, counters, results,
+ * the record arena and the sort scratch of a run live in one fixed device
+ * array of PCS_STATIC_SCRATCH_BYTES per GPU, reserved when the library is
+ * loaded and held until the process ends. Limits: n <= PCS_STATIC_MAX_SIMS,
+ * n_videos <= PCS_STATIC_MAX_VIDEOS, proposals with consecutive video ranges
+ * (props[i].first_video = sum of the earlier n_videos), and the planned
+ * record slices plus exact reruns of overflowed ones within the array
+ * (about 16 million records); beyond them it fails with PCS_EINVAL or
+ * PCS_EOVERFLOW. Every call on one GPU shares that GPU's array and leaves
+ * its contents behind: calls that overlap on one GPU (any context) race.
+ * Results equal pcs_simulate's. Blocks until done. tracks_out and exact_out
+ * (n uint64 exact distance sums) are nullable. */
+#define PCS_STATIC_SCRATCH_BYTES (128u << 20)
+#define PCS_STATIC_MAX_SIMS 8192u
+#define PCS_STATIC_MAX_VIDEOS 65536u
+int pcs_simulate_static(pcs_ctx* ctx, const pcs_proposal* props, uint32_t n,
+                        const uint32_t* video_frames, const uint32_t* video_particles, uint32_t n_videos,
+                        uint64_t seed, double* dist_out, uint32_t* tracks_out, uint64_t* exact_out);
+
 /* ---------------------------------------------------------------- batches */
 
 /* A batch holds proposals resident in HBM; load uploads and plans it,

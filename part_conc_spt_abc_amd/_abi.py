@@ -34,6 +34,9 @@ PCS_MODE_BATCHED_DIRECT = 2
 PCS_MODE_PLAIN_STREAMS = 3
 PCS_PLAIN_STREAMS = 4
 PCS_SERVE_DEVICE_LOOP = 1
+PCS_STATIC_SCRATCH_BYTES = 128 << 20
+PCS_STATIC_MAX_SIMS = 8192
+PCS_STATIC_MAX_VIDEOS = 65536
 
 F8 = C.c_float * PCS_MAX_SPECIES
 
@@ -125,6 +128,7 @@ PROTOTYPES = {
     "pcs_open": (C.c_int, [C.POINTER(pcs_system), P, C.c_int, C.POINTER(P)]),
     "pcs_close": (C.c_int, [P]),
     "pcs_simulate": (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, C.c_uint64, C.c_int, P, P, P]),
+    "pcs_simulate_static": (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, C.c_uint64, P, P, P]),
     "pcs_batch_create": (C.c_int, [P, C.POINTER(P)]),
     "pcs_batch_load": (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32]),
     "pcs_batch_run": (C.c_int, [P, C.c_uint64, C.c_int]),

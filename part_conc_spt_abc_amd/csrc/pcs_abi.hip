@@ -430,6 +430,79 @@ int restore_slices(pcs_batch* b)
 
 }  // namespace
 
+// ------------------------------------------------- allocation-free variant
+//
+// pcs_simulate_static keeps every device buffer of a run (proposal records,
+// tables, counters, results, record arena and sort scratch) in `scratch`, a
+// fixed array of PCS_STATIC_SCRATCH_BYTES on each GPU, held until the
+// process ends. The array's name is a host-side stand-in, not a device
+// address: its address on a GPU is looked up once, when the first context
+// opens on that GPU (pcs_open, with the GPU current), and kept per device in
+// g_scratch; host code uses only that pointer.
+static __device__ __align__(256) unsigned char scratch[PCS_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// With `device` current.
+int scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return set_error(PCS_EINVAL, "device %d above %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return PCS_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return set_error(PCS_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch[device] = static_cast<unsigned char*>(ptr);
+    return PCS_OK;
+}
+
+unsigned char* scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
+}
+
+// Byte offsets of one run's buffers in the scratch array (256-byte aligned);
+// the rest holds the record arena and, after it, the sort scratch.
+struct StaticLayout {
+    size_t props, in, vframes, vparticles, vprop, vtile, count, tracks, status, dist, exact, misc, rec, srt;
+    uint64_t arena_cap;  // records
+};
+
+bool static_layout(uint32_t n, uint32_t nv, StaticLayout& L)
+{
+    size_t o = 0;
+    auto take = [&o](size_t bytes) {
+        const size_t at = o;
+        o += (bytes + 255u) & ~(size_t)255u;
+        return at;
+    };
+    L.props = take((size_t)n * sizeof(DevProp));
+    L.in = take((size_t)n * sizeof(pcs_proposal));
+    L.vframes = take((size_t)nv * 4u);
+    L.vparticles = take((size_t)nv * 4u);
+    L.vprop = take((size_t)nv * 4u);
+    L.vtile = take(((size_t)nv + 1u) * 4u);
+    L.count = take((size_t)n * 4u);
+    L.tracks = take((size_t)n * 4u);
+    L.status = take((size_t)n * 4u);
+    L.dist = take((size_t)n * 8u);
+    L.exact = take((size_t)n * 8u);
+    L.misc = take(64);  // seed (u64), tile counter (u32), totals (2 x u64)
+    if (o >= PCS_STATIC_SCRATCH_BYTES) return false;
+    L.arena_cap = (PCS_STATIC_SCRATCH_BYTES - o) / 8u;
+    L.rec = o;
+    L.srt = o + (size_t)L.arena_cap * 4u;
+    return true;
+}
+
+}  // namespace
+
 // ------------------------------------------------------------------- ABI
 
 extern "C" {
@@ -490,6 +563,7 @@ int pcs_open(const pcs_system* sys, const uint32_t* obs_hist, int device, pcs_ct
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PCS_ENODEV, "no GPU visible");
     if (device < 0 || device >= ndev) return set_error(PCS_EINVAL, "device %d outside [0, %d)", device, ndev);
     PCS_HIP(hipSetDevice(device));
+    PCS_TRY(scratch_init(device));
     pcs_ctx* c = new (std::nothrow) pcs_ctx();
     if (!c) return set_error(PCS_ENOMEM, "context");
     c->device = device;
@@ -1089,6 +1163,152 @@ int pcs_pmc_weights(int device, const double* theta_new, uint32_t n_new, const d
     double sum = 0.0;
     for (uint32_t j = 0; j < n_new; ++j) sum += (w_out[j] = 1.0 / w_out[j]);
     for (uint32_t j = 0; j < n_new; ++j) w_out[j] /= sum;
+    return PCS_OK;
+}
+
+int pcs_simulate_static(pcs_ctx* c, const pcs_proposal* props, uint32_t n, const uint32_t* vframes,
+                        const uint32_t* vparticles, uint32_t n_videos, uint64_t seed, double* dist_out,
+                        uint32_t* tracks_out, uint64_t* exact_out)
+{
+    if (!c) return set_error(PCS_EINVAL, "context is NULL");
+    if (n && (!props || !vframes || !vparticles || !dist_out)) return set_error(PCS_EINVAL, "NULL argument");
+    if (n > PCS_STATIC_MAX_SIMS)
+        return set_error(PCS_EINVAL, "%u simulations, the static scratch takes at most %u", n, PCS_STATIC_MAX_SIMS);
+    if (n_videos > PCS_STATIC_MAX_VIDEOS)
+        return set_error(PCS_EINVAL, "%u videos, the static scratch takes at most %u", n_videos, PCS_STATIC_MAX_VIDEOS);
+    const pcs_tuning t = c->tune;
+    const uint32_t tp = 64u * t.particles_per_lane;
+    uint64_t next = 0, tiles = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const pcs_proposal& p = props[i];
+        if (p.reserved != 0) return set_error(PCS_EINVAL, "proposal %u: reserved field must be 0", i);
+        if (p.first_video != next)
+            return set_error(PCS_EINVAL, "proposal %u: first_video %u, expected %llu (videos follow the previous proposal's)",
+                             i, p.first_video, (unsigned long long)next);
+        next += p.n_videos;
+        if (next > n_videos) return set_error(PCS_EINVAL, "proposal %u: videos beyond %u", i, n_videos);
+        PCS_TRY(validate_proposal(c->sys, p));
+    }
+    if (next != n_videos) return set_error(PCS_EINVAL, "videos [%llu, %u) belong to no proposal", (unsigned long long)next, n_videos);
+    for (uint32_t v = 0; v < n_videos; ++v) tiles += (vparticles[v] + tp - 1u) / tp;
+    if (tiles > 0xffffffffull) return set_error(PCS_EINVAL, "more than 2^32 tiles");
+    if (n == 0) return PCS_OK;
+    PCS_HIP(hipSetDevice(c->device));
+    unsigned char* base = scratch_ptr(c->device);
+    if (!base) return set_error(PCS_ESTATE, "static scratch of device %d not set up", c->device);
+    StaticLayout L;
+    if (!static_layout(n, n_videos, L)) return set_error(PCS_EINVAL, "tables do not fit the static scratch");
+    DevProp* d_props = reinterpret_cast<DevProp*>(base + L.props);
+    uint32_t* d_vframes = reinterpret_cast<uint32_t*>(base + L.vframes);
+    uint32_t* d_vparticles = reinterpret_cast<uint32_t*>(base + L.vparticles);
+    uint32_t* d_vtile = reinterpret_cast<uint32_t*>(base + L.vtile);
+    uint32_t* d_count = reinterpret_cast<uint32_t*>(base + L.count);
+    uint32_t* d_tracks = reinterpret_cast<uint32_t*>(base + L.tracks);
+    double* d_dist = reinterpret_cast<double*>(base + L.dist);
+    uint64_t* d_exact = reinterpret_cast<uint64_t*>(base + L.exact);
+    uint64_t* d_seed = reinterpret_cast<uint64_t*>(base + L.misc);
+    uint32_t* d_tile_counter = reinterpret_cast<uint32_t*>(base + L.misc + 8u);
+    uint64_t* d_totals = reinterpret_cast<uint64_t*>(base + L.misc + 16u);
+    uint32_t* d_rec = reinterpret_cast<uint32_t*>(base + L.rec);
+    uint32_t* d_srt = reinterpret_cast<uint32_t*>(base + L.srt);
+    hipStream_t st = c->stream;
+    const bool gauss = c->sys.detect_model == PCS_DETECT_GAUSS_Z;
+    const bool zref = c->sys.z_boundary == PCS_Z_REFLECT;
+
+    PCS_HIP(hipMemcpyAsync(base + L.in, props, (size_t)n * sizeof(pcs_proposal), hipMemcpyHostToDevice, st));
+    PCS_HIP(hipMemcpyAsync(d_vframes, vframes, (size_t)n_videos * 4u, hipMemcpyHostToDevice, st));
+    PCS_HIP(hipMemcpyAsync(d_vparticles, vparticles, (size_t)n_videos * 4u, hipMemcpyHostToDevice, st));
+    PCS_HIP(hipMemcpyAsync(d_seed, &seed, sizeof(seed), hipMemcpyHostToDevice, st));
+    PrepArgs pa{};
+    pa.sys = c->d_sys;
+    pa.in = reinterpret_cast<const pcs_proposal*>(base + L.in);
+    pa.vframes = d_vframes;
+    pa.vparticles = d_vparticles;
+    pa.n = n;
+    pa.n_videos = n_videos;
+    pa.tile_particles = tp;
+    pa.cap_factor = c->cap_factor;
+    pa.props = d_props;
+    pa.vprop = reinterpret_cast<uint32_t*>(base + L.vprop);
+    pa.vtile = d_vtile;
+    pa.rec = d_rec;
+    pa.srt = d_srt;
+    pa.arena_cap = L.arena_cap;
+    pa.rec_count = d_count;
+    pa.tile_counter = d_tile_counter;
+    pa.totals = d_totals;
+    PCS_HIP(launch_prepare(pa, st));
+    WalkArgs a{};
+    a.sys = c->d_sys;
+    a.props = d_props;
+    a.vframes = d_vframes;
+    a.vparticles = d_vparticles;
+    a.vprop = pa.vprop;
+    a.vtile = d_vtile;
+    a.n_videos = n_videos;
+    a.tile_lo = 0;
+    a.tile_hi = (uint32_t)tiles;
+    a.stage_cap = t.stage_records;
+    a.rec_count = d_count;
+    a.seed = d_seed;
+    a.tile_counter = d_tile_counter;
+    PCS_HIP(launch_walk(a, (int)t.particles_per_lane, (int)t.waves_per_group, true, (uint32_t)c->n_cu * t.groups_per_cu,
+                        gauss, zref, st));
+    DistArgs d{};
+    d.sys = c->d_sys;
+    d.props = d_props;
+    d.prop_lo = 0;
+    d.n_props = n;
+    d.rec_count = d_count;
+    d.dist = d_dist;
+    d.exact = d_exact;
+    d.tracks = d_tracks;
+    d.status = reinterpret_cast<int32_t*>(base + L.status);
+    PCS_HIP(launch_distance(d, c->sys.n_k, c->sys.n_de, st));
+    uint64_t totals[2] = {0, 0};
+    PCS_HIP(hipMemcpyAsync(dist_out, d_dist, (size_t)n * 8u, hipMemcpyDeviceToHost, st));
+    if (tracks_out) PCS_HIP(hipMemcpyAsync(tracks_out, d_tracks, (size_t)n * 4u, hipMemcpyDeviceToHost, st));
+    if (exact_out) PCS_HIP(hipMemcpyAsync(exact_out, d_exact, (size_t)n * 8u, hipMemcpyDeviceToHost, st));
+    PCS_HIP(hipMemcpyAsync(totals, d_totals, sizeof(totals), hipMemcpyDeviceToHost, st));
+    PCS_HIP(hipStreamSynchronize(st));
+
+    // Slices that overflowed (distance NaN) are rerun, one launch pair each,
+    // with a slice of exactly the size the first pass counted, placed in the
+    // arena past the planned slices.
+    uint64_t used = std::min<uint64_t>(totals[0], L.arena_cap);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!(dist_out[i] != dist_out[i])) continue;
+        uint32_t cnt = 0, vt[2] = {0, 0};
+        DevProp dp;
+        PCS_HIP(hipMemcpyAsync(&cnt, d_count + i, 4u, hipMemcpyDeviceToHost, st));
+        PCS_HIP(hipMemcpyAsync(&dp, d_props + i, sizeof(DevProp), hipMemcpyDeviceToHost, st));
+        PCS_HIP(hipMemcpyAsync(&vt[0], d_vtile + props[i].first_video, 4u, hipMemcpyDeviceToHost, st));
+        PCS_HIP(hipMemcpyAsync(&vt[1], d_vtile + props[i].first_video + props[i].n_videos, 4u, hipMemcpyDeviceToHost, st));
+        PCS_HIP(hipStreamSynchronize(st));
+        if (used + cnt > L.arena_cap)
+            return set_error(PCS_EOVERFLOW, "simulation %u: %u track records do not fit the static scratch", i, cnt);
+        dp.rec = d_rec + used;
+        dp.srt = d_srt + used;
+        dp.rec_cap = cnt;
+        used += cnt;
+        const uint32_t zero = 0u;
+        PCS_HIP(hipMemcpyAsync(d_props + i, &dp, sizeof(DevProp), hipMemcpyHostToDevice, st));
+        PCS_HIP(hipMemcpyAsync(d_count + i, &zero, 4u, hipMemcpyHostToDevice, st));
+        WalkArgs ar = a;
+        ar.tile_lo = vt[0];
+        ar.tile_hi = vt[1];
+        PCS_HIP(launch_walk(ar, (int)t.particles_per_lane, (int)t.waves_per_group, false, 0, gauss, zref, st));
+        DistArgs dr = d;
+        dr.prop_lo = i;
+        dr.n_props = 1;
+        PCS_HIP(launch_distance(dr, c->sys.n_k, c->sys.n_de, st));
+        PCS_HIP(hipMemcpyAsync(dist_out + i, d_dist + i, 8u, hipMemcpyDeviceToHost, st));
+        if (tracks_out) PCS_HIP(hipMemcpyAsync(tracks_out + i, d_tracks + i, 4u, hipMemcpyDeviceToHost, st));
+        if (exact_out) PCS_HIP(hipMemcpyAsync(exact_out + i, d_exact + i, 8u, hipMemcpyDeviceToHost, st));
+        PCS_HIP(hipStreamSynchronize(st));
+        if (dist_out[i] != dist_out[i])
+            return set_error(PCS_EOVERFLOW, "simulation %u: rerun with an exact slice overflowed", i);
+    }
     return PCS_OK;
 }
 

@@ -185,6 +185,31 @@ struct LoopArgs {
     uint64_t idle_ticks;    // wall-clock ticks without work before the loop ends
 };
 
+// Tables of a batch built on the device from the caller's proposal records
+// (prepare_kernel): per proposal its DevProp with a record slice placed in
+// [rec, rec + arena_cap) (slices past the arena's end are cut short and
+// overflow), per video its proposal and tile prefix. Proposals hold
+// consecutive video ranges: props[i].first_video = sum of earlier n_videos.
+struct PrepArgs {
+    const DevSys* sys;
+    const pcs_proposal* in;   // n records
+    const uint32_t* vframes;
+    const uint32_t* vparticles;
+    uint32_t n, n_videos;
+    uint32_t tile_particles;  // 64 * particles per lane
+    uint32_t pad;
+    double cap_factor;
+    DevProp* props;
+    uint32_t* vprop;
+    uint32_t* vtile;          // n_videos + 1
+    uint32_t* rec;
+    uint32_t* srt;
+    uint64_t arena_cap;       // records in rec (and in srt)
+    uint32_t* rec_count;      // zeroed
+    uint32_t* tile_counter;   // zeroed
+    uint64_t* totals;         // [0] planned records, [1] tiles
+};
+
 // Launchers (pcs_kernels.hip). waves_per_group in {1,2,4}; ppl in {1,2}.
 hipError_t launch_reset(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter, hipStream_t s);
 hipError_t launch_walk(const WalkArgs& a, int ppl, int waves_per_group, bool persistent,
@@ -202,6 +227,7 @@ hipError_t launch_selftest_sqrt(uint32_t lo, uint32_t n, unsigned long long* bad
 hipError_t launch_serve_loop(const LoopArgs& a, uint32_t grid, uint32_t n_de_max, hipStream_t s);
 size_t serve_loop_lds(const LoopArgs& a);
 hipError_t serve_loop_occupancy(const LoopArgs& a, uint32_t n_de_max, int* groups_per_cu);
+hipError_t launch_prepare(const PrepArgs& a, hipStream_t s);
 hipError_t launch_dense_hist(const DevProp* props, uint32_t prop_lo, uint32_t n_props,
                              const uint32_t* rec_count, uint32_t max_count,
                              uint32_t n_k, uint32_t n_de, uint32_t* hist, hipStream_t s);

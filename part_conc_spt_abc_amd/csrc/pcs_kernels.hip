@@ -1216,6 +1216,89 @@ __global__ void dense_hist_kernel(const DevProp* props, uint32_t prop_lo, const 
     }
 }
 
+// Exclusive prefix sum over the 1024 threads of a block (sh: 1024 entries).
+constexpr int kPrepThreads = 1024;
+
+__device__ uint64_t block_scan_u64(uint64_t v, uint64_t* sh, uint64_t& total)
+{
+    const uint32_t tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (uint32_t o = 1; o < (uint32_t)kPrepThreads; o <<= 1) {
+        const uint64_t add = tid >= o ? sh[tid - o] : 0ull;
+        __syncthreads();
+        sh[tid] += add;
+        __syncthreads();
+    }
+    total = sh[kPrepThreads - 1];
+    const uint64_t incl = sh[tid];
+    __syncthreads();
+    return incl - v;
+}
+
+// Device-side plan of a batch (one workgroup): record slices by the
+// capacity estimate, prefix-summed into the arena; video -> proposal map;
+// tile prefix. Each thread owns a contiguous run of proposals (videos).
+__global__ __launch_bounds__(kPrepThreads) void prepare_kernel(PrepArgs a)
+{
+    __shared__ uint64_t sh[kPrepThreads];
+    const uint32_t tid = threadIdx.x;
+    const DevSys& sy = a.sys[0];
+    const uint32_t per = (a.n + kPrepThreads - 1) / kPrepThreads;
+    const uint32_t i0 = min(tid * per, a.n), i1 = min(i0 + per, a.n);
+    uint64_t run = 0;
+    for (uint32_t i = i0; i < i1; ++i) {
+        const pcs_proposal& p = a.in[i];
+        run += slice_capacity(sy, p.half_az, p.n_videos, a.vframes + p.first_video, a.vparticles + p.first_video,
+                              a.cap_factor);
+    }
+    uint64_t total_rec;
+    uint64_t off = block_scan_u64(run, sh, total_rec);
+    for (uint32_t i = i0; i < i1; ++i) {
+        const pcs_proposal& p = a.in[i];
+        const uint64_t cap = slice_capacity(sy, p.half_az, p.n_videos, a.vframes + p.first_video,
+                                            a.vparticles + p.first_video, a.cap_factor);
+        const uint64_t at = off < a.arena_cap ? off : a.arena_cap;
+        const uint64_t room = a.arena_cap - at;
+        DevProp d;
+        d.sim_id = p.sim_id;
+        d.first_video = p.first_video;
+        d.n_videos = p.n_videos;
+        d.sys_id = 0u;
+        d.rec = a.rec + at;
+        d.srt = a.srt + at;
+        d.rec_cap = (uint32_t)(cap < room ? cap : room);
+        d.key0 = 0u;
+        d.key1 = 0u;
+        d.pad = 0u;
+        for (int c = 0; c < PCS_MAX_SPECIES; ++c) {
+            d.half_az[c] = p.half_az[c];
+            d.cum_prob[c] = p.cum_prob[c];
+            d.sigma[c] = p.sigma[c];
+        }
+        a.props[i] = d;
+        a.rec_count[i] = 0u;
+        for (uint32_t v = p.first_video; v < p.first_video + p.n_videos; ++v) a.vprop[v] = i;
+        off += cap;
+    }
+    const uint32_t vper = (a.n_videos + kPrepThreads - 1) / kPrepThreads;
+    const uint32_t v0 = min(tid * vper, a.n_videos), v1 = min(v0 + vper, a.n_videos);
+    uint64_t tr = 0;
+    for (uint32_t v = v0; v < v1; ++v) tr += (a.vparticles[v] + a.tile_particles - 1u) / a.tile_particles;
+    uint64_t total_tiles;
+    uint64_t t = block_scan_u64(tr, sh, total_tiles);
+    for (uint32_t v = v0; v < v1; ++v) {
+        a.vtile[v] = (uint32_t)t;
+        t += (a.vparticles[v] + a.tile_particles - 1u) / a.tile_particles;
+    }
+    if (tid == 0) {
+        a.vtile[a.n_videos] = (uint32_t)total_tiles;
+        a.totals[0] = total_rec;
+        a.totals[1] = total_tiles;
+        *a.tile_counter = 0u;
+    }
+}
+
 // radius of word w[i] and sin/cos of the angle whose 16-bit code is w[i] >> 16
 __global__ __launch_bounds__(256) void selftest_math_kernel(const uint32_t* w, float* radius, float* s, float* c,
                                                             uint64_t n)
@@ -1448,6 +1531,13 @@ hipError_t launch_serve_loop(const LoopArgs& a, uint32_t grid, uint32_t n_de_max
     else if (n_de_max <= 1024) hipLaunchKernelGGL(serve_loop_kernel<4>, g, b, sh, s, a);
     else if (n_de_max <= 2048) hipLaunchKernelGGL(serve_loop_kernel<8>, g, b, sh, s, a);
     else hipLaunchKernelGGL(serve_loop_kernel<16>, g, b, sh, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_prepare(const PrepArgs& a, hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launch's error
+    hipLaunchKernelGGL(prepare_kernel, dim3(1), dim3(kPrepThreads), 0, s, a);
     return hipGetLastError();
 }
 
