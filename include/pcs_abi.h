@@ -194,6 +194,44 @@ int pcs_batch_overflow_reruns(pcs_batch* b, uint32_t* n);
 int pcs_batch_arena(pcs_batch* b, uint64_t* capacity_records, uint64_t* bytes);
 int pcs_batch_destroy(pcs_batch* b);
 
+/* Prior and model of the device proposal step. Parameters are rows of
+ * P = 2C + Z values per member: m_1..m_C (um^2/s), log10 c_1..log10 c_C
+ * (c per mL), az_1..az_Z (um), Z = 1 (fixed depth) or C. */
+typedef struct pcs_prior {
+    uint32_t n_components;       /* C, the context's n_species */
+    uint32_t n_depths;           /* Z */
+    double m_lo, m_hi;           /* prior interval of m */
+    double log10c_lo, log10c_hi; /* of log10 c */
+    double az_lo, az_hi;         /* of the detection depth az */
+    double deltat;               /* frame interval (s) */
+    double volume;               /* Lx Ly Lz / 1e12: particles per video per (1/mL) */
+} pcs_prior;
+
+/* Proposal step of one ABC-PMC round on the GPU, then the batch's plan (tile
+ * tables, record slices) on the GPU: the batch is left ready for
+ * pcs_batch_run, as after pcs_batch_load. Replaces the reference's host loop
+ * over waiting samples (src/gpu/detail/simulation_impl.inl:256-341;
+ * src/cpu/estimate.jl:126-150). For trial t < n_trials: a parent i with
+ * probability w_i (the first member whose cumulative weight cum_w[i] reaches
+ * a uniform u, wrand_idx_), every parameter p displaced by N(0, tau_p^2) and
+ * reflected into its closed prior interval (displace_), components sorted by
+ * m, c = 10^log10 c, cumulative abundances (last = 2), sigma = sqrt(2 m dt),
+ * half depths, and per video a Poisson(sum c * volume) particle count. The
+ * random words are Philox4x32-10 keyed by `seed` with counters (word, trial,
+ * slot, 0x70726F70), apart from the walk's; the normals are FP32 Box-Muller
+ * pairs of the walk's own math, the rest is FP64 in a fixed operation order,
+ * so a CPU restatement reproduces every result bit for bit.
+ * theta: P x n_pop (row-major), cum_w: n_pop non-decreasing, tau: P;
+ * video_frames: n_trials x videos_per_trial frame counts (trial-major);
+ * trial t gets sim_id first_sim_id + t and videos [t J, (t + 1) J).
+ * theta_out (nullable) receives the proposed parameters, P x n_trials. */
+int pcs_batch_propose(pcs_batch* b, const pcs_prior* prior, const double* theta, const double* cum_w,
+                      const double* tau, uint32_t n_pop, uint32_t n_trials, const uint32_t* video_frames,
+                      uint32_t videos_per_trial, uint32_t first_sim_id, uint64_t seed, double* theta_out);
+/* The loaded or proposed batch's proposal records (n) and video particle
+ * counts (n_videos); each nullable. */
+int pcs_batch_read_proposals(pcs_batch* b, pcs_proposal* props_out, uint32_t* video_particles_out);
+
 /* Launch shape of the batched walk kernel. Zero fields mean "default".
  * Results do not depend on the shape (the random stream is counter based). */
 typedef struct pcs_tuning {

@@ -770,7 +770,7 @@ uint64_t orc_distance(const uint32_t* H, const uint32_t* Hobs, uint32_t n_k, uin
      * of the row prefixes; after adding row i they are row i of the SAT. */
     uint64_t* S = (uint64_t*)calloc(n_de, sizeof(uint64_t));
     uint64_t* R = (uint64_t*)calloc(n_de, sizeof(uint64_t));
-    uint64_t acc = 0;
+    unsigned __int128 acc = 0;  /* 128 bits: the sum can pass 2^64 */
     for (uint32_t i = 0; i < n_k; ++i) {
         uint64_t rs = 0, rr = 0;
         for (uint32_t j = 0; j < n_de; ++j) {
@@ -779,13 +779,15 @@ uint64_t orc_distance(const uint32_t* H, const uint32_t* Hobs, uint32_t n_k, uin
             S[j] += rs;
             R[j] += rr;
             uint64_t dd = S[j] > R[j] ? S[j] - R[j] : R[j] - S[j];
-            acc += kind == 1 ? dd : dd * dd;
+            acc += kind == 1 ? (unsigned __int128)dd : (unsigned __int128)dd * dd;
         }
     }
     free(S);
     free(R);
-    if (log10_out) *log10_out = log10((double)acc);
-    return acc;
+    const uint64_t hi = (uint64_t)(acc >> 64), lo = (uint64_t)acc;
+    /* log10 of the 128-bit sum: 2^64 hi + lo rounded once to FP64 */
+    if (log10_out) *log10_out = log10(ldexp((double)hi, 64) + (double)lo);
+    return hi ? UINT64_MAX : lo;
 }
 
 /* FP32 accumulation in the spirit of src/gpu/cusim/kernel-distance.inl:
@@ -825,4 +827,152 @@ uint64_t orc_reference_hist(const uint64_t* K, const double* DE, uint64_t n, dou
     }
     if (dropped) *dropped = drop;
     return counted;
+}
+
+/* -------------------------------------------------- ABC-PMC proposal step */
+
+/* 10^x = 2^k 10^f, k = rint(x log2 10), f = x - k log10 2 with log10 2 split
+ * into a 42-bit head (k * head is exact for |k| < 2^11) and a tail;
+ * 10^f = e^y, y = f ln 10, |y| <= 0.35, by the Taylor series to degree 16
+ * (remainder below 1e-23 relative) in fma Horner form. */
+static const double EXP10_LOG2_10 = 0x1.a934f0979a371p+1;
+static const double EXP10_LOG10_2_HI = 0x1.34413509f7800p-2;
+static const double EXP10_LOG10_2_LO = 0x1.fef311f12b358p-46;
+static const double EXP10_LN10 = 0x1.26bb1bbb55516p+1;
+static const double EXP10_T[17] = {
+    0x1.0000000000000p+0, 0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5555555555555p-3,
+    0x1.5555555555555p-5, 0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+    0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
+    0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33, 0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-41,
+    0x1.ae7f3e733b81fp-45,
+};
+
+double orc_exp10(double x)
+{
+    const double k = rint(x * EXP10_LOG2_10);
+    double f = fma(-k, EXP10_LOG10_2_HI, x);
+    f = fma(-k, EXP10_LOG10_2_LO, f);
+    const double y = f * EXP10_LN10;
+    double p = EXP10_T[16];
+    for (int i = 15; i >= 0; --i) p = fma(p, y, EXP10_T[i]);
+    return ldexp(p, (int)k);
+}
+
+/* Philox counters of the proposal step: (word, trial, slot, PROP_TAG). */
+#define PROP_TAG 0x70726F70u
+#define PROP_PARENT 0xFFFFFFFFu
+#define PROP_POIS 0x40000000u
+#define PROP_REFLECT_MAX 64
+
+static void prop_block(uint64_t seed, uint32_t w0, uint32_t trial, uint32_t slot, uint32_t out[4])
+{
+    const uint32_t ctr[4] = { w0, trial, slot, PROP_TAG };
+    const uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    philox_i(ctr, key, out);
+}
+
+/* 53-bit uniform in [0, 1) from two words. */
+static inline double u53(uint32_t a, uint32_t b)
+{
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * 0x1p-53;
+}
+
+/* src/gpu/detail/simulation_impl.inl:66-79: reflect into [lo, hi]; the
+ * device stops after PROP_REFLECT_MAX sweeps (a step over 128 interval
+ * widths) and clamps. */
+static double reflect_closed(double x, double lo, double hi)
+{
+    for (int i = 0; i < PROP_REFLECT_MAX && (x < lo || x > hi); ++i) {
+        if (x < lo) x += 2.0 * (lo - x);
+        if (x > hi) x -= 2.0 * (x - hi);
+    }
+    return x < lo ? lo : (x > hi ? hi : x);
+}
+
+uint32_t orc_poisson(double lambda, uint64_t seed, uint32_t trial, uint32_t video)
+{
+    const double mu = lambda / 64.0;
+    uint32_t total = 0;
+    for (uint32_t l = 0; l < 64u; ++l) {
+        double s = 0.0;
+        uint32_t k = 0;
+        for (uint32_t b = 0;; ++b) {
+            uint32_t w[4];
+            prop_block(seed, b, trial, PROP_POIS | (video << 6) | l, w);
+            int done = 0;
+            for (int j = 0; j < 4; ++j) {
+                s += (double)(0.5f * neg2log_i(w[j]));
+                if (!(s <= mu)) { done = 1; break; }
+                ++k;
+            }
+            if (done) break;
+        }
+        total += k;
+    }
+    return total;
+}
+
+void orc_propose(const orc_prior* pr, const double* theta, const double* cum_w, const double* tau, uint32_t n_pop,
+                 uint32_t n_trials, uint32_t J, uint64_t seed, double* theta_out, float* half_az, float* cum_prob,
+                 float* sigma, uint32_t* particles, uint32_t* parent_out)
+{
+    const uint32_t C = pr->n_components, Z = pr->n_depths, P = 2u * C + Z;
+    for (uint32_t t = 0; t < n_trials; ++t) {
+        uint32_t w[4];
+        prop_block(seed, PROP_PARENT, t, 0u, w);
+        const double u = u53(w[0], w[1]);
+        uint32_t lo = 0, hi = n_pop;  /* first i with cum_w[i] >= u */
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cum_w[mid] < u) lo = mid + 1; else hi = mid;
+        }
+        const uint32_t parent = lo < n_pop ? lo : n_pop - 1u;
+        if (parent_out) parent_out[t] = parent;
+        double x[3 * ORC_MAX_SPECIES];
+        for (uint32_t p = 0; p < P; ++p) {
+            prop_block(seed, p, t, 0u, w);
+            float s, c;
+            orc_sincos16(w[1] >> 16, &s, &c);
+            const float n = bm_radius_i(w[0]) * c;
+            const double b_lo = p < C ? pr->m_lo : (p < 2u * C ? pr->log10c_lo : pr->az_lo);
+            const double b_hi = p < C ? pr->m_hi : (p < 2u * C ? pr->log10c_hi : pr->az_hi);
+            x[p] = reflect_closed(theta[(uint64_t)p * n_pop + parent] + tau[p] * (double)n, b_lo, b_hi);
+        }
+        /* stable sort of the components by m */
+        uint32_t perm[ORC_MAX_SPECIES];
+        for (uint32_t i = 0; i < C; ++i) perm[i] = i;
+        for (uint32_t i = 1; i < C; ++i) {
+            const uint32_t v = perm[i];
+            uint32_t j = i;
+            while (j > 0 && x[perm[j - 1]] > x[v]) { perm[j] = perm[j - 1]; --j; }
+            perm[j] = v;
+        }
+        double y[3 * ORC_MAX_SPECIES];
+        for (uint32_t i = 0; i < C; ++i) {
+            y[i] = x[perm[i]];
+            y[C + i] = x[C + perm[i]];
+            if (Z > 1u) y[2u * C + i] = x[2u * C + perm[i]];
+        }
+        if (Z == 1u) y[2u * C] = x[2u * C];
+        if (theta_out)
+            for (uint32_t p = 0; p < P; ++p) theta_out[(uint64_t)p * n_trials + t] = y[p];
+        double cc[ORC_MAX_SPECIES], csum = 0.0;
+        for (uint32_t i = 0; i < C; ++i) csum += (cc[i] = orc_exp10(y[C + i]));
+        double acc = 0.0;
+        for (uint32_t i = 0; i < ORC_MAX_SPECIES; ++i) {
+            float hz = 0.0f, cp = 0.0f, sg = 0.0f;
+            if (i < C) {
+                acc += cc[i] / csum;
+                cp = i + 1u == C ? 2.0f : (float)acc;
+                sg = sqrtf((float)(2.0 * y[i] * pr->deltat));
+                hz = 0.5f * (float)y[2u * C + (Z > 1u ? i : 0u)];
+            }
+            if (half_az) half_az[(uint64_t)t * ORC_MAX_SPECIES + i] = hz;
+            if (cum_prob) cum_prob[(uint64_t)t * ORC_MAX_SPECIES + i] = cp;
+            if (sigma) sigma[(uint64_t)t * ORC_MAX_SPECIES + i] = sg;
+        }
+        const double lambda = csum * pr->volume;
+        if (particles)
+            for (uint32_t v = 0; v < J; ++v) particles[(uint64_t)t * J + v] = orc_poisson(lambda, seed, t, v);
+    }
 }

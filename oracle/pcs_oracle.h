@@ -35,6 +35,14 @@
  *                         hand-checked tiny cases, FP64 twin agreement
  *   orc_distance          numpy cumsum(cumsum()) on random histograms
  *   orc_reference_hist    test_files fixture totals (36,701 in range)
+ *   orc_exp10             mpmath 10^x, ulp bound on a grid over [-30, 30]
+ *   orc_poisson           pmf against scipy.stats.poisson (chi-square) for
+ *                         small lambda, mean and variance for large lambda
+ *   orc_propose           parent frequencies against the weights (chi-square),
+ *                         displacements N(0, tau^2) (KS) away from the bounds,
+ *                         reflection into the bounds, tau = 0 leaves a member
+ *                         unchanged, sorted components, cum_prob / sigma /
+ *                         half_az from their closed forms
  */
 #ifndef PCS_ORACLE_H
 #define PCS_ORACLE_H
@@ -133,8 +141,9 @@ void orc_trace_particle(const orc_system* sys, const orc_proposal* p, const orc_
                         float* xyz32, double* xyz64);
 
 /* Dense textbook distance: S = cumsum(cumsum(H, rows), cols), same for the
- * observed histogram Hobs; kind 0: sum (S-R)^2, kind 1: sum |S-R|. Returns
- * the exact integer sum; *log10_out gets log10 of it in FP64. */
+ * observed histogram Hobs; kind 0: sum (S-R)^2, kind 1: sum |S-R|, summed in
+ * 128 bits. Returns the exact integer sum, saturated at 2^64 - 1; *log10_out
+ * gets log10 of the 128-bit sum, (2^64 hi + lo) rounded once to FP64. */
 uint64_t orc_distance(const uint32_t* H, const uint32_t* Hobs, uint32_t n_k, uint32_t n_de,
                       int kind, double* log10_out);
 
@@ -151,6 +160,39 @@ uint64_t orc_reference_hist(const uint64_t* K, const double* DE, uint64_t n, dou
 /* 2D inclusive prefix sum (summed-area table) in uint32 (wrapping like the
  * reference's uint32 SAT). */
 void orc_sat_u32(const uint32_t* H, uint32_t n_k, uint32_t n_de, uint32_t* S);
+
+/* 10^x in FP64 by the device's operation sequence (Cody-Waite reduction by
+ * log10 2, Taylor series of e^y to degree 16, fma Horner, ldexp). */
+double orc_exp10(double x);
+
+/* Prior and model of the ABC-PMC proposal step. */
+typedef struct orc_prior {
+    uint32_t n_components;  /* C */
+    uint32_t n_depths;      /* Z: 1 (fixed depth) or C */
+    double m_lo, m_hi, log10c_lo, log10c_hi, az_lo, az_hi;
+    double deltat;          /* s */
+    double volume;          /* Lx Ly Lz / 1e12 */
+} orc_prior;
+
+/* Proposal step of one ABC-PMC round (restating the reference's host loop,
+ * src/gpu/detail/simulation_impl.inl:256-341 and src/cpu/estimate.jl:126-150,
+ * with the device's Philox stream): for trial t, the parent is the first
+ * member whose cumulative weight reaches a 53-bit uniform (wrand_idx_); each
+ * of the P = 2C + Z parameters is displaced by tau_p times an FP32 Box-Muller
+ * normal and reflected into its closed prior interval (displace_); the
+ * components are sorted by m (stable); c = 10^log10c; cum_prob, sigma and
+ * half_az as the per-trial precompute (with sigma = sqrtf of the FP32-rounded
+ * 2 m dt); each of the J video counts is Poisson(sum c * volume), drawn as
+ * the sum over 64 lanes of the arrivals of a unit-rate Poisson process on
+ * [0, lambda / 64]. theta is P x n_pop and theta_out P x n_trials, row-major.
+ * half_az, cum_prob and sigma get 8 floats per trial, particles J per trial,
+ * parent_out one index per trial (each nullable). */
+void orc_propose(const orc_prior* pr, const double* theta, const double* cum_w, const double* tau, uint32_t n_pop,
+                 uint32_t n_trials, uint32_t J, uint64_t seed, double* theta_out, float* half_az, float* cum_prob,
+                 float* sigma, uint32_t* particles, uint32_t* parent_out);
+
+/* Poisson(lambda) count of the proposal step's stream for trial t, video v. */
+uint32_t orc_poisson(double lambda, uint64_t seed, uint32_t trial, uint32_t video);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,21 @@ class pcs_tuning(C.Structure):
     ]
 
 
+class pcs_prior(C.Structure):
+    _fields_ = [
+        ("n_components", C.c_uint32),
+        ("n_depths", C.c_uint32),
+        ("m_lo", C.c_double),
+        ("m_hi", C.c_double),
+        ("log10c_lo", C.c_double),
+        ("log10c_hi", C.c_double),
+        ("az_lo", C.c_double),
+        ("az_hi", C.c_double),
+        ("deltat", C.c_double),
+        ("volume", C.c_double),
+    ]
+
+
 class pcs_server_config(C.Structure):
     _fields_ = [
         ("arena_bytes", C.c_uint64),
@@ -139,6 +154,9 @@ PROTOTYPES = {
     "pcs_batch_overflow_reruns": (C.c_int, [P, U32P]),
     "pcs_batch_arena": (C.c_int, [P, U64P, U64P]),
     "pcs_batch_destroy": (C.c_int, [P]),
+    "pcs_batch_propose": (C.c_int, [P, C.POINTER(pcs_prior), P, P, P, C.c_uint32, C.c_uint32, P, C.c_uint32,
+                                    C.c_uint32, C.c_uint64, P]),
+    "pcs_batch_read_proposals": (C.c_int, [P, P, P]),
     "pcs_set_tuning": (C.c_int, [P, C.POINTER(pcs_tuning)]),
     "pcs_get_tuning": (C.c_int, [P, C.POINTER(pcs_tuning)]),
     "pcs_set_record_capacity_factor": (C.c_int, [P, C.c_double]),

@@ -242,6 +242,7 @@ void Estimator::simulate(const std::vector<Trial>& trials, std::vector<double>& 
         }
     }
     dist.resize(n);
+    autotune_once(props, frames, parts, J);
     const auto t1 = Clock::now();
     check(pcs_batch_load(batch_, props.data(), (uint32_t)n, frames.data(), parts.data(), (uint32_t)(n * J)),
           "pcs_batch_load");
@@ -251,6 +252,92 @@ void Estimator::simulate(const std::vector<Trial>& trials, std::vector<double>& 
     const auto t3 = Clock::now();
     if (cfg_.verbosity >= 2)
         std::printf("    round: %zu simulations, GPU run + read %.2f ms\n", n,
+                    std::chrono::duration<double, std::milli>(t3 - t2).count());
+    tPack_ += std::chrono::duration<double>(t1 - t0).count();
+    tLoad_ += std::chrono::duration<double>(t2 - t1).count();
+    tGpu_ += std::chrono::duration<double>(t3 - t2).count();
+    simulations_ += n;
+}
+
+void Estimator::autotune_once(const std::vector<pcs_proposal>& props, const std::vector<uint32_t>& frames,
+                              const std::vector<uint32_t>& parts, std::size_t J)
+{
+    if (tuned_ || !cfg_.autotune_min || props.size() < cfg_.autotune_min) return;
+    tuned_ = true;
+    // the first proposals hold videos [0, k J) in order
+    const std::size_t k = std::min<std::size_t>(props.size(), 4000);
+    pcs_tuning best{};
+    float ms = 0.0f;
+    check(pcs_autotune(ctx_, props.data(), (uint32_t)k, frames.data(), parts.data(), (uint32_t)(k * J), &best, &ms),
+          "pcs_autotune");
+    if (cfg_.verbosity >= 1)
+        std::printf("  autotune: %u particle(s)/lane, %u waves/group, %u groups/CU (%.2f ms on %zu proposals)\n",
+                    best.particles_per_lane, best.waves_per_group, best.groups_per_cu, ms, k);
+}
+
+void Estimator::simulate_proposed(std::vector<Trial>& trials, std::vector<double>& dist)
+{
+    using Clock = std::chrono::steady_clock;
+    const auto t0 = Clock::now();
+    const std::size_t n = trials.size(), J = par_.frameCounts.size(), P = 2 * C_ + Z_;
+    // population rows m_1..m_C, log10 c_1..log10 c_C, az_1..az_Z and their scales
+    std::vector<double> theta(P * N_), cum(N_), tau(P);
+    for (std::size_t c = 0; c < C_; ++c) {
+        std::copy(m_[c].begin(), m_[c].end(), theta.begin() + (std::ptrdiff_t)(c * N_));
+        std::copy(log10c_[c].begin(), log10c_[c].end(), theta.begin() + (std::ptrdiff_t)((C_ + c) * N_));
+        tau[c] = tauM_[c];
+        tau[C_ + c] = tauC_[c];
+    }
+    for (std::size_t z = 0; z < Z_; ++z) {
+        std::copy(az_[z].begin(), az_[z].end(), theta.begin() + (std::ptrdiff_t)((2 * C_ + z) * N_));
+        tau[2 * C_ + z] = tauAz_[z];
+    }
+    std::partial_sum(w_.begin(), w_.end(), cum.begin());
+    pcs_prior pr{};
+    pr.n_components = (uint32_t)C_;
+    pr.n_depths = (uint32_t)Z_;
+    pr.m_lo = par_.m.lower;
+    pr.m_hi = par_.m.upper;
+    pr.log10c_lo = lowerC_;
+    pr.log10c_hi = upperC_;
+    pr.az_lo = par_.az.lower;
+    pr.az_hi = par_.az.upper;
+    pr.deltat = par_.deltaT;
+    pr.volume = par_.Lx * par_.Ly * par_.Lz / 1e12;
+    std::vector<uint32_t> frames(n * J);
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t v = 0; v < J; ++v) frames[i * J + v] = (uint32_t)par_.frameCounts[v];
+    std::vector<double> out(P * n);
+    const auto t1 = Clock::now();
+    check(pcs_batch_propose(batch_, &pr, theta.data(), cum.data(), tau.data(), (uint32_t)N_, (uint32_t)n,
+                            frames.data(), (uint32_t)J, 0, rng_(), out.data()),
+          "pcs_batch_propose");
+    const auto t2 = Clock::now();
+    for (std::size_t a = 0; a < n; ++a) {
+        Trial& t = trials[a];
+        t.m.resize(C_);
+        t.log10c.resize(C_);
+        t.az.resize(Z_);
+        for (std::size_t c = 0; c < C_; ++c) {
+            t.m[c] = out[c * n + a];
+            t.log10c[c] = out[(C_ + c) * n + a];
+        }
+        for (std::size_t z = 0; z < Z_; ++z) t.az[z] = out[(2 * C_ + z) * n + a];
+    }
+    if (!tuned_ && cfg_.autotune_min && n >= cfg_.autotune_min) {
+        // calibrate on the proposals just drawn (records and counts from the GPU)
+        std::vector<pcs_proposal> props(n);
+        std::vector<uint32_t> parts(n * J);
+        check(pcs_batch_read_proposals(batch_, props.data(), parts.data()), "pcs_batch_read_proposals");
+        autotune_once(props, frames, parts, J);
+    }
+    check(pcs_batch_run(batch_, rng_(), cfg_.mode), "pcs_batch_run");
+    dist.resize(n);
+    check(pcs_batch_read(batch_, dist.data(), nullptr, nullptr), "pcs_batch_read");
+    const auto t3 = Clock::now();
+    if (cfg_.verbosity >= 2)
+        std::printf("    round: %zu proposals drawn on the GPU, propose + plan %.2f ms, run + read %.2f ms\n", n,
+                    std::chrono::duration<double, std::milli>(t2 - t1).count(),
                     std::chrono::duration<double, std::milli>(t3 - t2).count());
     tPack_ += std::chrono::duration<double>(t1 - t0).count();
     tLoad_ += std::chrono::duration<double>(t2 - t1).count();
@@ -303,15 +390,19 @@ void Estimator::run()
         double trialsTotal = 0.0, trialsAvg = 0.0;
         std::size_t rounds = 0;
         while (!active.empty() && trialsAvg < (double)par_.avgTrialCount) {
-            const auto tp = Clock::now();
             std::vector<Trial> batch(active.size());
-            for (std::size_t a = 0; a < active.size(); ++a) {
-                const std::size_t parent = pick_by_weight(cum, u01(rng_));
-                batch[a] = perturb(parent);
-            }
-            tPack_ += std::chrono::duration<double>(Clock::now() - tp).count();
             std::vector<double> d;
-            simulate(batch, d);
+            if (cfg_.device_propose_min && active.size() >= cfg_.device_propose_min) {
+                simulate_proposed(batch, d);
+            } else {
+                const auto tp = Clock::now();
+                for (std::size_t a = 0; a < active.size(); ++a) {
+                    const std::size_t parent = pick_by_weight(cum, u01(rng_));
+                    batch[a] = perturb(parent);
+                }
+                tPack_ += std::chrono::duration<double>(Clock::now() - tp).count();
+                simulate(batch, d);
+            }
             std::vector<std::size_t> still;
             for (std::size_t a = 0; a < active.size(); ++a) {
                 cur[active[a]] = std::move(batch[a]);
@@ -336,27 +427,33 @@ void Estimator::run()
         // Weights from the new population against the previous one.
         const auto tw = Clock::now();
         if (par_.weightingScheme == io::WeightingScheme::pmcStandard) {
+            // A parameter whose population has no spread (tau = 0: a prior
+            // interval of one point) was never displaced; its kernel factor
+            // is the same for every pair, so it is left out instead of
+            // turning every weight into 0/0.
             std::vector<std::vector<double>> tn, to;
             std::vector<double> tau;
+            auto add = [&](const std::vector<double>& row, const std::vector<double>& old, double t) {
+                if (!(t > 0.0)) return;
+                tn.push_back(row);
+                to.push_back(old);
+                tau.push_back(t);
+            };
             for (std::size_t c = 0; c < C_; ++c) {
                 std::vector<double> row(N_);
                 for (std::size_t i = 0; i < N_; ++i) row[i] = cur[i].m[c];
-                tn.push_back(row);
-                to.push_back(m_[c]);
-                tau.push_back(tauM_[c]);
+                add(row, m_[c], tauM_[c]);
                 for (std::size_t i = 0; i < N_; ++i) row[i] = cur[i].log10c[c];
-                tn.push_back(row);
-                to.push_back(log10c_[c]);
-                tau.push_back(tauC_[c]);
+                add(row, log10c_[c], tauC_[c]);
             }
             for (std::size_t z = 0; z < Z_; ++z) {
                 std::vector<double> row(N_);
                 for (std::size_t i = 0; i < N_; ++i) row[i] = cur[i].az[z];
-                tn.push_back(row);
-                to.push_back(az_[z]);
-                tau.push_back(tauAz_[z]);
+                add(row, az_[z], tauAz_[z]);
             }
-            if (N_ >= cfg_.device_weights_min) {
+            if (tn.empty()) {
+                w_.assign(N_, 1.0 / (double)N_);  // every member identical: equal weights
+            } else if (N_ >= cfg_.device_weights_min) {
                 // O(N^2) kernel sums on the GPU
                 std::vector<double> xn, xo;
                 for (std::size_t p = 0; p < tn.size(); ++p) {

@@ -44,6 +44,13 @@ struct Config {
     // pmc-standard weights move to the GPU (pcs_pmc_weights) from this
     // population size on; below it the host loop is cheaper than the copies.
     std::size_t device_weights_min = 256;
+    // Rounds of at least this many trials draw their proposals on the GPU
+    // (pcs_batch_propose: parents, displacements, Poisson counts and the
+    // batch plan); smaller rounds on the host. 0 = always on the host.
+    std::size_t device_propose_min = 2048;
+    // The batched walk's launch shape is autotuned (pcs_autotune) once, on
+    // the first round of at least this many trials. 0 = never.
+    std::size_t autotune_min = 4096;
 };
 
 class Estimator {
@@ -66,6 +73,12 @@ private:
 
     // Simulate trials[i] for the given members; writes log10 distances.
     void simulate(const std::vector<Trial>& trials, std::vector<double>& dist);
+    // Propose trials.size() trials from the population on the GPU and
+    // simulate them; fills trials and dist.
+    void simulate_proposed(std::vector<Trial>& trials, std::vector<double>& dist);
+    // pcs_autotune on the first calibration proposals of a large round, once.
+    void autotune_once(const std::vector<pcs_proposal>& props, const std::vector<uint32_t>& frames,
+                       const std::vector<uint32_t>& parts, std::size_t J);
     Trial perturb(std::size_t parent);
     void compute_tau();
 
@@ -86,6 +99,7 @@ private:
     double lowerC_, upperC_;
     pcs_ctx* ctx_ = nullptr;
     pcs_batch* batch_ = nullptr;
+    bool tuned_ = false;
 };
 
 }  // namespace pcs::abc

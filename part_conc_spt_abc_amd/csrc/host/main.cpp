@@ -29,6 +29,7 @@ struct Cli {
     int verbosity = 0;
     bool listDevices = false, parseOnly = false;
     std::string writeSample;
+    std::size_t deviceProposeMin = 2048, autotuneMin = 4096;
 };
 
 void help(FILE* f, const char* self)
@@ -45,6 +46,10 @@ void help(FILE* f, const char* self)
                  "  --max-steps, -S N        : stop after N generations\n"
                  "  --fixed-seed, -F N       : seed the host random numbers with N\n"
                  "  --quiet, -q / --verbose, -v\n"
+                 "  --device-propose-min N   : draw a round's proposals on the GPU from N trials on (default\n"
+                 "                             2048; 0 = always on the host)\n"
+                 "  --autotune-min N         : autotune the walk's launch shape on the first round of N or\n"
+                 "                             more trials (default 4096; 0 = never)\n"
                  "  --list-devices           : list GPUs and exit\n"
                  "  --parse-only             : print the parsed input as JSON and exit (no GPU)\n"
                  "  --write-sample FILE      : write a fixed output XML and exit (no GPU)\n"
@@ -99,7 +104,9 @@ Cli parse(int argc, char** argv)
             if (v != "f" && v != "d")
                 throw std::runtime_error("Scalar type must be either 'f' (float) or 'd' (double) and not '" + v + "'");
             std::fprintf(stderr, "Note: %s ignored: host FP64, device FP32, exact integer distance\n", a.c_str());
-        } else if (a == "--list-devices") c.listDevices = true;
+        } else if (a == "--device-propose-min") c.deviceProposeMin = number("a number of trials");
+        else if (a == "--autotune-min") c.autotuneMin = number("a number of trials");
+        else if (a == "--list-devices") c.listDevices = true;
         else if (a == "--parse-only") c.parseOnly = true;
         else if (a == "--write-sample") c.writeSample = value("a file");
         else throw std::runtime_error("Unknown command line argument '" + a + "'");
@@ -187,6 +194,8 @@ try {
     cfg.maxIter = cli.maxIter;
     cfg.verbosity = cli.verbosity;
     cfg.seed = cli.fixedSeed ? cli.seed : std::random_device{}();
+    cfg.device_propose_min = cli.deviceProposeMin;
+    cfg.autotune_min = cli.autotuneMin;
 
     const auto start = std::chrono::system_clock::now();
     const auto t0 = std::chrono::steady_clock::now();

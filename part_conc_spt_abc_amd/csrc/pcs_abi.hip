@@ -161,6 +161,18 @@ struct pcs_batch {
     uint64_t* d_seed = nullptr;
     uint64_t* h_seed = nullptr;  // pinned
     size_t cap_props = 0, cap_videos = 0, cap_arena = 0;
+    // device-planned batch (pcs_batch_propose): the proposal records, the
+    // population it was drawn from and the proposed parameters stay on the
+    // device; the plan (tiles, slices) is made there, host copies of vtile
+    // are fetched only when a plain launch or a rerun needs them
+    bool dev_planned = false;
+    bool host_vtile = false;     // vtile (host) holds the device plan
+    uint32_t J = 0;              // videos per trial
+    pcs_proposal* d_in = nullptr;
+    double *d_theta_pop = nullptr, *d_cumw = nullptr, *d_theta_new = nullptr;
+    uint64_t* d_totals = nullptr;  // [0] planned records, [1] tiles
+    size_t cap_in = 0, cap_pop = 0, cap_new = 0;
+    uint64_t last_need = 0;      // records the last device plan asked for
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -264,7 +276,51 @@ WalkArgs walk_args(pcs_batch* b, uint32_t tile_lo, uint32_t tile_hi)
     a.rec_count = b->d_count;
     a.seed = b->d_seed;
     a.tile_counter = b->d_tile_counter;
+    a.tile_total = b->dev_planned ? b->d_totals + 1 : nullptr;
     return a;
+}
+
+// Plan of a device-planned batch, made on the device from its resident
+// proposal records: DevProp table with record slices, video map, tile prefix.
+int device_plan(pcs_batch* b)
+{
+    pcs_ctx* c = b->ctx;
+    PrepArgs pa{};
+    pa.sys = c->d_sys;
+    pa.in = b->d_in;
+    pa.vframes = b->d_vframes;
+    pa.vparticles = b->d_vparticles;
+    pa.n = b->n_props;
+    pa.n_videos = b->n_videos;
+    pa.tile_particles = 64u * c->tune.particles_per_lane;
+    pa.cap_factor = c->cap_factor;
+    pa.props = b->d_props;
+    pa.vprop = b->d_vprop;
+    pa.vtile = b->d_vtile;
+    pa.rec = b->d_rec;
+    pa.srt = b->d_srt;
+    pa.arena_cap = b->d_rec ? b->cap_arena : 0u;
+    pa.rec_count = b->d_count;
+    pa.tile_counter = b->d_tile_counter;
+    pa.totals = b->d_totals;
+    PCS_HIP(launch_prepare(pa, b->stream));
+    b->planned_ppl = (int)c->tune.particles_per_lane;
+    b->host_vtile = false;
+    return PCS_OK;
+}
+
+// Host copy of a device plan's tile prefix (plain launches and reruns
+// address tiles from the host).
+int fetch_plan(pcs_batch* b)
+{
+    if (!b->dev_planned || b->host_vtile) return PCS_OK;
+    b->vtile.resize((size_t)b->n_videos + 1u);
+    PCS_HIP(hipMemcpyAsync(b->vtile.data(), b->d_vtile, ((size_t)b->n_videos + 1u) * 4u, hipMemcpyDeviceToHost,
+                           b->stream));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    b->n_tiles = b->vtile[b->n_videos];
+    b->host_vtile = true;
+    return PCS_OK;
 }
 
 DistArgs dist_args(pcs_batch* b, uint32_t lo, uint32_t n)
@@ -293,8 +349,8 @@ int enqueue_batched(pcs_batch* b, bool with_events)
     PCS_HIP(launch_reset(b->d_count, b->n_props, b->d_tile_counter, b->stream));
     if (with_events) PCS_HIP(hipEventRecord(b->ev[1], b->stream));
     const uint32_t grid = (uint32_t)c->n_cu * t.groups_per_cu;
-    PCS_HIP(launch_walk(walk_args(b, 0, b->n_tiles), (int)t.particles_per_lane, (int)t.waves_per_group, true, grid,
-                        gauss, zref, b->stream));
+    PCS_HIP(launch_walk(walk_args(b, 0, b->dev_planned ? 0xffffffffu : b->n_tiles), (int)t.particles_per_lane,
+                        (int)t.waves_per_group, true, grid, gauss, zref, b->stream));
     if (with_events) PCS_HIP(hipEventRecord(b->ev[2], b->stream));
     PCS_HIP(launch_distance(dist_args(b, 0, b->n_props), c->sys.n_k, c->sys.n_de, b->stream));
     if (with_events) PCS_HIP(hipEventRecord(b->ev[3], b->stream));
@@ -326,6 +382,7 @@ int enqueue_plain(pcs_batch* b, const std::vector<uint32_t>* subset, bool stream
     const bool gauss = c->sys.detect_model == PCS_DETECT_GAUSS_Z;
     const bool zref = c->sys.z_boundary == PCS_Z_REFLECT;
     const uint32_t n = subset ? (uint32_t)subset->size() : b->n_props;
+    PCS_TRY(fetch_plan(b));
     if (streams) {
         PCS_TRY(plain_streams(b));
         PCS_HIP(hipEventRecord(b->pfork, b->stream));
@@ -399,14 +456,23 @@ int rerun_overflows(pcs_batch* b)
     if (over.empty()) return PCS_OK;
     PCS_TRY(dev_alloc(&b->d_rec2, need));
     PCS_TRY(dev_alloc(&b->d_srt2, need));
+    // the overflowed proposals' table entries, patched to their exact slices
+    std::vector<DevProp> dp(over.size());
+    if (b->dev_planned) {
+        for (size_t k = 0; k < over.size(); ++k)
+            PCS_HIP(hipMemcpyAsync(&dp[k], b->d_props + over[k], sizeof(DevProp), hipMemcpyDeviceToHost, b->stream));
+        PCS_HIP(hipStreamSynchronize(b->stream));
+    } else {
+        for (size_t k = 0; k < over.size(); ++k) dp[k] = b->hprops[over[k]];
+    }
     uint64_t off = 0;
-    for (uint32_t P : over) {
-        DevProp dp = b->hprops[P];
-        dp.rec = b->d_rec2 + off;
-        dp.srt = b->d_srt2 + off;
-        dp.rec_cap = cnt[P];
+    for (size_t k = 0; k < over.size(); ++k) {
+        const uint32_t P = over[k];
+        dp[k].rec = b->d_rec2 + off;
+        dp[k].srt = b->d_srt2 + off;
+        dp[k].rec_cap = cnt[P];
         off += cnt[P];
-        PCS_HIP(hipMemcpyAsync(b->d_props + P, &dp, sizeof(DevProp), hipMemcpyHostToDevice, b->stream));
+        PCS_HIP(hipMemcpyAsync(b->d_props + P, &dp[k], sizeof(DevProp), hipMemcpyHostToDevice, b->stream));
         PCS_HIP(hipMemsetAsync(b->d_count + P, 0, 4, b->stream));
     }
     PCS_TRY(enqueue_plain(b, &over));
@@ -414,13 +480,18 @@ int rerun_overflows(pcs_batch* b)
     return PCS_OK;
 }
 
-// Undo rerun_overflows before a new run.
-int restore_slices(pcs_batch* b)
+// Undo rerun_overflows before a new run: the table again points at the
+// planned slices (re-planned on the device for a device-planned batch,
+// unless `replan` is false because a new plan follows anyway).
+int restore_slices(pcs_batch* b, bool replan = true)
 {
     if (!b->d_rec2) return PCS_OK;
-    if (b->n_props)
+    if (b->dev_planned) {
+        if (replan) PCS_TRY(device_plan(b));
+    } else if (b->n_props) {
         PCS_HIP(hipMemcpyAsync(b->d_props, b->hprops.data(), b->n_props * sizeof(DevProp), hipMemcpyHostToDevice,
                                b->stream));
+    }
     PCS_HIP(hipStreamSynchronize(b->stream));
     hipFree(b->d_rec2);
     hipFree(b->d_srt2);
@@ -663,7 +734,8 @@ int pcs_batch_destroy(pcs_batch* b)
         if (ps) hipStreamSynchronize(ps);
     destroy_graph(b);
     void* ptrs[] = {b->d_props, b->d_vframes, b->d_vparticles, b->d_vprop, b->d_vtile, b->d_rec, b->d_srt,
-                    b->d_rec2, b->d_srt2, b->d_count, b->d_tracks, b->d_tile_counter, b->d_dist, b->d_exact, b->d_status, b->d_seed};
+                    b->d_rec2, b->d_srt2, b->d_count, b->d_tracks, b->d_tile_counter, b->d_dist, b->d_exact, b->d_status,
+                    b->d_seed, b->d_in, b->d_theta_pop, b->d_cumw, b->d_theta_new, b->d_totals};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (b->h_seed) hipHostFree(b->h_seed);
@@ -692,6 +764,8 @@ int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const ui
     PCS_TRY(restore_slices(b));
     destroy_graph(b);
     b->ran = false;
+    b->dev_planned = false;
+    b->host_vtile = false;
     b->props.assign(props, props + n);
     b->hprops.resize(n);
     b->vframes.assign(vframes, vframes + n_videos);
@@ -785,8 +859,12 @@ int pcs_batch_run(pcs_batch* b, uint64_t seed, int mode)
     PCS_HIP(hipSetDevice(c->device));
     if (b->planned_ppl != (int)c->tune.particles_per_lane) {
         PCS_HIP(hipStreamSynchronize(b->stream));
-        plan_tiles(b, (int)c->tune.particles_per_lane);
-        PCS_TRY(upload_plan(b));
+        if (b->dev_planned) {
+            PCS_TRY(device_plan(b));
+        } else {
+            plan_tiles(b, (int)c->tune.particles_per_lane);
+            PCS_TRY(upload_plan(b));
+        }
         destroy_graph(b);
     }
     // The seed lives in device memory so one captured graph serves every seed.
@@ -838,6 +916,12 @@ int pcs_batch_timing(pcs_batch* b, float out[3])
 int pcs_batch_arena(pcs_batch* b, uint64_t* capacity_records, uint64_t* bytes)
 {
     if (!b) return set_error(PCS_EINVAL, "batch is NULL");
+    if (b->dev_planned && b->d_totals) {  // the device plan's record slices
+        uint64_t tot[2] = {0, 0};
+        PCS_HIP(hipMemcpyAsync(tot, b->d_totals, sizeof(tot), hipMemcpyDeviceToHost, b->stream));
+        PCS_HIP(hipStreamSynchronize(b->stream));
+        b->arena_records = std::min<uint64_t>(tot[0], b->cap_arena);
+    }
     if (capacity_records) *capacity_records = b->arena_records;
     if (bytes) *bytes = 2ull * 4ull * b->cap_arena;
     return PCS_OK;
@@ -876,7 +960,8 @@ int pcs_batch_read(pcs_batch* b, double* dist_out, uint32_t* hist_out, uint32_t*
         for (uint32_t lo = 0; lo < b->n_props && r == PCS_OK; lo += chunk) {
             const uint32_t m = std::min(chunk, b->n_props - lo);
             uint32_t mx = 0;
-            for (uint32_t i = lo; i < lo + m; ++i) mx = std::max(mx, std::min(cnt[i], b->hprops[i].rec_cap));
+            for (uint32_t i = lo; i < lo + m; ++i)  // the kernel caps each count at its slice
+                mx = std::max(mx, b->dev_planned ? cnt[i] : std::min(cnt[i], b->hprops[i].rec_cap));
             hipError_t e = hipMemsetAsync(d_h, 0, per * m * 4, b->stream);
             if (e == hipSuccess)
                 e = launch_dense_hist(b->d_props, lo, m, b->d_count, mx, c->sys.n_k, c->sys.n_de, d_h, b->stream);
@@ -889,6 +974,161 @@ int pcs_batch_read(pcs_batch* b, double* dist_out, uint32_t* hist_out, uint32_t*
         PCS_TRY(r);
     }
     PCS_HIP(hipStreamSynchronize(b->stream));
+    return PCS_OK;
+}
+
+int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, const double* cum_w,
+                      const double* tau, uint32_t n_pop, uint32_t n_trials, const uint32_t* video_frames,
+                      uint32_t J, uint32_t first_sim_id, uint64_t seed, double* theta_out)
+{
+    if (!b || !pr) return set_error(PCS_EINVAL, "NULL argument");
+    pcs_ctx* c = b->ctx;
+    const uint32_t C = pr->n_components, Z = pr->n_depths;
+    if (C != c->sys.n_species) return set_error(PCS_EINVAL, "prior has %u components, the model %u", C, c->sys.n_species);
+    if (Z != 1u && Z != C) return set_error(PCS_EINVAL, "n_depths %u is neither 1 nor n_components", Z);
+    const uint32_t P = 2u * C + Z;
+    if (n_pop == 0 || !theta || !cum_w || !tau) return set_error(PCS_EINVAL, "population is empty or NULL");
+    if (J == 0 || J > 4096u) return set_error(PCS_EINVAL, "videos_per_trial %u outside [1, 4096]", J);
+    if (n_trials && !video_frames) return set_error(PCS_EINVAL, "video_frames is NULL");
+    if ((uint64_t)n_trials * J >= 0x80000000ull) return set_error(PCS_EINVAL, "n_trials x videos_per_trial too large");
+    const double lo[3] = {pr->m_lo, pr->log10c_lo, pr->az_lo}, hi[3] = {pr->m_hi, pr->log10c_hi, pr->az_hi};
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]) || !(lo[k] <= hi[k]))
+            return set_error(PCS_EINVAL, "prior interval %d is empty or not finite", k);
+    if (!(pr->m_lo >= 0.0) || !(pr->az_lo >= 0.0)) return set_error(PCS_EINVAL, "m and az bounds must be >= 0");
+    if (!(pr->deltat > 0.0) || !(pr->volume > 0.0)) return set_error(PCS_EINVAL, "deltat and volume must be positive");
+    // the Poisson draws take ~lambda / 64 exponentials per lane: bound lambda
+    if ((double)C * std::pow(10.0, pr->log10c_hi) * pr->volume > 16777216.0)
+        return set_error(PCS_EINVAL, "prior allows more than 2^24 expected particles per video");
+    // bounded inputs keep every reflection and sum finite
+    for (uint32_t p = 0; p < P; ++p)
+        if (!(tau[p] >= 0.0 && tau[p] <= 1e100)) return set_error(PCS_EINVAL, "tau[%u] outside [0, 1e100]", p);
+    for (size_t i = 0; i < (size_t)P * n_pop; ++i)
+        if (!(std::fabs(theta[i]) <= 1e100)) return set_error(PCS_EINVAL, "theta[%zu] not finite or above 1e100", i);
+    for (int k = 0; k < 3; ++k)
+        if (!(std::fabs(lo[k]) <= 1e100 && std::fabs(hi[k]) <= 1e100))
+            return set_error(PCS_EINVAL, "prior bounds above 1e100");
+    for (uint32_t i = 0; i < n_pop; ++i)
+        if (!std::isfinite(cum_w[i]) || (i && cum_w[i] < cum_w[i - 1]))
+            return set_error(PCS_EINVAL, "cum_w must be finite and non-decreasing (member %u)", i);
+    PCS_HIP(hipSetDevice(c->device));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    PCS_TRY(restore_slices(b, false));
+    const uint32_t nv = n_trials * J;
+    bool relayout = !b->dev_planned || n_trials != b->n_props || J != b->J;
+    if (n_trials > b->cap_props) {
+        PCS_TRY(dev_alloc(&b->d_props, n_trials));
+        PCS_TRY(dev_alloc(&b->d_count, n_trials));
+        PCS_TRY(dev_alloc(&b->d_tracks, n_trials));
+        PCS_TRY(dev_alloc(&b->d_dist, n_trials));
+        PCS_TRY(dev_alloc(&b->d_exact, n_trials));
+        PCS_TRY(dev_alloc(&b->d_status, n_trials));
+        b->cap_props = n_trials;
+        relayout = true;
+    }
+    if (n_trials > b->cap_in || !b->d_in) {
+        PCS_TRY(dev_alloc(&b->d_in, n_trials));
+        PCS_TRY(dev_alloc(&b->d_theta_new, (size_t)3u * PCS_MAX_SPECIES * n_trials));
+        b->cap_in = b->cap_new = n_trials;
+    }
+    if ((size_t)nv + 1u > b->cap_videos) {
+        PCS_TRY(dev_alloc(&b->d_vframes, (size_t)nv + 1u));
+        PCS_TRY(dev_alloc(&b->d_vparticles, (size_t)nv + 1u));
+        PCS_TRY(dev_alloc(&b->d_vprop, (size_t)nv + 1u));
+        PCS_TRY(dev_alloc(&b->d_vtile, (size_t)nv + 1u));
+        b->cap_videos = (size_t)nv + 1u;
+        relayout = true;
+    }
+    if (n_pop > b->cap_pop || !b->d_theta_pop) {
+        PCS_TRY(dev_alloc(&b->d_theta_pop, (size_t)3u * PCS_MAX_SPECIES * n_pop));
+        PCS_TRY(dev_alloc(&b->d_cumw, n_pop));
+        b->cap_pop = n_pop;
+    }
+    if (!b->d_totals) PCS_TRY(dev_alloc(&b->d_totals, 2));
+    b->dev_planned = true;
+    b->host_vtile = false;
+    b->J = J;
+    b->n_props = n_trials;
+    b->n_videos = nv;
+    b->ran = false;
+    // host side of the plan: proposal i owns videos [i J, (i + 1) J)
+    b->props.assign(n_trials, pcs_proposal{});
+    for (uint32_t i = 0; i < n_trials; ++i) {
+        b->props[i].sim_id = first_sim_id + i;
+        b->props[i].first_video = i * J;
+        b->props[i].n_videos = J;
+    }
+    b->hprops.clear();
+    b->vframes.clear();
+    b->vparticles.clear();
+    hipStream_t st = b->stream;
+    PCS_HIP(hipMemcpyAsync(b->d_theta_pop, theta, (size_t)P * n_pop * 8u, hipMemcpyHostToDevice, st));
+    PCS_HIP(hipMemcpyAsync(b->d_cumw, cum_w, (size_t)n_pop * 8u, hipMemcpyHostToDevice, st));
+    if (nv) PCS_HIP(hipMemcpyAsync(b->d_vframes, video_frames, (size_t)nv * 4u, hipMemcpyHostToDevice, st));
+    ProposeArgs pa{};
+    pa.theta = b->d_theta_pop;
+    pa.cum_w = b->d_cumw;
+    pa.theta_out = b->d_theta_new;
+    pa.props = b->d_in;
+    pa.vparticles = b->d_vparticles;
+    pa.n_pop = n_pop;
+    pa.n_trials = n_trials;
+    pa.J = J;
+    pa.C = C;
+    pa.Z = Z;
+    pa.first_sim_id = first_sim_id;
+    pa.seed = seed;
+    for (int k = 0; k < 3; ++k) {
+        pa.lo[k] = lo[k];
+        pa.hi[k] = hi[k];
+    }
+    pa.deltat = pr->deltat;
+    pa.volume = pr->volume;
+    for (uint32_t p = 0; p < P; ++p) pa.tau[p] = tau[p];
+    PCS_HIP(launch_propose(pa, st));
+    PCS_TRY(device_plan(b));
+    // The plan's record slices must fit the arena: grow it and plan again
+    // when they do not (the first call, or proposals that outgrow the last).
+    uint64_t tot[2] = {0, 0};
+    PCS_HIP(hipMemcpyAsync(tot, b->d_totals, sizeof(tot), hipMemcpyDeviceToHost, st));
+    PCS_HIP(hipStreamSynchronize(st));
+    if (tot[0] > b->cap_arena || !b->d_rec) {
+        const uint64_t cap = tot[0] + tot[0] / 8u + 1024u;
+        PCS_TRY(dev_alloc(&b->d_rec, cap));
+        PCS_TRY(dev_alloc(&b->d_srt, cap));
+        b->cap_arena = cap;
+        relayout = true;
+        PCS_TRY(device_plan(b));
+    }
+    b->last_need = tot[0];
+    b->arena_records = tot[0];
+    if (theta_out && n_trials) {
+        for (uint32_t p = 0; p < P; ++p)
+            PCS_HIP(hipMemcpyAsync(theta_out + (size_t)p * n_trials, b->d_theta_new + (size_t)p * n_trials,
+                                   (size_t)n_trials * 8u, hipMemcpyDeviceToHost, st));
+    }
+    PCS_HIP(hipStreamSynchronize(st));
+    if (relayout) destroy_graph(b);
+    return PCS_OK;
+}
+
+int pcs_batch_read_proposals(pcs_batch* b, pcs_proposal* props_out, uint32_t* particles_out)
+{
+    if (!b) return set_error(PCS_EINVAL, "batch is NULL");
+    PCS_HIP(hipSetDevice(b->ctx->device));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    if (b->dev_planned) {
+        if (props_out && b->n_props)
+            PCS_HIP(hipMemcpyAsync(props_out, b->d_in, (size_t)b->n_props * sizeof(pcs_proposal), hipMemcpyDeviceToHost,
+                                   b->stream));
+        if (particles_out && b->n_videos)
+            PCS_HIP(hipMemcpyAsync(particles_out, b->d_vparticles, (size_t)b->n_videos * 4u, hipMemcpyDeviceToHost,
+                                   b->stream));
+        PCS_HIP(hipStreamSynchronize(b->stream));
+        return PCS_OK;
+    }
+    if (props_out) std::copy(b->props.begin(), b->props.end(), props_out);
+    if (particles_out) std::copy(b->vparticles.begin(), b->vparticles.end(), particles_out);
     return PCS_OK;
 }
 

@@ -229,4 +229,34 @@ __device__ __forceinline__ float wrap_reflect(float v, float h)
     return __builtin_fmaf(2.0f, __builtin_amdgcn_fmed3f(v, -h, h), -v);
 }
 
+// ---------------------------------------------------------------- FP64 10^x
+// 10^x = 2^k 10^f, k = rint(x log2 10), f = x - k log10 2 with log10 2 split
+// into a 42-bit head (k * head exact for |k| < 2^11) and a tail; 10^f = e^y,
+// y = f ln 10 (|y| <= 0.35), by its Taylor series to degree 16 (remainder below
+// 1e-23 relative) in fma Horner form; within 2 ulp. Fixed operation order, so
+// the same on any IEEE-754 host.
+constexpr double kExp10Log2_10 = 0x1.a934f0979a371p+1;
+constexpr double kExp10Log10_2Hi = 0x1.34413509f7800p-2;
+constexpr double kExp10Log10_2Lo = 0x1.fef311f12b358p-46;
+constexpr double kExp10Ln10 = 0x1.26bb1bbb55516p+1;
+
+__device__ __forceinline__ double exp10_d(double x)
+{
+    constexpr double T[17] = {
+        0x1.0000000000000p+0, 0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5555555555555p-3,
+        0x1.5555555555555p-5, 0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+        0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
+        0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33, 0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-41,
+        0x1.ae7f3e733b81fp-45,
+    };
+    const double k = __builtin_rint(x * kExp10Log2_10);
+    double f = __builtin_fma(-k, kExp10Log10_2Hi, x);
+    f = __builtin_fma(-k, kExp10Log10_2Lo, f);
+    const double y = f * kExp10Ln10;
+    double p = T[16];
+#pragma unroll
+    for (int i = 15; i >= 0; --i) p = __builtin_fma(p, y, T[i]);
+    return __builtin_ldexp(p, (int)k);
+}
+
 }  // namespace pcs

@@ -95,6 +95,7 @@ struct WalkArgs {
     uint32_t* rec_count;        // per proposal, atomically bumped
     const uint64_t* seed;       // device-resident Philox key
     uint32_t* tile_counter;     // persistent form: next tile (relative to tile_lo)
+    const uint64_t* tile_total; // persistent form, nullable: tiles planned on the device (caps tile_hi)
 };
 
 struct DistArgs {
@@ -210,6 +211,26 @@ struct PrepArgs {
     uint64_t* totals;         // [0] planned records, [1] tiles
 };
 
+// Proposal step of an ABC-PMC round (propose_kernel), one wave per trial:
+// parent by cumulative weight, displaced parameters reflected into the prior,
+// components sorted by m, the trial's pcs_proposal record and Poisson particle
+// counts of its videos. Parameters are rows of P = 2C + Z: m_1..m_C,
+// log10 c_1..log10 c_C, az_1..az_Z.
+struct ProposeArgs {
+    const double* theta;       // P x n_pop
+    const double* cum_w;       // n_pop
+    const uint32_t* frames_in; // n_trials x J, trial-major
+    double* theta_out;         // P x n_trials
+    pcs_proposal* props;       // n_trials records
+    uint32_t* vframes;         // n_trials x J
+    uint32_t* vparticles;      // n_trials x J
+    uint32_t n_pop, n_trials, J, C, Z, first_sim_id;
+    uint64_t seed;
+    double lo[3], hi[3];       // prior bounds of m, log10 c, az
+    double deltat, volume;
+    double tau[3 * PCS_MAX_SPECIES];
+};
+
 // Launchers (pcs_kernels.hip). waves_per_group in {1,2,4}; ppl in {1,2}.
 hipError_t launch_reset(uint32_t* rec_count, uint32_t n, uint32_t* tile_counter, hipStream_t s);
 hipError_t launch_walk(const WalkArgs& a, int ppl, int waves_per_group, bool persistent,
@@ -228,6 +249,7 @@ hipError_t launch_serve_loop(const LoopArgs& a, uint32_t grid, uint32_t n_de_max
 size_t serve_loop_lds(const LoopArgs& a);
 hipError_t serve_loop_occupancy(const LoopArgs& a, uint32_t n_de_max, int* groups_per_cu);
 hipError_t launch_prepare(const PrepArgs& a, hipStream_t s);
+hipError_t launch_propose(const ProposeArgs& a, hipStream_t s);
 hipError_t launch_dense_hist(const DevProp* props, uint32_t prop_lo, uint32_t n_props,
                              const uint32_t* rec_count, uint32_t max_count,
                              uint32_t n_k, uint32_t n_de, uint32_t* hist, hipStream_t s);

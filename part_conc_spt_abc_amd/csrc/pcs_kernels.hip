@@ -560,11 +560,13 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
     const uint64_t seed = *a.seed;
     const uint32_t k0 = uniform((uint32_t)seed), k1 = uniform((uint32_t)(seed >> 32));
     if (PERSIST) {
+        // tiles planned on the device end the range where the plan says
+        const uint32_t hi = a.tile_total ? (uint32_t)min((uint64_t)a.tile_hi, *a.tile_total) : a.tile_hi;
         for (;;) {
             uint32_t t = 0u;
             if (lane_id() == 0u) t = atomicAdd(a.tile_counter, 1u);
             t = uniform(t) + a.tile_lo;
-            if (t >= a.tile_hi) break;
+            if (t >= hi) break;
             process_tile<PPL, GAUSS, ZREF, TRACKS>(a, t, st, tab, k0, k1);
         }
     } else {
@@ -606,7 +608,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, 
 
 // LDS working set of the distance of one simulation: row counts / row
 // offsets (n_k), one row's column histogram (n_de), the block-scan partials,
-// the deepest occupied row and the final reduction.
+// the deepest occupied row and the final reduction (kDistRed words).
 struct DistLds {
     uint32_t* rowc;
     uint32_t* colh;
@@ -614,13 +616,31 @@ struct DistLds {
     uint32_t* smax;
     uint64_t* red;
 };
+constexpr int kDistRed = 2 * (kDistThreads / kWave);
 
-// Result of distance_block, valid in thread 0.
+// Result of distance_block, valid in thread 0: the exact sum in 128 bits
+// (hi:lo), the records and the status.
 struct DistOut {
-    uint64_t exact;
+    uint64_t lo, hi;
     uint32_t tracks;
     int32_t status;
 };
+
+// 128-bit accumulation of the distance sum (it passes 2^64 when cumulative
+// counts approach 2^32): (hi:lo) += (bh:bl).
+__device__ __forceinline__ void add128(uint64_t& lo, uint64_t& hi, uint64_t bl, uint64_t bh)
+{
+    lo += bl;
+    hi += bh + (lo < bl ? 1ull : 0ull);
+}
+
+// Exact sum saturated at 2^64 - 1, and its log10 from the full 128 bits
+// (2^64 hi + lo rounded once to FP64).
+__device__ __forceinline__ uint64_t dist_exact(const DistOut& o) { return o.hi ? ~0ull : o.lo; }
+__device__ __forceinline__ double dist_log10(const DistOut& o)
+{
+    return o.status == PCS_OK ? log10(ldexp((double)o.hi, 64) + (double)o.lo) : __builtin_nan("");
+}
 
 // Exact distance of one simulation's record slice against the observed
 // summed-area table, by a whole 256-thread block. SEG * 256 >= n_de.
@@ -637,7 +657,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     uint32_t* srt = pr.srt;
 
     if (count > pr.rec_cap)  // record slice overflowed: counted, not stored
-        return DistOut{0ull, count, PCS_EOVERFLOW};
+        return DistOut{0ull, 0ull, count, PCS_EOVERFLOW};
     const uint32_t T = count;
     // Issue priority over walking waves that share the SIMDs: a distance is
     // one workgroup on the critical path of its simulation's latency.
@@ -710,7 +730,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     uint32_t Ccol[SEG];
 #pragma unroll
     for (int m = 0; m < SEG; ++m) Ccol[m] = 0u;
-    uint64_t acc = 0ull;
+    uint64_t acc = 0ull, acc_hi = 0ull;
     const bool l1 = sy.dist_kind == PCS_DIST_L1;
     const uint32_t* R = sy.obs_sat;
     const uint32_t maxrow_obs = sy.maxrow_obs;
@@ -729,7 +749,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
 #pragma unroll
     for (int d = 0; d < PF; ++d) load_row(rb[d], min((uint32_t)d, last));
 
-    uint64_t racc_prev = 0ull;
+    uint64_t racc_prev = 0ull, racc_prev_hi = 0ull;
     for (uint32_t i0 = 0; i0 <= last; i0 += PF) {
 #pragma unroll
         for (int d = 0; d < PF; ++d) {
@@ -740,7 +760,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
             // Past the deepest observed row R repeats its row; a row without
             // records repeats S's row, so it contributes what the row above did.
             if (s1 == s0 && i > maxrow_obs && i != last) {
-                acc += racc_prev;
+                add128(acc, acc_hi, racc_prev, racc_prev_hi);
             } else {
                 const uint32_t nr = s1 - s0;
                 if (nr > 0u && nr <= (uint32_t)kDistSparseRow) {
@@ -771,31 +791,47 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
                         if (j0 + m < n_de) colh[j0 + m] = 0u;
                     __syncthreads();
                 }
-                uint64_t racc = 0ull;
+                uint64_t racc = 0ull, racc_hi = 0ull;
 #pragma unroll
                 for (int m = 0; m < SEG; ++m) {
                     if (j0 + m < n_de) {
                         const uint32_t rv = rb[d][m];
                         const uint32_t dd = Ccol[m] > rv ? Ccol[m] - rv : rv - Ccol[m];
-                        racc += l1 ? (uint64_t)dd : (uint64_t)dd * dd;
+                        add128(racc, racc_hi, l1 ? (uint64_t)dd : (uint64_t)dd * dd, 0ull);
                     }
                 }
                 // Rows past `last` repeat row `last` in both tables.
-                acc += (i == last) ? racc * (uint64_t)(n_k - last) : racc;
+                if (i == last) {
+                    const uint64_t rep = (uint64_t)(n_k - last);
+                    add128(acc, acc_hi, racc * rep, __umul64hi(racc, rep) + racc_hi * rep);
+                } else {
+                    add128(acc, acc_hi, racc, racc_hi);
+                }
                 racc_prev = racc;
+                racc_prev_hi = racc_hi;
             }
             load_row(rb[d], min(i + (uint32_t)PF, last));
         }
     }
 
-    red[tid] = acc;
-    __syncthreads();
-    for (uint32_t s = kDistThreads / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
+    // 128-bit sum over the block: across each wave by shuffles, then the
+    // waves' partial sums in LDS.
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const uint64_t l2 = __shfl_xor(acc, o, kWave), h2 = __shfl_xor(acc_hi, o, kWave);
+        add128(acc, acc_hi, l2, h2);
     }
+    const uint32_t nw = kDistThreads / kWave;
+    if (lane == 0u) {
+        red[tid / kWave] = acc;
+        red[nw + tid / kWave] = acc_hi;
+    }
+    __syncthreads();
+    uint64_t lo = 0ull, hi = 0ull;
+    for (uint32_t w = 0; w < nw; ++w) add128(lo, hi, red[w], red[nw + w]);
+    __syncthreads();
     __builtin_amdgcn_s_setprio(0);
-    return DistOut{red[0], T, PCS_OK};
+    return DistOut{lo, hi, T, PCS_OK};
 }
 
 // Dynamic LDS: row counts (n_k_max words, rounded up to 4) then one row's
@@ -809,14 +845,14 @@ __global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
     uint32_t* colh = dist_lds + a.colh_off;
     __shared__ uint32_t wsum[kDistThreads / kWave];
     __shared__ uint32_t smax;
-    __shared__ uint64_t red[kDistThreads];
+    __shared__ uint64_t red[kDistRed];
 
     const uint32_t P = a.prop_lo + blockIdx.x;
     const DevProp& pr = a.props[P];
     const DistOut o = distance_block<SEG>(pr, a.sys[pr.sys_id], a.rec_count[P], DistLds{rowc, colh, wsum, &smax, red});
     if (threadIdx.x == 0) {  // an overflowed slice is rerun by the host
-        a.exact[P] = o.exact;
-        a.dist[P] = o.status == PCS_OK ? log10((double)o.exact) : __builtin_nan("");
+        a.exact[P] = dist_exact(o);
+        a.dist[P] = dist_log10(o);
         a.tracks[P] = o.tracks;
         a.status[P] = o.status;
     }
@@ -1048,7 +1084,7 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
     extern __shared__ uint32_t loop_lds[];
     __shared__ uint32_t wsum[kDistThreads / kWave];
     __shared__ uint32_t smax;
-    __shared__ uint64_t red[kDistThreads];
+    __shared__ uint64_t red[kDistRed];
     __shared__ uint64_t cmd_r;
     __shared__ uint32_t cmd[5];  // kind, t0, nt, last, sleep units
 
@@ -1183,8 +1219,8 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
             const DistOut o = distance_block<SEG>(pr, sy, count, L);
             if (tid == 0u) {
                 LoopDone& d = a.done[slot];
-                d.exact = o.exact;
-                d.dist = o.status == PCS_OK ? log10((double)o.exact) : __builtin_nan("");
+                d.exact = dist_exact(o);
+                d.dist = dist_log10(o);
                 d.tracks = o.tracks;
                 d.status = o.status;
                 __hip_atomic_store(&S.state, 4ull * r + 3ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -1296,6 +1332,172 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_kernel(PrepArgs a)
         a.totals[0] = total_rec;
         a.totals[1] = total_tiles;
         *a.tile_counter = 0u;
+    }
+}
+
+// ------------------------------------------------------- ABC-PMC proposals
+//
+// propose_kernel: one wave per trial t. Its random words are Philox4x32-10
+// keyed by the seed with counter (word, t, slot, kPropTag): the parent from
+// (kPropParent, t, 0), parameter p's normal from (p, t, 0) - an FP32
+// Box-Muller normal of the walk's math: radius of word x times the cosine of
+// the 16-bit angle code in the top of word y - and lane l's Poisson arrivals
+// for video v from (b, t, kPropPois | v << 6 | l), b = 0, 1, ...
+// (src/gpu/detail/simulation_impl.inl:256-341, src/cpu/estimate.jl:126-150).
+constexpr uint32_t kPropTag = 0x70726F70u;
+constexpr uint32_t kPropParent = 0xFFFFFFFFu;
+constexpr uint32_t kPropPois = 0x40000000u;
+constexpr int kPropReflectMax = 64;
+
+__device__ __forceinline__ U4 prop_block(uint32_t w0, uint32_t trial, uint32_t slot, uint32_t k0, uint32_t k1)
+{
+    return philox10(kPropTag, trial, w0, slot, k0, k1);  // counter (w0, trial, slot, tag)
+}
+
+// Reflection into [lo, hi] of displace_ (src/gpu/detail/simulation_impl.inl:66-79),
+// at most kPropReflectMax sweeps (a step of over 128 interval widths) and
+// then clamped, so that every wave ends.
+__device__ __forceinline__ double reflect_closed(double x, double lo, double hi)
+{
+    for (int i = 0; i < kPropReflectMax && (x < lo || x > hi); ++i) {
+        if (x < lo) x += 2.0 * (lo - x);
+        if (x > hi) x -= 2.0 * (x - hi);
+    }
+    return x < lo ? lo : (x > hi ? hi : x);
+}
+
+__global__ __launch_bounds__(256) void propose_kernel(ProposeArgs a)
+{
+    __shared__ float tabf[kAngleTabFloats];
+    load_angle_table(tabf);
+    __syncthreads();
+    const float2* tab = reinterpret_cast<const float2*>(tabf);
+    const uint32_t lane = lane_id();
+    const uint32_t t = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (t >= a.n_trials) return;  // wave-uniform
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const uint32_t C = a.C, Z = a.Z, P = 2u * C + Z;
+
+    // parent: the first member whose cumulative weight reaches u (wrand_idx_)
+    uint32_t parent;
+    {
+        const U4 w = prop_block(kPropParent, t, 0u, k0, k1);
+        const double u = ((double)(w.x >> 5) * 67108864.0 + (double)(w.y >> 6)) * 0x1p-53;
+        uint32_t lo = 0u, hi = a.n_pop;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.cum_w[mid] < u) lo = mid + 1u;
+            else hi = mid;
+        }
+        parent = lo < a.n_pop ? lo : a.n_pop - 1u;
+    }
+
+    // lane p < P: parameter p displaced by tau_p n and reflected into its bounds
+    double x = 0.0;
+    if (lane < P) {
+        const U4 w = prop_block(lane, t, 0u, k0, k1);
+        float sn, cs;
+        sincos_tab(tab, (w.y >> 24) & 0xffu, (w.y >> 16) & 0xffu, sn, cs);
+        const float n = bm_radius(w.x) * cs;
+        double tp = 0.0;
+#pragma unroll
+        for (uint32_t q = 0; q < 3u * PCS_MAX_SPECIES; ++q) tp = (lane == q) ? a.tau[q] : tp;
+        const uint32_t kind = lane < C ? 0u : (lane < 2u * C ? 1u : 2u);
+        const double blo = kind == 0u ? a.lo[0] : (kind == 1u ? a.lo[1] : a.lo[2]);
+        const double bhi = kind == 0u ? a.hi[0] : (kind == 1u ? a.hi[1] : a.hi[2]);
+        x = reflect_closed(a.theta[(uint64_t)lane * a.n_pop + parent] + tp * (double)n, blo, bhi);
+    }
+
+    // components sorted by m (stable): lane c < C ranks component c, and each
+    // row's lane takes its value from the component of its rank
+    uint32_t rank = 0u;
+#pragma unroll
+    for (uint32_t c2 = 0; c2 < PCS_MAX_SPECIES; ++c2) {
+        const double m2 = __shfl(x, (int)c2, kWave);
+        if (c2 < C && lane < C && (m2 < x || (m2 == x && c2 < lane))) ++rank;
+    }
+    uint32_t target = 0xffffffffu, base = 0u;
+    if (lane < C) {
+        target = lane;
+    } else if (lane < 2u * C) {
+        target = lane - C;
+        base = C;
+    } else if (Z > 1u && lane < P) {
+        target = lane - 2u * C;
+        base = 2u * C;
+    }
+    uint32_t src = lane;
+#pragma unroll
+    for (uint32_t c2 = 0; c2 < PCS_MAX_SPECIES; ++c2) {
+        const uint32_t r2 = __shfl(rank, (int)c2, kWave);
+        if (c2 < C && r2 == target) src = base + c2;
+    }
+    const double y = __shfl(x, (int)src, kWave);
+    if (lane < P) a.theta_out[(uint64_t)lane * a.n_trials + t] = y;
+
+    // per-trial precompute (src/gpu/detail/simulation_impl.inl:306-341)
+    double mq[PCS_MAX_SPECIES], cq[PCS_MAX_SPECIES], aq[PCS_MAX_SPECIES];
+#pragma unroll
+    for (uint32_t c = 0; c < PCS_MAX_SPECIES; ++c) {
+        mq[c] = __shfl(y, (int)c, kWave);
+        cq[c] = __shfl(y, (int)((C + c) & 63u), kWave);
+        aq[c] = __shfl(y, (int)((2u * C + (Z > 1u ? c : 0u)) & 63u), kWave);
+    }
+    double csum = 0.0;
+#pragma unroll
+    for (uint32_t c = 0; c < PCS_MAX_SPECIES; ++c) {
+        cq[c] = c < C ? exp10_d(cq[c]) : 0.0;
+        if (c < C) csum += cq[c];
+    }
+    // lane w < 28 writes word w of the trial's pcs_proposal record
+    uint32_t word = 0u;
+    if (lane == 0u) {
+        word = a.first_sim_id + t;
+    } else if (lane == 1u) {
+        word = t * a.J;
+    } else if (lane == 2u) {
+        word = a.J;
+    } else if (lane >= 4u && lane < 28u) {
+        const uint32_t f = (lane - 4u) / PCS_MAX_SPECIES, i = (lane - 4u) % PCS_MAX_SPECIES;
+        float v = 0.0f;
+        double acc = 0.0;
+#pragma unroll
+        for (uint32_t c = 0; c < PCS_MAX_SPECIES; ++c) {
+            if (c < C) acc += cq[c] / csum;
+            if (c == i && c < C) {
+                if (f == 0u) v = 0.5f * (float)aq[c];
+                else if (f == 1u) v = (c + 1u == C) ? 2.0f : (float)acc;
+                else v = sqrtf((float)(2.0 * mq[c] * a.deltat));
+            }
+        }
+        word = __float_as_uint(v);
+    }
+    if (lane < 28u) reinterpret_cast<uint32_t*>(a.props + t)[lane] = word;
+
+    // per video: Poisson(lambda) = sum over the lanes of the arrivals of a
+    // unit-rate Poisson process on [0, lambda / 64]
+    const double mu = csum * a.volume / 64.0;
+    for (uint32_t v = 0; v < a.J; ++v) {
+        const uint32_t slot = kPropPois | (v << 6) | lane;
+        double s = 0.0;
+        uint32_t k = 0u;
+        for (uint32_t blk = 0;; ++blk) {
+            const U4 w = prop_block(blk, t, slot, k0, k1);
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+            bool done = false;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (!done) {
+                    s += (double)(0.5f * neg2log_u24(ws[j]));
+                    if (!(s <= mu)) done = true;  // also ends on a NaN
+                    else ++k;
+                }
+            }
+            if (done) break;
+        }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) k += __shfl_xor(k, o, kWave);
+        if (lane == 0u) a.vparticles[(uint64_t)t * a.J + v] = k;
     }
 }
 
@@ -1538,6 +1740,14 @@ hipError_t launch_prepare(const PrepArgs& a, hipStream_t s)
 {
     (void)hipGetLastError();  // report only this launch's error
     hipLaunchKernelGGL(prepare_kernel, dim3(1), dim3(kPrepThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_propose(const ProposeArgs& a, hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launch's error
+    if (a.n_trials == 0) return hipSuccess;
+    hipLaunchKernelGGL(propose_kernel, dim3((a.n_trials + 3u) / 4u), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

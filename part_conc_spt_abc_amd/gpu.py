@@ -113,6 +113,33 @@ class Batch:
                                            _ptr(self._particles), len(self._frames)), "pcs_batch_load")
         self.n = len(props)
 
+    def propose(self, prior: _abi.pcs_prior, theta: np.ndarray, cum_w: np.ndarray, tau: np.ndarray, n_trials: int,
+                frames: np.ndarray, seed: int, first_sim_id: int = 0, want_theta: bool = True):
+        """Proposal step of an ABC-PMC round on the GPU (pcs_batch_propose):
+        theta (P x n_pop) the population, cum_w its cumulative weights, tau the
+        displacement scales; frames (n_trials x J) the trials' video lengths.
+        Returns the proposed parameters (P x n_trials) or None."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        cw = np.ascontiguousarray(cum_w, dtype=np.float64)
+        ta = np.ascontiguousarray(tau, dtype=np.float64)
+        fr = np.ascontiguousarray(frames, dtype=np.uint32).reshape(n_trials, -1)
+        out = np.empty((th.shape[0], n_trials), np.float64) if want_theta else None
+        _abi.check(self.lib.pcs_batch_propose(self.h, C.byref(prior), _ptr(th), _ptr(cw), _ptr(ta), th.shape[1],
+                                              n_trials, _ptr(fr), fr.shape[1], first_sim_id, C.c_uint64(seed),
+                                              _ptr(out)), "pcs_batch_propose")
+        self._frames = fr.ravel()
+        self.n = n_trials
+        return out
+
+    def read_proposals(self):
+        """(pcs_proposal records, video particle counts) of the loaded or
+        proposed batch."""
+        recs = (_abi.pcs_proposal * self.n)()
+        parts = np.empty(len(self._frames), np.uint32)
+        _abi.check(self.lib.pcs_batch_read_proposals(self.h, C.cast(recs, C.c_void_p), _ptr(parts)),
+                   "pcs_batch_read_proposals")
+        return list(recs), parts
+
     def run(self, seed: int, mode: int = _abi.PCS_MODE_BATCHED):
         _abi.check(self.lib.pcs_batch_run(self.h, C.c_uint64(seed), mode), "pcs_batch_run")
 

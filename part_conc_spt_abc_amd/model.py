@@ -110,3 +110,13 @@ def proposals_array(records: Sequence[_abi.pcs_proposal]):
     for i, r in enumerate(records):
         arr[i] = r
     return arr
+
+
+def prior_record(model: Model, m_bounds, log10c_bounds, az_bounds, fixed_depth: bool = True) -> _abi.pcs_prior:
+    """Prior of the device proposal step for a model: parameter rows m_1..m_C,
+    log10 c_1..log10 c_C, az (one shared depth when fixed, else one per
+    component)."""
+    C = model.n_species
+    return _abi.pcs_prior(C, 1 if fixed_depth else C, float(m_bounds[0]), float(m_bounds[1]),
+                          float(log10c_bounds[0]), float(log10c_bounds[1]), float(az_bounds[0]),
+                          float(az_bounds[1]), float(model.deltat), float(model.volume_factor))

@@ -46,6 +46,21 @@ class orc_proposal(C.Structure):
     ]
 
 
+class orc_prior(C.Structure):
+    _fields_ = [
+        ("n_components", C.c_uint32),
+        ("n_depths", C.c_uint32),
+        ("m_lo", C.c_double),
+        ("m_hi", C.c_double),
+        ("log10c_lo", C.c_double),
+        ("log10c_hi", C.c_double),
+        ("az_lo", C.c_double),
+        ("az_hi", C.c_double),
+        ("deltat", C.c_double),
+        ("volume", C.c_double),
+    ]
+
+
 class orc_proposal_f64(C.Structure):
     _fields_ = [("half_az", D8), ("cum_prob", D8), ("sigma", D8)]
 
@@ -94,6 +109,12 @@ def lib():
         L.orc_bm_angle16.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_bm_angle16.restype = C.c_uint32
         L.orc_chunk_normals_n.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, vp]
+        L.orc_exp10.argtypes = [C.c_double]
+        L.orc_exp10.restype = C.c_double
+        L.orc_poisson.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]
+        L.orc_poisson.restype = C.c_uint32
+        L.orc_propose.argtypes = [C.POINTER(orc_prior), vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                  vp, vp, vp, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -217,3 +238,28 @@ def reference_hist(K, DE, ub_de, n_k, n_de):
     dropped = np.zeros(1, dtype=np.uint64)
     n = lib().orc_reference_hist(_p(K), _p(DE), len(K), ub_de, n_k, n_de, _p(H), _p(dropped))
     return H, int(n), int(dropped[0])
+
+
+def exp10(x: float) -> float:
+    return float(lib().orc_exp10(float(x)))
+
+
+def poisson(lam: float, seed: int, trial: int, video: int = 0) -> int:
+    return int(lib().orc_poisson(float(lam), seed, trial, video))
+
+
+def propose(prior: orc_prior, theta, cum_w, tau, n_trials: int, J: int, seed: int):
+    """Proposal step: returns dict(theta (P, n_trials), half_az / cum_prob /
+    sigma (n_trials, 8), particles (n_trials, J), parent (n_trials,))."""
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    cum_w = np.ascontiguousarray(cum_w, dtype=np.float64)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    P, n_pop = theta.shape
+    out = dict(theta=np.zeros((P, n_trials)), half_az=np.zeros((n_trials, MAX_SPECIES), np.float32),
+               cum_prob=np.zeros((n_trials, MAX_SPECIES), np.float32),
+               sigma=np.zeros((n_trials, MAX_SPECIES), np.float32),
+               particles=np.zeros((n_trials, J), np.uint32), parent=np.zeros(n_trials, np.uint32))
+    lib().orc_propose(C.byref(prior), _p(theta), _p(cum_w), _p(tau), n_pop, n_trials, J, seed, _p(out["theta"]),
+                      _p(out["half_az"]), _p(out["cum_prob"]), _p(out["sigma"]), _p(out["particles"]),
+                      _p(out["parent"]))
+    return out

@@ -22,18 +22,23 @@ ROOT = pathlib.Path(__file__).resolve().parents[1]
 PCS_ABC = ROOT / "part_conc_spt_abc_amd" / "lib" / "pcs_abc"
 
 
-def run_abc(inp, out, mode, seed=7, steps=60):
+def run_abc(inp, out, mode, seed=7, steps=60, extra=()):
     r = subprocess.run([str(PCS_ABC), "-i", str(inp), "-o", str(out), "-F", str(seed), "-S", str(steps), "--mode",
-                        mode], capture_output=True, text=True, timeout=600)
+                        mode, *extra], capture_output=True, text=True, timeout=600)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0, r.stderr
     return read_output(out)
 
 
-def test_abc_recovers_truth(tmp_path):
+@pytest.mark.parametrize("proposals", ["host", "device"])
+def test_abc_recovers_truth(tmp_path, proposals):
+    """Proposals drawn on the host, or on the GPU (pcs_batch_propose) in every
+    round, with the launch shape autotuned on the first round."""
     inp, ntracks, _ = make_data(tmp_path)
     assert ntracks > 100
-    txt, vec, scal = run_abc(inp, tmp_path / "out.xml", "batched")
+    extra = ("--device-propose-min", "1", "--autotune-min", "64", "-v") if proposals == "device" else (
+        "--device-propose-min", "0")
+    txt, vec, scal = run_abc(inp, tmp_path / "out.xml", "batched", extra=extra)
     m, c, w = vec("m").reshape(-1, 2), vec("c").reshape(-1, 2), vec("w")
     assert m.shape == (128, 2) and abs(w.sum() - 1.0) < 1e-9
     assert np.all(m[:, 0] <= m[:, 1])  # components ordered by m
@@ -48,8 +53,9 @@ def test_plain_and_batched_forms_give_identical_estimates(tmp_path):
     batch per round): every simulation gives the same distance, so the
     whole ABC run, and its output, is identical."""
     inp, _, _ = make_data(tmp_path, weighting="pmc-standard", trials=4, samples=48)
-    a, va, sa = run_abc(inp, tmp_path / "a.xml", "batched", seed=3, steps=6)
-    b, vb, sb = run_abc(inp, tmp_path / "b.xml", "plain", seed=3, steps=6)
-    for k in ("m", "c", "az", "dist", "w"):
-        assert np.array_equal(va(k), vb(k)), k
-    assert sa("gamma") == sb("gamma")
+    for proposals in ("0", "1"):  # drawn on the host, drawn on the GPU
+        a, va, sa = run_abc(inp, tmp_path / "a.xml", "batched", seed=3, steps=6, extra=("--device-propose-min", proposals))
+        b, vb, sb = run_abc(inp, tmp_path / "b.xml", "plain", seed=3, steps=6, extra=("--device-propose-min", proposals))
+        for k in ("m", "c", "az", "dist", "w"):
+            assert np.array_equal(va(k), vb(k)), (proposals, k)
+        assert sa("gamma") == sb("gamma")

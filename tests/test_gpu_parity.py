@@ -445,3 +445,26 @@ def test_pmc_weights_on_device(gpu_lib):
     ws /= ws.sum()
     assert np.allclose(out, ws, rtol=1e-11, atol=0)
     assert abs(out.sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("mode", [_abi.PCS_MODE_BATCHED, _abi.PCS_MODE_PLAIN])
+def test_distance_sum_past_two_to_the_64(gpu_lib, mode):
+    """Observed counts near 2^32 in every cumulative cell: the sum of squares
+    passes 2^64. The exact value saturates at 2^64 - 1 and the log10 distance
+    is the oracle's, taken from the full 128-bit sum (checked against Python
+    integers)."""
+    model = cases.cpu_model()
+    obs = np.zeros((model.n_k, model.n_de), np.uint32)
+    obs[0, 0] = 4_000_000_000
+    obs[5, 7] = 150_000_000
+    p, f, n = cases.fixed_proposal(model, [600], [300], sim_id=3)
+    with Context(model, obs) as ctx:
+        dist, tracks, H, exact, _ = run_gpu(ctx, [p], f, n, 1, mode)
+    Ho, _, _ = O.simulate_f32(O.system_from_abi(model.to_abi()), O.OracleProposal(p, f, n), 1)
+    assert np.array_equal(H[0], Ho)
+    ex, lg = O.distance(Ho, obs, 0)
+    S = np.cumsum(np.cumsum(Ho.astype(object), axis=0), axis=1)
+    T = np.cumsum(np.cumsum(obs.astype(object), axis=0), axis=1)
+    want = sum(int(v) * int(v) for v in (S - T).ravel())
+    assert want > 2 ** 64 and ex == 2 ** 64 - 1 and int(exact[0]) == ex
+    assert dist[0] == pytest.approx(lg, rel=1e-15) and lg == pytest.approx(math.log10(want), rel=1e-15)

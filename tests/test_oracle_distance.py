@@ -4,6 +4,7 @@ builder (src/gpu/detail/simulation_impl.inl:605-616) against textbook numpy
 and the reference's own test_files fixture.
 """
 import hashlib
+import math
 import json
 import pathlib
 import re
@@ -83,3 +84,26 @@ def test_observed_histogram_from_test_files():
     S = np.zeros_like(H)
     O.lib().orc_sat_u32(H.ctypes.data, 1000, n_de, S.ctypes.data)
     assert S[-1, -1] == 36701
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_distance_sum_past_two_to_the_64(kind):
+    """A sum of squares past 2^64 (observed counts near 2^32 in every
+    cumulative cell): the exact value saturates at 2^64 - 1 and the log10
+    distance is that of the exact (Python integer) sum."""
+    rng = np.random.default_rng(5)
+    shape = (30, 20)
+    R = np.zeros(shape, np.uint32)
+    R[0, 0] = 4_000_000_000
+    R[3, 5] = 200_000_000
+    H = ((rng.random(shape) < 0.2) * rng.integers(0, 50, shape)).astype(np.uint32)
+    S = np.cumsum(np.cumsum(H.astype(object), axis=0), axis=1)
+    T = np.cumsum(np.cumsum(R.astype(object), axis=0), axis=1)
+    d = S - T
+    want = int(sum(int(x) * int(x) for x in d.ravel())) if kind == 0 else int(sum(abs(int(x)) for x in d.ravel()))
+    v, lg = O.distance(H, R, kind)
+    if kind == 0:
+        assert want > 2 ** 64 and v == 2 ** 64 - 1
+    else:
+        assert v == want
+    assert lg == pytest.approx(math.log10(want), rel=1e-15)
