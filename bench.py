@@ -41,8 +41,8 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 from part_conc_spt_abc_amd import _abi  # noqa: E402
-from part_conc_spt_abc_amd.gpu import Context, Server  # noqa: E402
-from part_conc_spt_abc_amd.model import Model, proposal_record, stokes_einstein  # noqa: E402
+from part_conc_spt_abc_amd.gpu import Context, Server, pmc_weights  # noqa: E402
+from part_conc_spt_abc_amd.model import Model, prior_record, proposal_record, stokes_einstein  # noqa: E402
 
 # The walk is bound by VALU issue. Its roofline counts the work the algorithm
 # needs per particle-step for the random stream it draws, one lane-operation
@@ -358,6 +358,57 @@ def cpu_batched_sample(model, props, frames, parts, obs, n_sample, threads):
             "seconds": dt}
 
 
+def generation_population(model, n, rng, frames_choices=(500, 1000, 2000)):
+    """Population that the batched generation proposes from: n members with
+    total concentration log-uniform on 1e8..1e10 /mL split 50/50 between the
+    500 nm and 175 nm species (rows m_1 <= m_2, log10 c_1, log10 c_2, depth
+    2 um), equal weights, and tau = sqrt(2 var) of each row as the ABC-PMC
+    driver sets it (0 for the rows without spread). Each trial has one video
+    of U{500, 1000, 2000} frames. The reflected displacement keeps each
+    log10 c row uniform on its prior interval, so the trials' particle counts
+    span 120..12,000 as in the resident batched workload."""
+    ct = 10.0 ** rng.uniform(8.0, 10.0, n)
+    lc = np.log10(ct / 2.0)
+    theta = np.vstack([np.full(n, D500), np.full(n, D175), lc, lc, np.full(n, 2.0)])
+    w = np.full(n, 1.0 / n)
+    tau = np.sqrt(2.0 * theta.var(axis=1))
+    frames = rng.choice(frames_choices, n).astype(np.uint32).reshape(n, 1)
+    prior = prior_record(model, (0.1, 5.0), (math.log10(0.5e8), math.log10(0.5e10)), (0.5, 4.0))
+    return theta, w, tau, frames, prior
+
+
+def run_generations(batch, pop, steps, warmup, seed0=1000):
+    """Timed ABC-PMC generations on the GPU. One step: the cumulative weights
+    of the population, the proposal step of all its trials on the GPU
+    (parents, displacements, Poisson counts, plan; the proposed parameters
+    come back to the host), the walk and distance of the trials as one
+    hipGraph launch, the read-back of the distances, and the trials'
+    pmc-standard importance weights against the population on the GPU
+    (n x n kernel sums over the parameters with spread). The population is
+    the same at every step, so every step does the same work. Returns the
+    wall times, per-step (propose, run + read, weights) ms and particle-steps."""
+    theta, w, tau, frames, prior = pop
+    n = theta.shape[1]
+    keep = tau > 0
+    walls, parts, psteps = [], [], []
+    for i in range(warmup + steps):
+        t0 = time.perf_counter()
+        cum = np.cumsum(w)
+        th = batch.propose(prior, theta, cum, tau, n, frames, seed0 + i)
+        t1 = time.perf_counter()
+        batch.run(seed0 + i, _abi.PCS_MODE_BATCHED)
+        batch.read()
+        t2 = time.perf_counter()
+        pmc_weights(th[keep], theta[keep], w, tau[keep])
+        t3 = time.perf_counter()
+        if i >= warmup:
+            walls.append(t3 - t0)
+            parts.append(((t1 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3))
+            _, vp = batch.read_proposals()
+            psteps.append(particle_steps(frames.ravel(), vp))
+    return walls, parts, psteps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -400,7 +451,7 @@ def main():
     t_pack = time.perf_counter()
     props, frames, parts = workload(model, args.batched_sims, rng, (500, 1000, 2000))
     t_pack = time.perf_counter() - t_pack
-    steps_per_gen = particle_steps(frames, parts)
+    steps_resident = particle_steps(frames, parts)
     tuning = None
     if args.tuning:
         shape = [int(x) for x in args.tuning.split(",")]
@@ -413,27 +464,43 @@ def main():
         k = min(len(props), 4000)
         tuning, tms = ctx.autotune(props[:k], frames[:k], parts[:k])
         log(f"autotune: {tuning} ({tms:.2f} ms on {k} proposals)")
+
+    # The headline step: one ABC-PMC generation of args.batched_sims trials,
+    # everything on the GPU.
+    pop = generation_population(model, args.batched_sims, np.random.default_rng(11))
+    gb = ctx.batch()
+    walls, gparts, gsteps = run_generations(gb, pop, args.steps, args.warmup)
+    reruns = gb.overflow_reruns()
+    wall = sum(walls)
+    sims_s = args.batched_sims * args.steps / wall
+    psteps_s = sum(gsteps) / wall
+    steps_per_gen = float(np.mean(gsteps))
+    part_ms = np.mean(np.array(gparts), axis=0)
+    # The last generation's trials again with the three batched kernels
+    # launched without the graph, HIP events between them on the batch
+    # stream: the walk kernel's duration.
+    dwalls, total_ms, walk_ms = run_timed(gb, args.steps, 0, _abi.PCS_MODE_BATCHED_DIRECT, 100)
+    _, gtracks, _ = gb.read()
+    arena_cap, arena_bytes = gb.arena()
+    gb.close()
+    arena_use = float(gtracks.astype(np.float64).sum()) / max(arena_cap, 1)
+    walk_s = float(np.mean(walk_ms)) / 1e3 if min(walk_ms) > 0 else float("nan")
+    log(f"generation: {sims_s:.0f} sims/s, {psteps_s:.3e} particle-steps/s, {np.mean(walls)*1e3:.1f} ms per "
+        f"generation (propose + plan {part_ms[0]:.1f} ms, walk + distance + read {part_ms[1]:.1f} ms, pmc weights "
+        f"{part_ms[2]:.1f} ms), {steps_per_gen:.3e} particle-steps per generation, walk kernel {walk_s*1e3:.1f} ms, "
+        f"reruns {reruns}")
+
+    # Beside it: the same kind of generation with its proposals packed on the
+    # host and resident in HBM before timing (walk + distance + read only).
     b = ctx.batch()
     t_load = time.perf_counter()
     b.load(props, frames, parts)
     t_load = time.perf_counter() - t_load
-    arena_cap, arena_bytes = b.arena()
-    log(f"batched: {args.batched_sims} proposals, {steps_per_gen:.3e} particle-steps per generation, "
-        f"host packing {t_pack:.2f} s, load {t_load * 1e3:.1f} ms, arena {arena_bytes / 2**30:.2f} GiB")
-    walls, total_ms, _ = run_timed(b, args.steps, args.warmup, _abi.PCS_MODE_BATCHED, 100)
-    reruns = b.overflow_reruns()
-    # The same three kernels launched without the graph, with HIP events
-    # between them on the batch stream: the walk kernel's duration.
-    dwalls, _, walk_ms = run_timed(b, args.steps, 0, _abi.PCS_MODE_BATCHED_DIRECT, 100)
-    dist, tracks, _ = b.read()
+    rwalls, rtotal_ms, _ = run_timed(b, args.steps, args.warmup, _abi.PCS_MODE_BATCHED, 100)
     b.close()
-    arena_use = float(tracks.astype(np.float64).sum()) / max(arena_cap, 1)
-    wall = sum(walls)
-    sims_s = args.batched_sims * args.steps / wall
-    psteps_s = steps_per_gen * args.steps / wall
-    walk_s = float(np.mean(walk_ms)) / 1e3 if min(walk_ms) > 0 else float("nan")
-    log(f"batched: {sims_s:.0f} sims/s, {psteps_s:.3e} particle-steps/s, walk kernel {walk_s*1e3:.1f} ms, "
-        f"graph {np.mean(total_ms):.1f} ms, wall {np.mean(walls)*1e3:.1f} ms, reruns {reruns}")
+    resident_s = args.batched_sims * args.steps / sum(rwalls)
+    log(f"resident batch: {resident_s:.0f} sims/s ({steps_resident:.3e} particle-steps), host packing "
+        f"{t_pack:.2f} s, load {t_load * 1e3:.1f} ms, graph {np.mean(rtotal_ms):.1f} ms")
     if args.calibrate_copy:
         import ctypes as C
 
@@ -461,7 +528,7 @@ def main():
         b2.close()
         c2.close()
         other_z = {"z_boundary": oz, "sims_per_s": args.batched_sims * args.steps / sum(w2),
-                   "particle_steps_per_s": steps_per_gen * args.steps / sum(w2),
+                   "particle_steps_per_s": steps_resident * args.steps / sum(w2),
                    "walk_kernel_ms": float(np.mean(wk2))}
         log(f"batched, z {oz}: {other_z}")
 
@@ -517,9 +584,10 @@ def main():
     # rocprofv3 runs (copy-calibrated FETCH_SIZE / WRITE_SIZE).
     prof = json.loads(PMC_PROFILE.read_text())
     ops_step = walk_ops_per_step(gauss=False)
-    ach = steps_per_gen * ops_step / walk_s if walk_s == walk_s else None
-    issue = steps_per_gen * prof["valu_lane_instr_per_particle_step"] / walk_s if walk_s == walk_s else None
-    traffic = prof["hbm_bytes_per_particle_step"] * steps_per_gen
+    # the walk timed on the last generation's trials
+    ach = gsteps[-1] * ops_step / walk_s if walk_s == walk_s else None
+    issue = gsteps[-1] * prof["valu_lane_instr_per_particle_step"] / walk_s if walk_s == walk_s else None
+    traffic = prof["hbm_bytes_per_particle_step"] * gsteps[-1]
     ratio = (sims_s / plain["sims_per_s"]) if plain else None
     ratio1 = (sims_s / plain["one_stream"]["sims_per_s"]) if plain else None
     line = {
@@ -536,10 +604,17 @@ def main():
         "dtype": "f32+u32",
         "data": "synthetic",
         "config": {
-            "workload": f"batched ABC generation: {args.batched_sims} proposals, c log-uniform 1e8-1e10/mL "
-                        f"(120-12,000 particles), frames U{{500,1000,2000}}, box 200x200x30 um "
+            "workload": f"batched ABC-PMC generation on the GPU: {args.batched_sims} trials proposed from a "
+                        f"resident population (total c log-uniform 1e8-1e10/mL split between 175/500 nm, "
+                        f"120-12,000 particles, tau = sqrt(2 var)), frames U{{500,1000,2000}}, box 200x200x30 um "
                         f"{'periodic' if args.z_boundary == 'periodic' else 'periodic in x/y, reflecting in z'}, "
-                        f"2 species 175/500 nm, H 1000x500 uint32, one hipGraph launch per generation",
+                        f"H 1000x500 uint32; step = proposal step + plan, walk + distance in one hipGraph launch, "
+                        f"distances read back, pmc-standard weights of the trials against the population",
+            "step_propose_ms": float(part_ms[0]),
+            "step_walk_distance_read_ms": float(part_ms[1]),
+            "step_weights_ms": float(part_ms[2]),
+            "resident_batch_sims_per_s": resident_s,
+            "resident_batch_particle_steps_per_s": steps_resident * args.steps / sum(rwalls),
             "z_boundary": args.z_boundary,
             "particle_steps_per_generation": steps_per_gen,
             "particle_steps_per_s": psteps_s,
@@ -550,6 +625,7 @@ def main():
             "plain_one_stream_sims_per_s": plain["one_stream"]["sims_per_s"] if plain else None,
             "batched_vs_plain": ratio,
             "batched_vs_plain_one_stream": ratio1,
+            "resident_vs_plain": (resident_s / plain["sims_per_s"]) if plain else None,
             # headline latency: the device-resident loop at the second swept rate (5,000 req/s by
             # default), from each request's scheduled arrival
             "p50_ms": pick(serve_dev, "p50_ms"),
@@ -565,8 +641,8 @@ def main():
             "load_ms": t_load * 1e3,
             "arena_bytes": arena_bytes,
             "arena_use": arena_use,
-            "ms_per_step_graph_device": float(np.mean(total_ms)),
-            "ms_per_step_direct": float(np.mean(dwalls)) * 1e3,
+            "resident_graph_device_ms": float(np.mean(rtotal_ms)),
+            "direct_launch_wall_ms": float(np.mean(dwalls)) * 1e3,
             "parallelism": "single GPU",
             "tuning": tuning,
             "other_z_boundary": other_z,

@@ -189,6 +189,21 @@ class Batch:
             pass
 
 
+def pmc_weights(theta_new: np.ndarray, theta_old: np.ndarray, w_old: np.ndarray, tau: np.ndarray,
+                device: int = 0) -> np.ndarray:
+    """pmc-standard importance weights on the GPU (pcs_pmc_weights): rows are
+    parameters, columns members; returns the normalised new weights."""
+    lib = _abi.load()
+    xn = np.ascontiguousarray(theta_new, dtype=np.float64)
+    xo = np.ascontiguousarray(theta_old, dtype=np.float64)
+    wo = np.ascontiguousarray(w_old, dtype=np.float64)
+    ta = np.ascontiguousarray(tau, dtype=np.float64)
+    out = np.empty(xn.shape[1], np.float64)
+    _abi.check(lib.pcs_pmc_weights(device, _ptr(xn), xn.shape[1], _ptr(xo), _ptr(wo), xo.shape[1], _ptr(ta),
+                                   len(ta), _ptr(out)), "pcs_pmc_weights")
+    return out
+
+
 def selftest_math(words: np.ndarray, device: int = 0):
     lib = _abi.load()
     w = np.ascontiguousarray(words, dtype=np.uint32)
