@@ -1382,21 +1382,22 @@ int pcs_pmc_weights(int device, const double* theta_new, uint32_t n_new, const d
         for (uint32_t j = 0; j < n_new; ++j) xn[(size_t)p * n_new + j] = theta_new[(size_t)p * n_new + j] / tau[p];
         for (uint32_t i = 0; i < n_old; ++i) xo[(size_t)p * n_old + i] = theta_old[(size_t)p * n_old + i] / tau[p];
     }
-    double *d_xn = nullptr, *d_xo = nullptr, *d_w = nullptr, *d_s = nullptr;
+    double *d_xn = nullptr, *d_xo = nullptr, *d_w = nullptr, *d_s = nullptr, *d_part = nullptr;
     int r = dev_alloc(&d_xn, xn.size());
     if (r == PCS_OK) r = dev_alloc(&d_xo, xo.size() + 1);
     if (r == PCS_OK) r = dev_alloc(&d_w, (size_t)n_old + 1);
     if (r == PCS_OK) r = dev_alloc(&d_s, n_new);
+    if (r == PCS_OK) r = dev_alloc(&d_part, (size_t)pmc_chunks(n_old) * n_new);
     if (r == PCS_OK) {
         hipError_t e = hipMemcpy(d_xn, xn.data(), xn.size() * 8, hipMemcpyHostToDevice);
         if (e == hipSuccess && n_old) e = hipMemcpy(d_xo, xo.data(), xo.size() * 8, hipMemcpyHostToDevice);
         if (e == hipSuccess && n_old) e = hipMemcpy(d_w, w_old, (size_t)n_old * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = launch_pmc_sums(d_xn, n_new, d_xo, d_w, n_old, n_params, d_s, nullptr);
+        if (e == hipSuccess) e = launch_pmc_sums(d_xn, n_new, d_xo, d_w, n_old, n_params, d_part, d_s, nullptr);
         if (e == hipSuccess) e = hipDeviceSynchronize();
         if (e == hipSuccess) e = hipMemcpy(w_out, d_s, (size_t)n_new * 8, hipMemcpyDeviceToHost);
         if (e != hipSuccess) r = hip_error(e, "pcs_pmc_weights");
     }
-    void* ptrs[] = {d_xn, d_xo, d_w, d_s};
+    void* ptrs[] = {d_xn, d_xo, d_w, d_s, d_part};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (r != PCS_OK) return r;

@@ -238,8 +238,10 @@ hipError_t launch_walk(const WalkArgs& a, int ppl, int waves_per_group, bool per
 hipError_t launch_distance(const DistArgs& a, uint32_t n_k_max, uint32_t n_de_max, hipStream_t s);
 // Track-list form (k, DE) of the walk, one wave per tile, tiles of 64 particles.
 hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s);
+// pmc kernel sums: `part` holds pmc_chunks(n_old) x n_new partial sums.
+uint32_t pmc_chunks(uint32_t n_old);
 hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, const double* w_old, uint32_t n_old,
-                           uint32_t P, double* wsum, hipStream_t s);
+                           uint32_t P, double* part, double* wsum, hipStream_t s);
 hipError_t launch_selftest_math(const uint32_t* w, float* radius, float* s, float* c, uint64_t n, hipStream_t st);
 hipError_t launch_selftest_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st);
 hipError_t launch_selftest_sqrt(uint32_t lo, uint32_t n, unsigned long long* bad, hipStream_t st);

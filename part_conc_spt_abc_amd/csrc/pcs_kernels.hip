@@ -1548,53 +1548,101 @@ __global__ void selftest_copy_kernel(const uint4* __restrict__ src, uint4* __res
         dst[i] = src[i];
 }
 
-// pmc-standard importance kernel sums: wsum[j] = sum_i w_old[i] exp(-1/2
-// sum_p ((x_new[p][j] - x_old[p][i]) / tau_p)^2), inputs pre-divided by tau.
-// One thread per new member, old members staged through LDS in tiles.
+// pmc-standard importance kernel sums (src/gpu/detail/simulation_impl.inl:832-871):
+// wsum[j] = sum_i w_old[i] exp(-1/2 sum_p ((x_new[p][j] - x_old[p][i]) / tau_p)^2),
+// inputs pre-divided by tau. The old members are split into chunks of
+// kPmcChunk (a 2D grid: new members x chunks, so that a large population fills
+// the GPU); a block stages 256 old members of its chunk at a time in LDS and
+// each thread sums them for its new member; pmc_reduce_kernel adds a new
+// member's chunk sums in chunk order. The order of every sum depends only on
+// the sizes, so results are reproducible.
 constexpr int kPmcMaxP = 3 * PCS_MAX_SPECIES;
+constexpr uint32_t kPmcChunk = 4096;
 
-__global__ __launch_bounds__(256) void pmc_sums_kernel(const double* xn, uint32_t n_new, const double* xo,
-                                                       const double* w_old, uint32_t n_old, uint32_t P, double* wsum)
+// e^y for y <= 0: y = n ln 2 + r (Cody-Waite, ln 2 in a 42-bit head and a
+// tail), |r| <= ln2 / 2, e^r by its Taylor series to degree 13 (remainder
+// below 5e-18 relative), 2^n by ldexp; 0 below the smallest subnormal.
+__device__ __forceinline__ double exp_nonpos(double y)
 {
-    __shared__ double sh[kPmcMaxP][256];
-    __shared__ double shw[256];
+    constexpr double T[14] = {
+        0x1.0000000000000p+0, 0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5555555555555p-3,
+        0x1.5555555555555p-5, 0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+        0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
+        0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33,
+    };
+    if (!(y > -745.2)) return 0.0;
+    const double n = __builtin_rint(y * 0x1.71547652b82fep+0);
+    double r = __builtin_fma(-n, 0x1.62e42fefa3800p-1, y);
+    r = __builtin_fma(-n, 0x1.ef35793c76730p-45, r);
+    double p = T[13];
+#pragma unroll
+    for (int k = 12; k >= 0; --k) p = __builtin_fma(p, r, T[k]);
+    return __builtin_ldexp(p, (int)n);
+}
+
+template <int PMAX>
+__global__ __launch_bounds__(256) void pmc_sums_kernel(const double* xn, uint32_t n_new, const double* xo,
+                                                       const double* w_old, uint32_t n_old, uint32_t P, double* part)
+{
+    extern __shared__ double pmc_lds[];  // P rows of 256 old coordinates, then 256 weights
     const uint32_t tid = threadIdx.x;
     const uint32_t j = blockIdx.x * 256u + tid;
-    double x[kPmcMaxP];
+    const uint32_t i_lo = blockIdx.y * kPmcChunk, i_hi = min(n_old, i_lo + kPmcChunk);
+    double* shw = pmc_lds + (size_t)P * 256u;
+    double x[PMAX];
 #pragma unroll
-    for (int p = 0; p < kPmcMaxP; ++p) x[p] = ((uint32_t)p < P && j < n_new) ? xn[(uint64_t)p * n_new + j] : 0.0;
+    for (int p = 0; p < PMAX; ++p) x[p] = ((uint32_t)p < P && j < n_new) ? xn[(uint64_t)p * n_new + j] : 0.0;
     double acc = 0.0;
-    for (uint32_t t0 = 0; t0 < n_old; t0 += 256u) {
+    for (uint32_t t0 = i_lo; t0 < i_hi; t0 += 256u) {
         const uint32_t i = t0 + tid;
-        for (uint32_t p = 0; p < P; ++p) sh[p][tid] = i < n_old ? xo[(uint64_t)p * n_old + i] : 0.0;
-        shw[tid] = i < n_old ? w_old[i] : 0.0;
+        for (uint32_t p = 0; p < P; ++p) pmc_lds[p * 256u + tid] = i < i_hi ? xo[(uint64_t)p * n_old + i] : 0.0;
+        shw[tid] = i < i_hi ? w_old[i] : 0.0;
         __syncthreads();
-        const uint32_t m = min(256u, n_old - t0);
+        const uint32_t m = min(256u, i_hi - t0);
         for (uint32_t ii = 0; ii < m; ++ii) {
             double s2 = 0.0;
 #pragma unroll
-            for (int p = 0; p < kPmcMaxP; ++p) {
+            for (int p = 0; p < PMAX; ++p) {
                 if ((uint32_t)p < P) {
-                    const double d = x[p] - sh[p][ii];
-                    s2 = fma(d, d, s2);
+                    const double d = x[p] - pmc_lds[p * 256u + ii];
+                    s2 = __builtin_fma(d, d, s2);
                 }
             }
-            acc = fma(shw[ii], exp(-0.5 * s2), acc);
+            acc = __builtin_fma(shw[ii], exp_nonpos(-0.5 * s2), acc);
         }
         __syncthreads();
     }
-    if (j < n_new) wsum[j] = acc;
+    if (j < n_new) part[(uint64_t)blockIdx.y * n_new + j] = acc;
+}
+
+__global__ void pmc_reduce_kernel(const double* part, uint32_t n_new, uint32_t chunks, double* wsum)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_new) return;
+    double s = 0.0;
+    for (uint32_t c = 0; c < chunks; ++c) s += part[(uint64_t)c * n_new + j];
+    wsum[j] = s;
 }
 
 // ---------------------------------------------------------------- launchers
 
+uint32_t pmc_chunks(uint32_t n_old) { return n_old ? (n_old + kPmcChunk - 1u) / kPmcChunk : 1u; }
+
 hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, const double* w_old, uint32_t n_old,
-                           uint32_t P, double* wsum, hipStream_t s)
+                           uint32_t P, double* part, double* wsum, hipStream_t s)
 {
     (void)hipGetLastError();  // report only this launch's error
     if (n_new == 0) return hipSuccess;
-    hipLaunchKernelGGL(pmc_sums_kernel, dim3((n_new + 255u) / 256u), dim3(256), 0, s, xn, n_new, xo, w_old, n_old, P,
-                       wsum);
+    if (P == 0 || P > (uint32_t)kPmcMaxP) return hipErrorInvalidValue;
+    const uint32_t chunks = pmc_chunks(n_old);
+    const dim3 g((n_new + 255u) / 256u, chunks);
+    const size_t lds = ((size_t)P + 1u) * 256u * sizeof(double);
+    if (P <= 4) hipLaunchKernelGGL(pmc_sums_kernel<4>, g, dim3(256), lds, s, xn, n_new, xo, w_old, n_old, P, part);
+    else if (P <= 8) hipLaunchKernelGGL(pmc_sums_kernel<8>, g, dim3(256), lds, s, xn, n_new, xo, w_old, n_old, P, part);
+    else hipLaunchKernelGGL(pmc_sums_kernel<kPmcMaxP>, g, dim3(256), lds, s, xn, n_new, xo, w_old, n_old, P, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pmc_reduce_kernel, dim3((n_new + 255u) / 256u), dim3(256), 0, s, part, n_new, chunks, wsum);
     return hipGetLastError();
 }
 
