@@ -516,6 +516,15 @@ namespace {
 
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
+
+// Workspace of pcs_pmc_weights on one GPU (grown on demand, kept until exit).
+struct PmcWork {
+    std::mutex mu;
+    double *xn = nullptr, *xo = nullptr, *w = nullptr, *s = nullptr, *part = nullptr;
+    size_t cap[5] = {0, 0, 0, 0, 0};
+    hipStream_t stream = nullptr;
+};
+PmcWork g_pmc[kMaxDevices];
 unsigned char* g_scratch[kMaxDevices] = {};
 
 // With `device` current.
@@ -1368,6 +1377,7 @@ int pcs_generate_tracks(pcs_ctx* c, const pcs_proposal* prop, const uint32_t* vf
 int pcs_pmc_weights(int device, const double* theta_new, uint32_t n_new, const double* theta_old, const double* w_old,
                     uint32_t n_old, const double* tau, uint32_t n_params, double* w_out)
 {
+    if (device < 0 || device >= kMaxDevices) return set_error(PCS_EINVAL, "device %d outside [0, %d)", device, kMaxDevices);
     if (n_params < 1 || n_params > 3 * PCS_MAX_SPECIES) return set_error(PCS_EINVAL, "n_params outside [1, 24]");
     if ((n_new && (!theta_new || !w_out)) || (n_old && (!theta_old || !w_old)) || !tau)
         return set_error(PCS_EINVAL, "NULL argument");
@@ -1382,24 +1392,37 @@ int pcs_pmc_weights(int device, const double* theta_new, uint32_t n_new, const d
         for (uint32_t j = 0; j < n_new; ++j) xn[(size_t)p * n_new + j] = theta_new[(size_t)p * n_new + j] / tau[p];
         for (uint32_t i = 0; i < n_old; ++i) xo[(size_t)p * n_old + i] = theta_old[(size_t)p * n_old + i] / tau[p];
     }
-    double *d_xn = nullptr, *d_xo = nullptr, *d_w = nullptr, *d_s = nullptr, *d_part = nullptr;
-    int r = dev_alloc(&d_xn, xn.size());
-    if (r == PCS_OK) r = dev_alloc(&d_xo, xo.size() + 1);
-    if (r == PCS_OK) r = dev_alloc(&d_w, (size_t)n_old + 1);
-    if (r == PCS_OK) r = dev_alloc(&d_s, n_new);
-    if (r == PCS_OK) r = dev_alloc(&d_part, (size_t)pmc_chunks(n_old) * n_new);
+    // Device workspace of this GPU, kept between calls (a hipFree would
+    // synchronise the whole device every generation); calls on one GPU take
+    // it in turn.
+    PmcWork& wk = g_pmc[device];
+    std::lock_guard<std::mutex> lk(wk.mu);
+    int r = PCS_OK;
+    auto grow = [&r](double** p, size_t& cap, size_t n) {
+        if (r == PCS_OK && n > cap) {
+            r = dev_alloc(p, n);
+            cap = r == PCS_OK ? n : 0;
+        }
+    };
+    grow(&wk.xn, wk.cap[0], xn.size());
+    grow(&wk.xo, wk.cap[1], xo.size() + 1);
+    grow(&wk.w, wk.cap[2], (size_t)n_old + 1);
+    grow(&wk.s, wk.cap[3], n_new);
+    grow(&wk.part, wk.cap[4], (size_t)pmc_chunks(n_old) * n_new);
+    if (r == PCS_OK && !wk.stream) {
+        const hipError_t e = hipStreamCreateWithFlags(&wk.stream, hipStreamNonBlocking);
+        if (e != hipSuccess) r = hip_error(e, "pcs_pmc_weights stream");
+    }
     if (r == PCS_OK) {
-        hipError_t e = hipMemcpy(d_xn, xn.data(), xn.size() * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess && n_old) e = hipMemcpy(d_xo, xo.data(), xo.size() * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess && n_old) e = hipMemcpy(d_w, w_old, (size_t)n_old * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = launch_pmc_sums(d_xn, n_new, d_xo, d_w, n_old, n_params, d_part, d_s, nullptr);
-        if (e == hipSuccess) e = hipDeviceSynchronize();
-        if (e == hipSuccess) e = hipMemcpy(w_out, d_s, (size_t)n_new * 8, hipMemcpyDeviceToHost);
+        hipStream_t st = wk.stream;
+        hipError_t e = hipMemcpyAsync(wk.xn, xn.data(), xn.size() * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && n_old) e = hipMemcpyAsync(wk.xo, xo.data(), xo.size() * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && n_old) e = hipMemcpyAsync(wk.w, w_old, (size_t)n_old * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = launch_pmc_sums(wk.xn, n_new, wk.xo, wk.w, n_old, n_params, wk.part, wk.s, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(w_out, wk.s, (size_t)n_new * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) r = hip_error(e, "pcs_pmc_weights");
     }
-    void* ptrs[] = {d_xn, d_xo, d_w, d_s, d_part};
-    for (void* p : ptrs)
-        if (p) hipFree(p);
     if (r != PCS_OK) return r;
     double sum = 0.0;
     for (uint32_t j = 0; j < n_new; ++j) sum += (w_out[j] = 1.0 / w_out[j]);

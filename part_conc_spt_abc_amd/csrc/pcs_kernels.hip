@@ -1252,85 +1252,97 @@ __global__ void dense_hist_kernel(const DevProp* props, uint32_t prop_lo, const 
     }
 }
 
-// Exclusive prefix sum over the 1024 threads of a block (sh: 1024 entries).
+// Exclusive prefix sum over the 1024 threads of a block: within each wave by
+// shuffles, then over the 16 wave totals by wave 0 (sh: 32 entries).
 constexpr int kPrepThreads = 1024;
+constexpr int kPrepWaves = kPrepThreads / kWave;
 
 __device__ uint64_t block_scan_u64(uint64_t v, uint64_t* sh, uint64_t& total)
 {
-    const uint32_t tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (uint32_t o = 1; o < (uint32_t)kPrepThreads; o <<= 1) {
-        const uint64_t add = tid >= o ? sh[tid - o] : 0ull;
-        __syncthreads();
-        sh[tid] += add;
-        __syncthreads();
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint64_t y = __shfl_up(x, o, kWave);
+        if (lane >= (uint32_t)o) x += y;
     }
-    total = sh[kPrepThreads - 1];
-    const uint64_t incl = sh[tid];
+    if (lane == kWave - 1) sh[wave] = x;
     __syncthreads();
-    return incl - v;
+    if (wave == 0u) {
+        uint64_t t = lane < (uint32_t)kPrepWaves ? sh[lane] : 0ull;
+#pragma unroll
+        for (int o = 1; o < kPrepWaves; o <<= 1) {
+            const uint64_t y = __shfl_up(t, o, kWave);
+            if (lane >= (uint32_t)o) t += y;
+        }
+        if (lane < (uint32_t)kPrepWaves) sh[kPrepWaves + lane] = t;
+    }
+    __syncthreads();
+    const uint64_t base = wave ? sh[kPrepWaves + wave - 1] : 0ull;
+    total = sh[2 * kPrepWaves - 1];
+    __syncthreads();
+    return base + x - v;
 }
 
 // Device-side plan of a batch (one workgroup): record slices by the
 // capacity estimate, prefix-summed into the arena; video -> proposal map;
-// tile prefix. Each thread owns a contiguous run of proposals (videos).
+// tile prefix. Proposals (videos) go in rounds of 1024, thread t taking
+// element r * 1024 + t, so that each round's loads and stores are coalesced.
 __global__ __launch_bounds__(kPrepThreads) void prepare_kernel(PrepArgs a)
 {
-    __shared__ uint64_t sh[kPrepThreads];
+    __shared__ uint64_t sh[2 * kPrepWaves];
     const uint32_t tid = threadIdx.x;
     const DevSys& sy = a.sys[0];
-    const uint32_t per = (a.n + kPrepThreads - 1) / kPrepThreads;
-    const uint32_t i0 = min(tid * per, a.n), i1 = min(i0 + per, a.n);
-    uint64_t run = 0;
-    for (uint32_t i = i0; i < i1; ++i) {
-        const pcs_proposal& p = a.in[i];
-        run += slice_capacity(sy, p.half_az, p.n_videos, a.vframes + p.first_video, a.vparticles + p.first_video,
-                              a.cap_factor);
-    }
-    uint64_t total_rec;
-    uint64_t off = block_scan_u64(run, sh, total_rec);
-    for (uint32_t i = i0; i < i1; ++i) {
-        const pcs_proposal& p = a.in[i];
-        const uint64_t cap = slice_capacity(sy, p.half_az, p.n_videos, a.vframes + p.first_video,
-                                            a.vparticles + p.first_video, a.cap_factor);
-        const uint64_t at = off < a.arena_cap ? off : a.arena_cap;
-        const uint64_t room = a.arena_cap - at;
-        DevProp d;
-        d.sim_id = p.sim_id;
-        d.first_video = p.first_video;
-        d.n_videos = p.n_videos;
-        d.sys_id = 0u;
-        d.rec = a.rec + at;
-        d.srt = a.srt + at;
-        d.rec_cap = (uint32_t)(cap < room ? cap : room);
-        d.key0 = 0u;
-        d.key1 = 0u;
-        d.pad = 0u;
-        for (int c = 0; c < PCS_MAX_SPECIES; ++c) {
-            d.half_az[c] = p.half_az[c];
-            d.cum_prob[c] = p.cum_prob[c];
-            d.sigma[c] = p.sigma[c];
+    uint64_t base = 0ull;
+    for (uint32_t r0 = 0; r0 < a.n; r0 += kPrepThreads) {
+        const uint32_t i = r0 + tid;
+        uint64_t cap = 0ull;
+        if (i < a.n) {
+            const pcs_proposal& p = a.in[i];
+            cap = slice_capacity(sy, p.half_az, p.n_videos, a.vframes + p.first_video, a.vparticles + p.first_video,
+                                 a.cap_factor);
         }
-        a.props[i] = d;
-        a.rec_count[i] = 0u;
-        for (uint32_t v = p.first_video; v < p.first_video + p.n_videos; ++v) a.vprop[v] = i;
-        off += cap;
+        uint64_t round_total;
+        const uint64_t off = base + block_scan_u64(cap, sh, round_total);
+        if (i < a.n) {
+            const pcs_proposal& p = a.in[i];
+            const uint64_t at = off < a.arena_cap ? off : a.arena_cap;
+            const uint64_t room = a.arena_cap - at;
+            DevProp d;
+            d.sim_id = p.sim_id;
+            d.first_video = p.first_video;
+            d.n_videos = p.n_videos;
+            d.sys_id = 0u;
+            d.rec = a.rec + at;
+            d.srt = a.srt + at;
+            d.rec_cap = (uint32_t)(cap < room ? cap : room);
+            d.key0 = 0u;
+            d.key1 = 0u;
+            d.pad = 0u;
+            for (int c = 0; c < PCS_MAX_SPECIES; ++c) {
+                d.half_az[c] = p.half_az[c];
+                d.cum_prob[c] = p.cum_prob[c];
+                d.sigma[c] = p.sigma[c];
+            }
+            a.props[i] = d;
+            a.rec_count[i] = 0u;
+            for (uint32_t v = p.first_video; v < p.first_video + p.n_videos; ++v) a.vprop[v] = i;
+        }
+        base += round_total;
     }
-    const uint32_t vper = (a.n_videos + kPrepThreads - 1) / kPrepThreads;
-    const uint32_t v0 = min(tid * vper, a.n_videos), v1 = min(v0 + vper, a.n_videos);
-    uint64_t tr = 0;
-    for (uint32_t v = v0; v < v1; ++v) tr += (a.vparticles[v] + a.tile_particles - 1u) / a.tile_particles;
-    uint64_t total_tiles;
-    uint64_t t = block_scan_u64(tr, sh, total_tiles);
-    for (uint32_t v = v0; v < v1; ++v) {
-        a.vtile[v] = (uint32_t)t;
-        t += (a.vparticles[v] + a.tile_particles - 1u) / a.tile_particles;
+    uint64_t tbase = 0ull;
+    for (uint32_t r0 = 0; r0 < a.n_videos; r0 += kPrepThreads) {
+        const uint32_t v = r0 + tid;
+        const uint64_t tiles = v < a.n_videos ? (a.vparticles[v] + a.tile_particles - 1u) / a.tile_particles : 0ull;
+        uint64_t round_total;
+        const uint64_t t = tbase + block_scan_u64(tiles, sh, round_total);
+        if (v < a.n_videos) a.vtile[v] = (uint32_t)t;
+        tbase += round_total;
     }
     if (tid == 0) {
-        a.vtile[a.n_videos] = (uint32_t)total_tiles;
-        a.totals[0] = total_rec;
-        a.totals[1] = total_tiles;
+        a.vtile[a.n_videos] = (uint32_t)tbase;
+        a.totals[0] = base;
+        a.totals[1] = tbase;
         *a.tile_counter = 0u;
     }
 }
