@@ -41,7 +41,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 from part_conc_spt_abc_amd import _abi  # noqa: E402
-from part_conc_spt_abc_amd.gpu import Context, Server, pmc_weights  # noqa: E402
+from part_conc_spt_abc_amd.gpu import Context, Server  # noqa: E402
 from part_conc_spt_abc_amd.model import Model, prior_record, proposal_record, stokes_einstein  # noqa: E402
 
 # The walk is bound by VALU issue. Its roofline counts the work the algorithm
@@ -378,28 +378,28 @@ def generation_population(model, n, rng, frames_choices=(500, 1000, 2000)):
 
 
 def run_generations(batch, pop, steps, warmup, seed0=1000):
-    """Timed ABC-PMC generations on the GPU. One step: the cumulative weights
-    of the population, the proposal step of all its trials on the GPU
-    (parents, displacements, Poisson counts, plan; the proposed parameters
-    come back to the host), the walk and distance of the trials as one
-    hipGraph launch, the read-back of the distances, and the trials'
-    pmc-standard importance weights against the population on the GPU
-    (n x n kernel sums over the parameters with spread). The population is
-    the same at every step, so every step does the same work. Returns the
-    wall times, per-step (propose, run + read, weights) ms and particle-steps."""
+    """Timed ABC-PMC generations on the GPU, from a population resident in
+    HBM before timing (pcs_batch_set_population). One step: the proposal
+    step of all the trials on the GPU (parents, displacements, Poisson
+    counts, plan; the proposed parameters come back to the host), the walk
+    and distance of the trials as one hipGraph launch, the read-back of the
+    distances, and the trials' pmc-standard importance weights against the
+    population on the GPU (n x n kernel sums over the parameters with
+    spread), read back. The population is the same at every step, so every
+    step does the same work. Returns the wall times, per-step (propose,
+    run + read, weights) ms and particle-steps."""
     theta, w, tau, frames, prior = pop
     n = theta.shape[1]
-    keep = tau > 0
+    batch.set_population(theta, w)
     walls, parts, psteps = [], [], []
     for i in range(warmup + steps):
         t0 = time.perf_counter()
-        cum = np.cumsum(w)
-        th = batch.propose(prior, theta, cum, tau, n, frames, seed0 + i)
+        batch.propose(prior, None, None, tau, n, frames, seed0 + i)
         t1 = time.perf_counter()
         batch.run(seed0 + i, _abi.PCS_MODE_BATCHED)
         batch.read()
         t2 = time.perf_counter()
-        pmc_weights(th[keep], theta[keep], w, tau[keep])
+        batch.pmc_weights(tau)
         t3 = time.perf_counter()
         if i >= warmup:
             walls.append(t3 - t0)

@@ -221,13 +221,30 @@ typedef struct pcs_prior {
  * slot, 0x70726F70), apart from the walk's; the normals are FP32 Box-Muller
  * pairs of the walk's own math, the rest is FP64 in a fixed operation order,
  * so a CPU restatement reproduces every result bit for bit.
- * theta: P x n_pop (row-major), cum_w: n_pop non-decreasing, tau: P;
+ * theta: P x n_pop (row-major), cum_w: n_pop non-decreasing, tau: P; or
+ * theta = cum_w = NULL and n_pop = 0: the resident population of
+ * pcs_batch_set_population;
  * video_frames: n_trials x videos_per_trial frame counts (trial-major);
  * trial t gets sim_id first_sim_id + t and videos [t J, (t + 1) J).
  * theta_out (nullable) receives the proposed parameters, P x n_trials. */
 int pcs_batch_propose(pcs_batch* b, const pcs_prior* prior, const double* theta, const double* cum_w,
                       const double* tau, uint32_t n_pop, uint32_t n_trials, const uint32_t* video_frames,
                       uint32_t videos_per_trial, uint32_t first_sim_id, uint64_t seed, double* theta_out);
+/* Population of an ABC-PMC generation kept resident in the batch: theta
+ * (n_params x n_pop, the rows of pcs_batch_propose) and normalised weights w
+ * (n_pop). Their cumulative sums are formed on the device in member order
+ * (the same sums as a sequential cumsum). pcs_batch_propose then takes
+ * theta = cum_w = NULL (and n_pop = 0) to draw from this population, and
+ * pcs_batch_pmc_weights weighs the proposed trials against it. */
+int pcs_batch_set_population(pcs_batch* b, const double* theta, const double* w, uint32_t n_pop, uint32_t n_params);
+/* pmc-standard importance weights of the batch's proposed trials (the last
+ * pcs_batch_propose, parameters resident on the device) against its resident
+ * population (pcs_batch_set_population), as pcs_pmc_weights on the same
+ * values: w_out[j] proportional to 1 / sum_i w_i prod_p N(theta_new[p][j] -
+ * theta[p][i]; 0, tau[p]) over the rows with tau[p] > 0 (rows with tau 0 have
+ * no spread and give every pair the same factor); normalised to sum 1
+ * (n_trials doubles). */
+int pcs_batch_pmc_weights(pcs_batch* b, const double* tau, double* w_out);
 /* The loaded or proposed batch's proposal records (n) and video particle
  * counts (n_videos); each nullable. */
 int pcs_batch_read_proposals(pcs_batch* b, pcs_proposal* props_out, uint32_t* video_particles_out);

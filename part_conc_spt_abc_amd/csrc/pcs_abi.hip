@@ -173,6 +173,15 @@ struct pcs_batch {
     uint64_t* d_totals = nullptr;  // [0] planned records, [1] tiles
     size_t cap_in = 0, cap_pop = 0, cap_new = 0;
     uint64_t last_need = 0;      // records the last device plan asked for
+    // resident population (pcs_batch_set_population): d_theta_pop, d_cumw
+    // and its weights d_wpop; the trials' parameters of the last proposal
+    // step stay in d_theta_new (P rows of n_props)
+    double* d_wpop = nullptr;
+    bool pop_resident = false;
+    uint32_t pop_n = 0, pop_P = 0, new_P = 0;
+    // pcs_batch_pmc_weights workspace: scaled rows, kernel sums, chunk sums
+    double *d_xn = nullptr, *d_xo = nullptr, *d_wsum = nullptr, *d_part = nullptr;
+    size_t cap_xn = 0, cap_xo = 0, cap_wsum = 0, cap_part = 0;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -744,7 +753,8 @@ int pcs_batch_destroy(pcs_batch* b)
     destroy_graph(b);
     void* ptrs[] = {b->d_props, b->d_vframes, b->d_vparticles, b->d_vprop, b->d_vtile, b->d_rec, b->d_srt,
                     b->d_rec2, b->d_srt2, b->d_count, b->d_tracks, b->d_tile_counter, b->d_dist, b->d_exact, b->d_status,
-                    b->d_seed, b->d_in, b->d_theta_pop, b->d_cumw, b->d_theta_new, b->d_totals};
+                    b->d_seed, b->d_in, b->d_theta_pop, b->d_cumw, b->d_theta_new, b->d_totals, b->d_wpop,
+                    b->d_xn, b->d_xo, b->d_wsum, b->d_part};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (b->h_seed) hipHostFree(b->h_seed);
@@ -996,7 +1006,16 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
     if (C != c->sys.n_species) return set_error(PCS_EINVAL, "prior has %u components, the model %u", C, c->sys.n_species);
     if (Z != 1u && Z != C) return set_error(PCS_EINVAL, "n_depths %u is neither 1 nor n_components", Z);
     const uint32_t P = 2u * C + Z;
-    if (n_pop == 0 || !theta || !cum_w || !tau) return set_error(PCS_EINVAL, "population is empty or NULL");
+    const bool resident = !theta && !cum_w;
+    if (resident) {
+        if (!b->pop_resident) return set_error(PCS_ESTATE, "no resident population (pcs_batch_set_population)");
+        if (b->pop_P != P) return set_error(PCS_EINVAL, "resident population has %u parameters, the prior %u", b->pop_P, P);
+        if (n_pop != 0 && n_pop != b->pop_n)
+            return set_error(PCS_EINVAL, "n_pop %u, the resident population has %u", n_pop, b->pop_n);
+        n_pop = b->pop_n;
+    }
+    if (n_pop == 0 || (!resident && (!theta || !cum_w)) || !tau)
+        return set_error(PCS_EINVAL, "population is empty or NULL");
     if (J == 0 || J > 4096u) return set_error(PCS_EINVAL, "videos_per_trial %u outside [1, 4096]", J);
     if (n_trials && !video_frames) return set_error(PCS_EINVAL, "video_frames is NULL");
     if ((uint64_t)n_trials * J >= 0x80000000ull) return set_error(PCS_EINVAL, "n_trials x videos_per_trial too large");
@@ -1012,14 +1031,16 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
     // bounded inputs keep every reflection and sum finite
     for (uint32_t p = 0; p < P; ++p)
         if (!(tau[p] >= 0.0 && tau[p] <= 1e100)) return set_error(PCS_EINVAL, "tau[%u] outside [0, 1e100]", p);
-    for (size_t i = 0; i < (size_t)P * n_pop; ++i)
-        if (!(std::fabs(theta[i]) <= 1e100)) return set_error(PCS_EINVAL, "theta[%zu] not finite or above 1e100", i);
+    if (!resident)
+        for (size_t i = 0; i < (size_t)P * n_pop; ++i)
+            if (!(std::fabs(theta[i]) <= 1e100)) return set_error(PCS_EINVAL, "theta[%zu] not finite or above 1e100", i);
     for (int k = 0; k < 3; ++k)
         if (!(std::fabs(lo[k]) <= 1e100 && std::fabs(hi[k]) <= 1e100))
             return set_error(PCS_EINVAL, "prior bounds above 1e100");
-    for (uint32_t i = 0; i < n_pop; ++i)
-        if (!std::isfinite(cum_w[i]) || (i && cum_w[i] < cum_w[i - 1]))
-            return set_error(PCS_EINVAL, "cum_w must be finite and non-decreasing (member %u)", i);
+    if (!resident)
+        for (uint32_t i = 0; i < n_pop; ++i)
+            if (!std::isfinite(cum_w[i]) || (i && cum_w[i] < cum_w[i - 1]))
+                return set_error(PCS_EINVAL, "cum_w must be finite and non-decreasing (member %u)", i);
     PCS_HIP(hipSetDevice(c->device));
     PCS_HIP(hipStreamSynchronize(b->stream));
     PCS_TRY(restore_slices(b, false));
@@ -1048,11 +1069,14 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
         b->cap_videos = (size_t)nv + 1u;
         relayout = true;
     }
-    if (n_pop > b->cap_pop || !b->d_theta_pop) {
+    if (!resident && (n_pop > b->cap_pop || !b->d_theta_pop)) {
         PCS_TRY(dev_alloc(&b->d_theta_pop, (size_t)3u * PCS_MAX_SPECIES * n_pop));
         PCS_TRY(dev_alloc(&b->d_cumw, n_pop));
+        PCS_TRY(dev_alloc(&b->d_wpop, n_pop));
         b->cap_pop = n_pop;
     }
+    if (!resident) b->pop_resident = false;  // replaced by an uploaded population without weights
+    b->new_P = P;
     if (!b->d_totals) PCS_TRY(dev_alloc(&b->d_totals, 2));
     b->dev_planned = true;
     b->host_vtile = false;
@@ -1071,8 +1095,10 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
     b->vframes.clear();
     b->vparticles.clear();
     hipStream_t st = b->stream;
-    PCS_HIP(hipMemcpyAsync(b->d_theta_pop, theta, (size_t)P * n_pop * 8u, hipMemcpyHostToDevice, st));
-    PCS_HIP(hipMemcpyAsync(b->d_cumw, cum_w, (size_t)n_pop * 8u, hipMemcpyHostToDevice, st));
+    if (!resident) {
+        PCS_HIP(hipMemcpyAsync(b->d_theta_pop, theta, (size_t)P * n_pop * 8u, hipMemcpyHostToDevice, st));
+        PCS_HIP(hipMemcpyAsync(b->d_cumw, cum_w, (size_t)n_pop * 8u, hipMemcpyHostToDevice, st));
+    }
     if (nv) PCS_HIP(hipMemcpyAsync(b->d_vframes, video_frames, (size_t)nv * 4u, hipMemcpyHostToDevice, st));
     ProposeArgs pa{};
     pa.theta = b->d_theta_pop;
@@ -1118,6 +1144,78 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
     }
     PCS_HIP(hipStreamSynchronize(st));
     if (relayout) destroy_graph(b);
+    return PCS_OK;
+}
+
+int pcs_batch_set_population(pcs_batch* b, const double* theta, const double* w, uint32_t n_pop, uint32_t n_params)
+{
+    if (!b || !theta || !w) return set_error(PCS_EINVAL, "NULL argument");
+    if (n_pop == 0) return set_error(PCS_EINVAL, "population is empty");
+    if (n_params < 1 || n_params > 3u * PCS_MAX_SPECIES) return set_error(PCS_EINVAL, "n_params outside [1, 24]");
+    for (size_t i = 0; i < (size_t)n_params * n_pop; ++i)
+        if (!(std::fabs(theta[i]) <= 1e100)) return set_error(PCS_EINVAL, "theta[%zu] not finite or above 1e100", i);
+    for (uint32_t i = 0; i < n_pop; ++i)
+        if (!(w[i] >= 0.0) || !std::isfinite(w[i])) return set_error(PCS_EINVAL, "w[%u] negative or not finite", i);
+    PCS_HIP(hipSetDevice(b->ctx->device));
+    PCS_HIP(hipStreamSynchronize(b->stream));
+    if (n_pop > b->cap_pop || !b->d_theta_pop || !b->d_wpop) {
+        PCS_TRY(dev_alloc(&b->d_theta_pop, (size_t)3u * PCS_MAX_SPECIES * n_pop));
+        PCS_TRY(dev_alloc(&b->d_cumw, n_pop));
+        PCS_TRY(dev_alloc(&b->d_wpop, n_pop));
+        b->cap_pop = n_pop;
+    }
+    hipStream_t st = b->stream;
+    PCS_HIP(hipMemcpyAsync(b->d_theta_pop, theta, (size_t)n_params * n_pop * 8u, hipMemcpyHostToDevice, st));
+    PCS_HIP(hipMemcpyAsync(b->d_wpop, w, (size_t)n_pop * 8u, hipMemcpyHostToDevice, st));
+    PCS_HIP(launch_cumsum(b->d_wpop, n_pop, b->d_cumw, st));
+    PCS_HIP(hipStreamSynchronize(st));
+    b->pop_resident = true;
+    b->pop_n = n_pop;
+    b->pop_P = n_params;
+    return PCS_OK;
+}
+
+int pcs_batch_pmc_weights(pcs_batch* b, const double* tau, double* w_out)
+{
+    if (!b || !tau || !w_out) return set_error(PCS_EINVAL, "NULL argument");
+    if (!b->pop_resident) return set_error(PCS_ESTATE, "no resident population (pcs_batch_set_population)");
+    if (!b->dev_planned || b->new_P != b->pop_P)
+        return set_error(PCS_ESTATE, "no trials proposed from the resident population (pcs_batch_propose)");
+    const uint32_t P = b->pop_P, n_new = b->n_props, n_old = b->pop_n;
+    ScaleRows sr{};
+    for (uint32_t p = 0; p < P; ++p) {
+        if (!(tau[p] >= 0.0) || !std::isfinite(tau[p])) return set_error(PCS_EINVAL, "tau[%u] invalid", p);
+        if (tau[p] > 0.0) {
+            sr.rows[sr.n_rows] = p;
+            sr.tau[sr.n_rows] = tau[p];
+            ++sr.n_rows;
+        }
+    }
+    if (n_new == 0) return PCS_OK;
+    if (sr.n_rows == 0) {  // every member identical: equal weights
+        for (uint32_t j = 0; j < n_new; ++j) w_out[j] = 1.0 / (double)n_new;
+        return PCS_OK;
+    }
+    PCS_HIP(hipSetDevice(b->ctx->device));
+    auto grow = [](double** ptr, size_t& cap, size_t n) -> int {
+        if (n <= cap && *ptr) return PCS_OK;
+        const int r = dev_alloc(ptr, n);
+        cap = r == PCS_OK ? n : 0;
+        return r;
+    };
+    PCS_TRY(grow(&b->d_xn, b->cap_xn, (size_t)sr.n_rows * n_new));
+    PCS_TRY(grow(&b->d_xo, b->cap_xo, (size_t)sr.n_rows * n_old));
+    PCS_TRY(grow(&b->d_wsum, b->cap_wsum, n_new));
+    PCS_TRY(grow(&b->d_part, b->cap_part, (size_t)pmc_chunks(n_old) * n_new));
+    hipStream_t st = b->stream;
+    PCS_HIP(launch_scale_rows(b->d_theta_new, n_new, sr, b->d_xn, st));
+    PCS_HIP(launch_scale_rows(b->d_theta_pop, n_old, sr, b->d_xo, st));
+    PCS_HIP(launch_pmc_sums(b->d_xn, n_new, b->d_xo, b->d_wpop, n_old, sr.n_rows, b->d_part, b->d_wsum, st));
+    PCS_HIP(hipMemcpyAsync(w_out, b->d_wsum, (size_t)n_new * 8u, hipMemcpyDeviceToHost, st));
+    PCS_HIP(hipStreamSynchronize(st));
+    double sum = 0.0;
+    for (uint32_t j = 0; j < n_new; ++j) sum += (w_out[j] = 1.0 / w_out[j]);
+    for (uint32_t j = 0; j < n_new; ++j) w_out[j] /= sum;
     return PCS_OK;
 }
 

@@ -252,6 +252,15 @@ size_t serve_loop_lds(const LoopArgs& a);
 hipError_t serve_loop_occupancy(const LoopArgs& a, uint32_t n_de_max, int* groups_per_cu);
 hipError_t launch_prepare(const PrepArgs& a, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t s);
+// Sequential prefix sum of w (one lane, member order) into cum.
+hipError_t launch_cumsum(const double* w, uint32_t n, double* cum, hipStream_t s);
+// dst[k][j] = src[rows[k]][j] / tau[k] for k < n_rows (rows of n doubles).
+struct ScaleRows {
+    uint32_t rows[3 * PCS_MAX_SPECIES];
+    double tau[3 * PCS_MAX_SPECIES];
+    uint32_t n_rows;
+};
+hipError_t launch_scale_rows(const double* src, uint32_t n, const ScaleRows& sr, double* dst, hipStream_t s);
 hipError_t launch_dense_hist(const DevProp* props, uint32_t prop_lo, uint32_t n_props,
                              const uint32_t* rec_count, uint32_t max_count,
                              uint32_t n_k, uint32_t n_de, uint32_t* hist, hipStream_t s);

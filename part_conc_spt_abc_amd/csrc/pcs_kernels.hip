@@ -1627,6 +1627,32 @@ __global__ __launch_bounds__(256) void pmc_sums_kernel(const double* xn, uint32_
     if (j < n_new) part[(uint64_t)blockIdx.y * n_new + j] = acc;
 }
 
+// Cumulative weights in member order, one lane: the same FP64 sums as a
+// sequential cumsum (a population is set once per generation).
+__global__ void cumsum_kernel(const double* w, uint32_t n, double* cum)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double acc = 0.0;
+    for (uint32_t i = 0; i < n; ++i) cum[i] = (acc += w[i]);
+}
+
+// Parameter rows in units of their displacement scale (the pmc kernel's
+// inputs): correctly rounded FP64 division, as on the host.
+__global__ void scale_rows_kernel(const double* src, uint32_t n, ScaleRows sr, double* dst)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = blockIdx.y;
+    if (j >= n || k >= sr.n_rows) return;
+    uint32_t row = 0u;
+    double t = 1.0;
+#pragma unroll
+    for (uint32_t q = 0; q < 3u * PCS_MAX_SPECIES; ++q) {
+        row = (k == q) ? sr.rows[q] : row;
+        t = (k == q) ? sr.tau[q] : t;
+    }
+    dst[(uint64_t)k * n + j] = src[(uint64_t)row * n + j] / t;
+}
+
 __global__ void pmc_reduce_kernel(const double* part, uint32_t n_new, uint32_t chunks, double* wsum)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1808,6 +1834,22 @@ hipError_t launch_propose(const ProposeArgs& a, hipStream_t s)
     (void)hipGetLastError();  // report only this launch's error
     if (a.n_trials == 0) return hipSuccess;
     hipLaunchKernelGGL(propose_kernel, dim3((a.n_trials + 3u) / 4u), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cumsum(const double* w, uint32_t n, double* cum, hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launch's error
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(cumsum_kernel, dim3(1), dim3(64), 0, s, w, n, cum);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_rows(const double* src, uint32_t n, const ScaleRows& sr, double* dst, hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launch's error
+    if (n == 0 || sr.n_rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_rows_kernel, dim3((n + 255u) / 256u, sr.n_rows), dim3(256), 0, s, src, n, sr, dst);
     return hipGetLastError();
 }
 

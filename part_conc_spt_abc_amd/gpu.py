@@ -113,22 +113,40 @@ class Batch:
                                            _ptr(self._particles), len(self._frames)), "pcs_batch_load")
         self.n = len(props)
 
-    def propose(self, prior: _abi.pcs_prior, theta: np.ndarray, cum_w: np.ndarray, tau: np.ndarray, n_trials: int,
+    def set_population(self, theta: np.ndarray, w: np.ndarray):
+        """Keep a population (P x n_pop parameters, weights) resident in the
+        batch (pcs_batch_set_population)."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        ww = np.ascontiguousarray(w, dtype=np.float64)
+        _abi.check(self.lib.pcs_batch_set_population(self.h, _ptr(th), _ptr(ww), th.shape[1], th.shape[0]),
+                   "pcs_batch_set_population")
+
+    def propose(self, prior: _abi.pcs_prior, theta, cum_w, tau: np.ndarray, n_trials: int,
                 frames: np.ndarray, seed: int, first_sim_id: int = 0, want_theta: bool = True):
         """Proposal step of an ABC-PMC round on the GPU (pcs_batch_propose):
-        theta (P x n_pop) the population, cum_w its cumulative weights, tau the
-        displacement scales; frames (n_trials x J) the trials' video lengths.
-        Returns the proposed parameters (P x n_trials) or None."""
-        th = np.ascontiguousarray(theta, dtype=np.float64)
-        cw = np.ascontiguousarray(cum_w, dtype=np.float64)
+        theta (P x n_pop) the population, cum_w its cumulative weights (both
+        None: the resident population), tau the displacement scales; frames
+        (n_trials x J) the trials' video lengths. Returns the proposed
+        parameters (P x n_trials) or None."""
+        th = None if theta is None else np.ascontiguousarray(theta, dtype=np.float64)
+        cw = None if cum_w is None else np.ascontiguousarray(cum_w, dtype=np.float64)
         ta = np.ascontiguousarray(tau, dtype=np.float64)
         fr = np.ascontiguousarray(frames, dtype=np.uint32).reshape(n_trials, -1)
-        out = np.empty((th.shape[0], n_trials), np.float64) if want_theta else None
-        _abi.check(self.lib.pcs_batch_propose(self.h, C.byref(prior), _ptr(th), _ptr(cw), _ptr(ta), th.shape[1],
+        out = np.empty((len(ta), n_trials), np.float64) if want_theta else None
+        n_pop = 0 if th is None else th.shape[1]
+        _abi.check(self.lib.pcs_batch_propose(self.h, C.byref(prior), _ptr(th), _ptr(cw), _ptr(ta), n_pop,
                                               n_trials, _ptr(fr), fr.shape[1], first_sim_id, C.c_uint64(seed),
                                               _ptr(out)), "pcs_batch_propose")
         self._frames = fr.ravel()
         self.n = n_trials
+        return out
+
+    def pmc_weights(self, tau: np.ndarray) -> np.ndarray:
+        """pmc-standard weights of the proposed trials against the resident
+        population, on the GPU (pcs_batch_pmc_weights)."""
+        ta = np.ascontiguousarray(tau, dtype=np.float64)
+        out = np.empty(self.n, np.float64)
+        _abi.check(self.lib.pcs_batch_pmc_weights(self.h, _ptr(ta), _ptr(out)), "pcs_batch_pmc_weights")
         return out
 
     def read_proposals(self):

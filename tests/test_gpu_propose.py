@@ -154,3 +154,37 @@ def test_displacement_is_gaussian(gpu_lib):
     for p in range(3):
         assert stats.kstest(z[p], "norm").pvalue > 1e-4, p
         assert abs(z[p].mean()) < 5 / math.sqrt(n) and abs(z[p].var() - 1) < 5 * math.sqrt(2 / n)
+
+
+def test_resident_population_equals_host_arrays(gpu_lib):
+    """A population set resident in the batch (cumulative weights formed on
+    the device) gives the same proposals as the host arrays with numpy's
+    cumsum, and pcs_batch_pmc_weights equals pcs_pmc_weights on the same
+    values (rows with tau 0 left out), bit for bit."""
+    from part_conc_spt_abc_amd.gpu import pmc_weights
+
+    model = cases.headline_model(n_k=100, n_de=50, n_species=2)
+    rng = np.random.default_rng(17)
+    theta, _, tau = population(rng, 2, 1, 3000)
+    theta[4] = 2.0  # a depth without spread: tau 0, left out of the weights
+    tau = np.sqrt(2.0 * theta.var(axis=1))
+    assert tau[4] == 0.0
+    w = rng.dirichlet(np.ones(theta.shape[1]))
+    n = 2500
+    frames = np.full((n, 1), 40, np.uint32)
+    pr = prior_record(model, M_B, C_B, AZ_B)
+    with Context(model) as ctx:
+        b = ctx.batch()
+        th_host = b.propose(pr, theta, np.cumsum(w), tau, n, frames, 99)
+        _, parts_host = b.read_proposals()
+        b.set_population(theta, w)
+        th_res = b.propose(pr, None, None, tau, n, frames, 99)
+        _, parts_res = b.read_proposals()
+        w_res = b.pmc_weights(tau)
+        b.close()
+    assert np.array_equal(th_host.view(np.uint64), th_res.view(np.uint64))
+    assert np.array_equal(parts_host, parts_res)
+    keep = tau > 0
+    w_ref = pmc_weights(th_res[keep], theta[keep], w, tau[keep])
+    assert np.array_equal(w_res.view(np.uint64), w_ref.view(np.uint64))
+    assert abs(w_res.sum() - 1.0) < 1e-12
