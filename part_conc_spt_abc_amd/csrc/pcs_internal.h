@@ -109,6 +109,8 @@ struct DistArgs {
     uint32_t* tracks;           // binned tracks per proposal
     int32_t* status;            // PCS_OK or PCS_EOVERFLOW per proposal
     uint32_t colh_off;          // LDS word offset of the column histogram (set by launch_distance)
+    uint32_t srtl_off;          // LDS word offset of the sorted bins (set by launch_distance)
+    uint32_t srtl_cap;          // sorted 16-bit bins that fit in LDS (set by launch_distance)
 };
 
 // ------------------------------------------------ device-resident request loop
@@ -182,7 +184,7 @@ struct LoopArgs {
     uint32_t R, maxv;
     uint32_t stage_cap;     // LDS records per wave
     uint32_t nk_lds, nde_lds;
-    uint32_t pad;
+    uint32_t srtl_cap;      // sorted 16-bit bins of a distance that fit in LDS (after the column histogram)
     uint64_t idle_ticks;    // wall-clock ticks without work before the loop ends
 };
 
@@ -250,6 +252,10 @@ hipError_t launch_selftest_sqrt(uint32_t lo, uint32_t n, unsigned long long* bad
 hipError_t launch_serve_loop(const LoopArgs& a, uint32_t grid, uint32_t n_de_max, hipStream_t s);
 size_t serve_loop_lds(const LoopArgs& a);
 hipError_t serve_loop_occupancy(const LoopArgs& a, uint32_t n_de_max, int* groups_per_cu);
+// Sets a.srtl_cap to the sorted bins that fit in the LDS each of the loop's
+// groups leaves on a CU without costing a group per CU, then reports the
+// groups per CU at that size.
+hipError_t serve_loop_fit(LoopArgs& a, uint32_t n_de_max, int* groups_per_cu);
 hipError_t launch_prepare(const PrepArgs& a, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t s);
 // Sequential prefix sum of w (one lane, member order) into cum.

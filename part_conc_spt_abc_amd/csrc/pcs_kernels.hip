@@ -10,7 +10,9 @@
 //                    atomic per flush (replaces the per-track global atomicInc
 //                    of src/gpu/cusim/histogram2d.inl:98-106).
 //   distance_kernel  one workgroup per proposal: counting sort of the records
-//                    by row (LDS row counters, scratch slice in HBM), then a
+//                    by row (LDS row counters; the records are read once and
+//                    held in registers, the sorted 16-bit bins kept in LDS
+//                    when they fit), then a
 //                    row sweep over the 2D cumulative histogram with the
 //                    running column sums kept in registers: a row's records
 //                    enter through an LDS histogram and block scan, or, when
@@ -30,6 +32,11 @@
 //                    memset so it can sit inside a replayed hipGraph).
 //   selftest_*       exhaustive checks of the device math and a copy kernel
 //                    that calibrates the byte counters (test hooks).
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "pcs_device.h"
 #include "pcs_internal.h"
 
@@ -608,13 +615,17 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, 
 
 // LDS working set of the distance of one simulation: row counts / row
 // offsets (n_k), one row's column histogram (n_de), the block-scan partials,
-// the deepest occupied row and the final reduction (kDistRed words).
+// the deepest occupied row, the final reduction (kDistRed words) and room for
+// the bins sorted by row (srtl_cap 16-bit entries; a slice with more records
+// sorts them into its HBM scratch instead).
 struct DistLds {
     uint32_t* rowc;
     uint32_t* colh;
     uint32_t* wsum;
     uint32_t* smax;
     uint64_t* red;
+    uint16_t* srtl;
+    uint32_t srtl_cap;
 };
 constexpr int kDistRed = 2 * (kDistThreads / kWave);
 
@@ -644,7 +655,14 @@ __device__ __forceinline__ double dist_log10(const DistOut& o)
 
 // Exact distance of one simulation's record slice against the observed
 // summed-area table, by a whole 256-thread block. SEG * 256 >= n_de.
-template <int SEG>
+//
+// The slice is read from HBM once when it holds at most KREG * 256 records:
+// record t = tid + 256 u stays in register held[u] between the row count and
+// the scatter. Records past that are read a second time by the scatter. The
+// scatter writes the 16-bit bins sorted by row into LDS when the slice fits
+// L.srtl_cap, else into the slice's HBM scratch. Within a row the order of the
+// bins depends on the atomics; the sum depends only on the row's counts.
+template <int SEG, int KREG>
 __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSys& sy, uint32_t count, const DistLds& L)
 {
     uint32_t* rowc = L.rowc;
@@ -654,7 +672,6 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     const uint32_t tid = threadIdx.x;
     const uint32_t n_k = sy.n_k, n_de = sy.n_de;
     const uint32_t* rec = pr.rec;
-    uint32_t* srt = pr.srt;
 
     if (count > pr.rec_cap)  // record slice overflowed: counted, not stored
         return DistOut{0ull, 0ull, count, PCS_EOVERFLOW};
@@ -668,10 +685,32 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     if (tid == 0) *L.smax = 0u;
     __syncthreads();
 
-    // Row histogram of the records and the deepest occupied row. The records
-    // come from HBM: each thread keeps kDistLoads loads in flight.
+    // Row histogram of the records and the deepest occupied row.
+    constexpr uint32_t kHeldMax = (uint32_t)KREG * kDistThreads;
+    const uint32_t nheld = min(T, kHeldMax);
+    // A register past the slice holds kNoRecord (row 0xffff > PCS_MAX_NK).
+    // The scatter takes the held words through an empty asm statement, so
+    // that nothing the count derived from them (rows, lane masks) is kept
+    // alive across the scan in place of being recomputed.
+    constexpr uint32_t kNoRecord = ~0u;
+    uint32_t held[KREG > 0 ? KREG : 1];
     uint32_t mr = 0u;
-    for (uint32_t t0 = tid; t0 < T; t0 += kDistLoads * kDistThreads) {
+#pragma unroll
+    for (int u = 0; u < KREG; ++u) {
+        const uint32_t t = tid + (uint32_t)u * kDistThreads;
+        held[u] = kNoRecord;
+        if ((uint32_t)u * kDistThreads < nheld) held[u] = (t < nheld) ? rec[t] : kNoRecord;
+    }
+#pragma unroll
+    for (int u = 0; u < KREG; ++u) {
+        if (held[u] != kNoRecord) {
+            const uint32_t r = held[u] >> 16;
+            atomicAdd(&rowc[r], 1u);
+            mr = max(mr, r);
+        }
+    }
+    // Past the held records: each thread keeps kDistLoads loads in flight.
+    for (uint32_t t0 = nheld + tid; t0 < T; t0 += kDistLoads * kDistThreads) {
         uint32_t v[kDistLoads];
 #pragma unroll
         for (int u = 0; u < kDistLoads; ++u) {
@@ -707,7 +746,21 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     __syncthreads();
 
     // Scatter bins by row; afterwards rowc[r] is one past the last slot of row r.
-    for (uint32_t t0 = tid; t0 < T; t0 += kDistLoads * kDistThreads) {
+    const bool in_lds = T <= L.srtl_cap;
+    uint16_t* srtl = L.srtl;
+    uint16_t* srtg = reinterpret_cast<uint16_t*>(pr.srt);
+    auto put = [&](uint32_t v) {
+        const uint32_t pos = atomicAdd(&rowc[v >> 16], 1u);
+        if (in_lds) srtl[pos] = (uint16_t)v;
+        else srtg[pos] = (uint16_t)v;
+    };
+#pragma unroll
+    for (int u = 0; u < KREG; ++u) {
+        uint32_t v = held[u];
+        asm volatile("" : "+v"(v));
+        if (v != kNoRecord) put(v);
+    }
+    for (uint32_t t0 = nheld + tid; t0 < T; t0 += kDistLoads * kDistThreads) {
         uint32_t v[kDistLoads];
 #pragma unroll
         for (int u = 0; u < kDistLoads; ++u) {
@@ -715,15 +768,12 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
             v[u] = (t < T) ? rec[t] : 0u;
         }
 #pragma unroll
-        for (int u = 0; u < kDistLoads; ++u) {
-            if (t0 + (uint32_t)u * kDistThreads < T) {
-                const uint32_t pos = atomicAdd(&rowc[v[u] >> 16], 1u);
-                srt[pos] = v[u] & 0xffffu;
-            }
-        }
+        for (int u = 0; u < kDistLoads; ++u)
+            if (t0 + (uint32_t)u * kDistThreads < T) put(v[u]);
     }
     __threadfence_block();
     __syncthreads();
+    auto bin_at = [&](uint32_t t) -> uint32_t { return in_lds ? (uint32_t)srtl[t] : (uint32_t)srtg[t]; };
 
     const uint32_t last = max(*L.smax, sy.maxrow_obs);  // < n_k
     const uint32_t j0 = tid * SEG;
@@ -737,8 +787,10 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     const uint32_t lane = tid % kWave;
 
     // Observed rows are loaded PF rows ahead of their use into a ring of
-    // register slots (slot d holds row i0 + d); a refill asks for row
-    // min(i + PF, last), which exists, so no load leaves the table.
+    // register slots (slot d holds row i0 + d). Past the deepest observed row
+    // the table repeats that row, so a refill asks for row
+    // min(i + PF, maxrow_obs): no load leaves the table, and the rows past it
+    // are all served from one row in cache.
     constexpr int PF = SEG >= 8 ? 2 : (SEG >= 4 ? 4 : 8);
     uint32_t rb[PF][SEG];
     auto load_row = [&](uint32_t (&dst)[SEG], uint32_t row) {
@@ -747,7 +799,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
         for (int m = 0; m < SEG; ++m) dst[m] = (j0 + m < n_de) ? Rrow[j0 + m] : 0u;
     };
 #pragma unroll
-    for (int d = 0; d < PF; ++d) load_row(rb[d], min((uint32_t)d, last));
+    for (int d = 0; d < PF; ++d) load_row(rb[d], min((uint32_t)d, maxrow_obs));
 
     uint64_t racc_prev = 0ull, racc_prev_hi = 0ull;
     for (uint32_t i0 = 0; i0 <= last; i0 += PF) {
@@ -766,14 +818,14 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
                 if (nr > 0u && nr <= (uint32_t)kDistSparseRow) {
                     // A few records: every wave reads them once (one per lane)
                     // and each thread adds those at or left of its columns.
-                    const uint32_t mine = (lane < nr) ? srt[s0 + lane] : 0u;
+                    const uint32_t mine = (lane < nr) ? bin_at(s0 + lane) : 0u;
                     for (uint32_t q = 0u; q < nr; ++q) {
                         const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)q);
 #pragma unroll
                         for (int m = 0; m < SEG; ++m) Ccol[m] += (j0 + (uint32_t)m >= b) ? 1u : 0u;
                     }
                 } else if (nr > 0u) {
-                    for (uint32_t t = s0 + tid; t < s1; t += kDistThreads) atomicAdd(&colh[srt[t]], 1u);
+                    for (uint32_t t = s0 + tid; t < s1; t += kDistThreads) atomicAdd(&colh[bin_at(t)], 1u);
                     __syncthreads();
                     uint32_t pre[SEG];
                     uint32_t run = 0u;
@@ -810,7 +862,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
                 racc_prev = racc;
                 racc_prev_hi = racc_hi;
             }
-            load_row(rb[d], min(i + (uint32_t)PF, last));
+            load_row(rb[d], min(i + (uint32_t)PF, maxrow_obs));
         }
     }
 
@@ -834,22 +886,29 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     return DistOut{lo, hi, T, PCS_OK};
 }
 
-// Dynamic LDS: row counts (n_k_max words, rounded up to 4) then one row's
-// column histogram (n_de_max words), sized to the models of the launch so
-// that more workgroups share a CU.
+// Records a distance workgroup holds in registers between the row count and
+// the scatter (per thread; the register peak stays at or below the row
+// sweep's, so occupancy is unchanged).
+constexpr int kDistHeld = 32;
+
+// Dynamic LDS: row counts (n_k_max words, rounded up to 4), one row's column
+// histogram (n_de_max words), then srtl_cap 16-bit sorted bins; sized to the
+// models of the launch and to the occupancy the registers allow.
 template <int SEG>
 __global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
 {
     extern __shared__ uint32_t dist_lds[];
     uint32_t* rowc = dist_lds;
     uint32_t* colh = dist_lds + a.colh_off;
+    uint16_t* srtl = reinterpret_cast<uint16_t*>(dist_lds + a.srtl_off);
     __shared__ uint32_t wsum[kDistThreads / kWave];
     __shared__ uint32_t smax;
     __shared__ uint64_t red[kDistRed];
 
     const uint32_t P = a.prop_lo + blockIdx.x;
     const DevProp& pr = a.props[P];
-    const DistOut o = distance_block<SEG>(pr, a.sys[pr.sys_id], a.rec_count[P], DistLds{rowc, colh, wsum, &smax, red});
+    const DistOut o = distance_block<SEG, kDistHeld>(pr, a.sys[pr.sys_id], a.rec_count[P],
+                                                     DistLds{rowc, colh, wsum, &smax, red, srtl, a.srtl_cap});
     if (threadIdx.x == 0) {  // an overflowed slice is rerun by the host
         a.exact[P] = dist_exact(o);
         a.dist[P] = dist_log10(o);
@@ -1103,7 +1162,8 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
     __syncthreads();
     const float2* tab = reinterpret_cast<const float2*>(tabf);
     uint32_t* rowc = loop_lds + kLoopWaves * a.stage_cap + kAngleTabFloats;
-    const DistLds L{rowc, rowc + a.nk_lds, wsum, &smax, red};
+    const DistLds L{rowc, rowc + a.nk_lds, wsum, &smax, red, reinterpret_cast<uint16_t*>(rowc + a.nk_lds + a.nde_lds),
+                    a.srtl_cap};
     const uint32_t vw = 3u * (a.maxv + 1u);
     // Polling: one idle group per CU holds the CU's poller token and polls
     // every sleep unit (~3.4 us); the CU's other idle groups back off from 1
@@ -1216,7 +1276,7 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
             const DevProp pr = load_uniform(&S.prop);
             const DevSys sy = load_uniform(&a.sys[pr.sys_id]);
             const uint32_t count = uniform(ld_agent32(&S.rec_count));
-            const DistOut o = distance_block<SEG>(pr, sy, count, L);
+            const DistOut o = distance_block<SEG, 0>(pr, sy, count, L);
             if (tid == 0u) {
                 LoopDone& d = a.done[slot];
                 d.exact = dist_exact(o);
@@ -1769,6 +1829,62 @@ hipError_t launch_tracks(const WalkArgs& a, bool gauss, bool zref, hipStream_t s
     return hipGetLastError();
 }
 
+// Sorted 16-bit bins that fit in the LDS a workgroup of `fn` may add to its
+// `base` bytes of dynamic LDS without lowering the workgroups per CU that its
+// registers and `base` allow; at most `most`, a multiple of 64, 0 if none.
+static uint32_t lds_bins_free(const void* fn, int threads, size_t base, uint32_t most)
+{
+    int dev = 0, lds_cu = 0, lds_block = 0, n = 0;
+    hipFuncAttributes fa{};
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        hipFuncGetAttributes(&fa, fn) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, base) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        return 0u;
+    }
+    const int64_t per_block = std::min<int64_t>((int64_t)lds_cu / n, lds_block);
+    const int64_t room = per_block - (int64_t)base - (int64_t)fa.sharedSizeBytes;
+    uint32_t cap = room > 0 ? (uint32_t)std::min<int64_t>(room / 2, most) & ~63u : 0u;
+    int n2 = 0;
+    while (cap > 0u) {
+        const size_t sz = base + ((size_t)cap + 1u) / 2u * sizeof(uint32_t);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n2, fn, threads, sz) == hipSuccess && n2 >= n) break;
+        (void)hipGetLastError();
+        cap = cap > 1024u ? (cap - 1024u) & ~63u : 0u;
+    }
+    return cap;
+}
+
+// LDS sorted-bin capacity of distance_kernel<SEG> per (device, base size),
+// found once: the occupancy queries are host calls that do not touch a stream.
+template <int SEG>
+static uint32_t dist_lds_bins(size_t base)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, size_t>, uint32_t> memo;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(dev, base);
+    auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+    const uint32_t cap = lds_bins_free(reinterpret_cast<const void*>(distance_kernel<SEG>), kDistThreads, base,
+                                       (uint32_t)kDistHeld * kDistThreads);
+    memo.emplace(key, cap);
+    return cap;
+}
+
+template <int SEG>
+static void launch_distance_t(DistArgs a, size_t base, hipStream_t s)
+{
+    a.srtl_off = (uint32_t)(base / sizeof(uint32_t));
+    a.srtl_cap = dist_lds_bins<SEG>(base);
+    const size_t shmem = base + ((size_t)a.srtl_cap + 1u) / 2u * sizeof(uint32_t);
+    hipLaunchKernelGGL(distance_kernel<SEG>, dim3(a.n_props), dim3(kDistThreads), shmem, s, a);
+}
+
 hipError_t launch_distance(const DistArgs& args, uint32_t n_k_max, uint32_t n_de_max, hipStream_t s)
 {
     (void)hipGetLastError();  // report only this launch's error
@@ -1777,19 +1893,19 @@ hipError_t launch_distance(const DistArgs& args, uint32_t n_k_max, uint32_t n_de
         return hipErrorInvalidValue;
     DistArgs a = args;
     a.colh_off = (n_k_max + 3u) & ~3u;
-    const size_t shmem = ((size_t)a.colh_off + n_de_max) * sizeof(uint32_t);
-    const dim3 g(a.n_props), b(kDistThreads);
-    if (n_de_max <= 256) hipLaunchKernelGGL(distance_kernel<1>, g, b, shmem, s, a);
-    else if (n_de_max <= 512) hipLaunchKernelGGL(distance_kernel<2>, g, b, shmem, s, a);
-    else if (n_de_max <= 1024) hipLaunchKernelGGL(distance_kernel<4>, g, b, shmem, s, a);
-    else if (n_de_max <= 2048) hipLaunchKernelGGL(distance_kernel<8>, g, b, shmem, s, a);
-    else hipLaunchKernelGGL(distance_kernel<16>, g, b, shmem, s, a);
+    const size_t base = ((size_t)a.colh_off + ((n_de_max + 3u) & ~3u)) * sizeof(uint32_t);
+    if (n_de_max <= 256) launch_distance_t<1>(a, base, s);
+    else if (n_de_max <= 512) launch_distance_t<2>(a, base, s);
+    else if (n_de_max <= 1024) launch_distance_t<4>(a, base, s);
+    else if (n_de_max <= 2048) launch_distance_t<8>(a, base, s);
+    else launch_distance_t<16>(a, base, s);
     return hipGetLastError();
 }
 
 size_t serve_loop_lds(const LoopArgs& a)
 {
-    return ((size_t)kLoopWaves * a.stage_cap + kAngleTabFloats + a.nk_lds + a.nde_lds) * sizeof(uint32_t);
+    return ((size_t)kLoopWaves * a.stage_cap + kAngleTabFloats + a.nk_lds + a.nde_lds + (a.srtl_cap + 1u) / 2u) *
+           sizeof(uint32_t);
 }
 
 template <int SEG>
@@ -1797,6 +1913,25 @@ static hipError_t loop_occ_t(const LoopArgs& a, int* n)
 {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, serve_loop_kernel<SEG>, kLoopWaves * kWave,
                                                         serve_loop_lds(a));
+}
+
+template <int SEG>
+static hipError_t loop_fit_t(LoopArgs& a, int* n)
+{
+    a.srtl_cap = 0u;
+    const size_t base = serve_loop_lds(a);
+    a.srtl_cap = lds_bins_free(reinterpret_cast<const void*>(serve_loop_kernel<SEG>), kLoopWaves * kWave, base,
+                               1u << 16);
+    return loop_occ_t<SEG>(a, n);
+}
+
+hipError_t serve_loop_fit(LoopArgs& a, uint32_t n_de_max, int* groups_per_cu)
+{
+    if (n_de_max <= 256) return loop_fit_t<1>(a, groups_per_cu);
+    if (n_de_max <= 512) return loop_fit_t<2>(a, groups_per_cu);
+    if (n_de_max <= 1024) return loop_fit_t<4>(a, groups_per_cu);
+    if (n_de_max <= 2048) return loop_fit_t<8>(a, groups_per_cu);
+    return loop_fit_t<16>(a, groups_per_cu);
 }
 
 hipError_t serve_loop_occupancy(const LoopArgs& a, uint32_t n_de_max, int* groups_per_cu)

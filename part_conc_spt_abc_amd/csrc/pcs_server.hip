@@ -568,7 +568,7 @@ int loop_launch(pcs_server* s, uint32_t nsys)
     a.nde_lds = nde;
     a.idle_ticks = L.idle_ticks;
     int gpc = 0;
-    hipError_t e = serve_loop_occupancy(a, nde, &gpc);
+    hipError_t e = serve_loop_fit(a, nde, &gpc);
     if (e != hipSuccess) return hip_error(e, "serving loop occupancy");
     if (gpc < 1) return set_error(PCS_EINVAL, "serving loop does not fit a CU (n_k %u, n_de %u)", nk, nde);
     // group 0 admits requests, the others walk them
