@@ -1687,13 +1687,38 @@ __global__ __launch_bounds__(256) void pmc_sums_kernel(const double* xn, uint32_
     if (j < n_new) part[(uint64_t)blockIdx.y * n_new + j] = acc;
 }
 
-// Cumulative weights in member order, one lane: the same FP64 sums as a
-// sequential cumsum (a population is set once per generation).
-__global__ void cumsum_kernel(const double* w, uint32_t n, double* cum)
+// Cumulative weights in member order: the same FP64 sums as a sequential
+// cumsum. One wave loads kCumLoads x 64 weights at a time (coalesced, all in
+// flight), then one running sum passes through them lane by lane (readlane of
+// each weight, a wave-uniform add), and each lane keeps the sum at its member.
+constexpr int kCumLoads = 8;
+__global__ __launch_bounds__(kWave) void cumsum_kernel(const double* w, uint32_t n, double* cum)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint32_t lane = threadIdx.x;
     double acc = 0.0;
-    for (uint32_t i = 0; i < n; ++i) cum[i] = (acc += w[i]);
+    for (uint32_t base = 0; base < n; base += (uint32_t)(kCumLoads * kWave)) {
+        double v[kCumLoads];
+#pragma unroll
+        for (int u = 0; u < kCumLoads; ++u) {
+            const uint32_t i = base + (uint32_t)u * kWave + lane;
+            v[u] = i < n ? w[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kCumLoads; ++u) {
+            const uint32_t first = base + (uint32_t)u * kWave;
+            if (first >= n) break;
+            const uint32_t nl = min((uint32_t)kWave, n - first);
+            const uint64_t bits = __double_as_longlong(v[u]);
+            double mine = 0.0;
+            for (uint32_t l = 0; l < nl; ++l) {
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits, (int)l);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), (int)l);
+                acc += __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+                mine = (lane == l) ? acc : mine;
+            }
+            if (lane < nl) cum[first + lane] = mine;
+        }
+    }
 }
 
 // Parameter rows in units of their displacement scale (the pmc kernel's
@@ -1976,7 +2001,7 @@ hipError_t launch_cumsum(const double* w, uint32_t n, double* cum, hipStream_t s
 {
     (void)hipGetLastError();  // report only this launch's error
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(cumsum_kernel, dim3(1), dim3(64), 0, s, w, n, cum);
+    hipLaunchKernelGGL(cumsum_kernel, dim3(1), dim3(kWave), 0, s, w, n, cum);
     return hipGetLastError();
 }
 
