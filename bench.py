@@ -69,8 +69,15 @@ ISSUE_PEAK = 157.3e12 / 2
 PHILOX_BLOCK_OPS = 104
 
 
-def walk_ops_per_step(gauss: bool = False) -> float:
-    philox = (3.0 if gauss else 2.0) / 4.0 * PHILOX_BLOCK_OPS
+def walk_ops_per_step(gauss: bool = False, words_per_normal: float = 0.0) -> float:
+    """Algorithmic lane-ops per particle-step of the implemented stream (2
+    Philox blocks per 4 steps, 3 with Gaussian detection). words_per_normal
+    > 0 counts a stream that draws that many Philox words per unit normal
+    instead (1.0: one word each, 0.75 blocks per step, 123 ops)."""
+    if words_per_normal > 0.0:
+        philox = (3.0 * words_per_normal + (1.0 if gauss else 0.0)) / 4.0 * PHILOX_BLOCK_OPS
+    else:
+        philox = (3.0 if gauss else 2.0) / 4.0 * PHILOX_BLOCK_OPS
     normals = 1.5 * (2 * 2 + 8)
     step = 27.0
     detect = 6.0 if gauss else 0.0
@@ -662,6 +669,11 @@ def main():
             "kernel": "walk_kernel",
             "kernel_ms": walk_s * 1e3,
             "algorithmic_ops_per_particle_step": ops_step,
+            # the same fraction with one Philox word per unit normal counted
+            # (0.75 blocks per step: 123 ops), the count this stream's two
+            # blocks per four steps undercut
+            "frac_one_word_per_normal": (ach / FP32_PEAK * walk_ops_per_step(words_per_normal=1.0) / ops_step)
+            if ach else None,
             "issue_slot_util": (issue / ISSUE_PEAK) if issue else None,
             "issued_valu_lane_instr_per_particle_step": prof["valu_lane_instr_per_particle_step"],
             "issued_and_traffic_from": str(PMC_PROFILE.relative_to(ROOT)),
