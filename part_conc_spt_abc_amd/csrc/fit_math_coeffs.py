@@ -14,7 +14,11 @@ hex floats.
   plus beta = 2 pi (r - 127.5) / 2^16: SINCOS_TAB holds sin and cos of the
   sector centres (i = 0..255), BETA_TAB holds sin beta and cos beta - 1
   (r = 0..255); every entry computed to 50 digits and rounded to
-  nearest-even FP32.
+  nearest-even FP32;
+- the FP64 table of 2^(j/64), j = 0..63, for the pmc weights' exponential,
+  and its Cody-Waite split of ln2 / 64 (a head of 35 significant bits, so
+  that k * head is exact for |k| < 2^18, and the rounded tail), both from 50
+  digits, rounded to nearest FP64.
 """
 import mpmath
 import numpy as np
@@ -76,6 +80,19 @@ def beta_table():
     return tab
 
 
+def exp2_64_table():
+    mpmath.mp.dps = 50
+    return [float(mpmath.power(2, mpmath.mpf(j) / 64)) for j in range(64)]
+
+
+def ln2_64_split():
+    mpmath.mp.dps = 50
+    c = mpmath.log(2) / 64
+    e = int(mpmath.floor(mpmath.log(c, 2)))
+    hi = mpmath.nint(c / mpmath.power(2, e - 34)) * mpmath.power(2, e - 34)  # 35 significant bits
+    return float(hi), float(c - hi), float(64 / mpmath.log(2))
+
+
 def main():
     a, b = np.sqrt(0.5) - 1, np.sqrt(2) - 1
     x = cheb(a, b)
@@ -88,6 +105,8 @@ def main():
     print("EXP2_Q", [hexf(v) for v in exq])
     print("SINCOS_TAB", [float.hex(v) for v in sincos_table()])
     print("BETA_TAB", [float.hex(v) for v in beta_table()])
+    print("EXP2_64_TAB", [float.hex(v) for v in exp2_64_table()])
+    print("LN2_64_HI_LO_INV", [float.hex(v) for v in ln2_64_split()])
 
 
 if __name__ == "__main__":

@@ -1631,56 +1631,88 @@ __global__ void selftest_copy_kernel(const uint4* __restrict__ src, uint4* __res
 constexpr int kPmcMaxP = 3 * PCS_MAX_SPECIES;
 constexpr uint32_t kPmcChunk = 4096;
 
-// e^y for y <= 0: y = n ln 2 + r (Cody-Waite, ln 2 in a 42-bit head and a
-// tail), |r| <= ln2 / 2, e^r by its Taylor series to degree 13 (remainder
-// below 5e-18 relative), 2^n by ldexp; 0 below the smallest subnormal.
-__device__ __forceinline__ double exp_nonpos(double y)
+// 2^(j/64), j = 0..63, rounded to nearest FP64 (fit_math_coeffs.py), and ln2/64
+// split into a head of 35 significant bits (k * head is exact for |k| < 2^18)
+// and the rounded tail.
+__constant__ double kExp2Tab64[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
+    0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
+    0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,
+    0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,
+    0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,
+    0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,
+    0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,
+    0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,
+    0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,
+    0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0,
+};
+constexpr double kLn2d64Hi = 0x1.62e42fefc0000p-7, kLn2d64Lo = -0x1.c610ca86c3899p-43;
+constexpr double k64dLn2 = 0x1.71547652b82fep+6;
+
+// e^y for y <= 0, tab = kExp2Tab64 in LDS: y = (64 e + j) ln2 / 64 + r with
+// k = 64 e + j = rint(64 y / ln2) (Cody-Waite), |r| <= ln2 / 128, e^r by its
+// Taylor series to degree 5 (remainder below 4e-17 relative), then
+// 2^(j/64) e^r scaled by 2^e; within 3 ulp. y is clamped at -800, where the
+// result has long underflowed to 0.
+__device__ __forceinline__ double exp_nonpos(double y, const double* tab)
 {
-    constexpr double T[14] = {
-        0x1.0000000000000p+0, 0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5555555555555p-3,
-        0x1.5555555555555p-5, 0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
-        0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
-        0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33,
-    };
-    if (!(y > -745.2)) return 0.0;
-    const double n = __builtin_rint(y * 0x1.71547652b82fep+0);
-    double r = __builtin_fma(-n, 0x1.62e42fefa3800p-1, y);
-    r = __builtin_fma(-n, 0x1.ef35793c76730p-45, r);
-    double p = T[13];
-#pragma unroll
-    for (int k = 12; k >= 0; --k) p = __builtin_fma(p, r, T[k]);
-    return __builtin_ldexp(p, (int)n);
+    y = __builtin_fmax(y, -800.0);
+    const double k = __builtin_rint(y * k64dLn2);
+    double r = __builtin_fma(-k, kLn2d64Hi, y);
+    r = __builtin_fma(-k, kLn2d64Lo, r);
+    double p = __builtin_fma(r, 0x1.1111111111111p-7, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    const int ki = (int)k;
+    return __builtin_ldexp(tab[ki & 63] * p, ki >> 6);
 }
 
-template <int PMAX>
+// P = the number of parameter rows when it is known at compile time (1..8);
+// P = 0 reads it from p_rt (up to kPmcMaxP). Dynamic LDS: the exponential's
+// table (64 doubles), then the rows of 256 staged old members, then their
+// weights.
+template <int P>
 __global__ __launch_bounds__(256) void pmc_sums_kernel(const double* xn, uint32_t n_new, const double* xo,
-                                                       const double* w_old, uint32_t n_old, uint32_t P, double* part)
+                                                       const double* w_old, uint32_t n_old, uint32_t p_rt, double* part)
 {
-    extern __shared__ double pmc_lds[];  // P rows of 256 old coordinates, then 256 weights
+    constexpr int PM = P > 0 ? P : kPmcMaxP;
+    extern __shared__ double pmc_lds[];
+    const uint32_t np = P > 0 ? (uint32_t)P : p_rt;
     const uint32_t tid = threadIdx.x;
     const uint32_t j = blockIdx.x * 256u + tid;
     const uint32_t i_lo = blockIdx.y * kPmcChunk, i_hi = min(n_old, i_lo + kPmcChunk);
-    double* shw = pmc_lds + (size_t)P * 256u;
-    double x[PMAX];
+    double* tab = pmc_lds;
+    double* sx = pmc_lds + 64;
+    double* shw = sx + (size_t)np * 256u;
+    if (tid < 64u) tab[tid] = kExp2Tab64[tid];  // first read after the loop's barrier
+    double x[PM];
 #pragma unroll
-    for (int p = 0; p < PMAX; ++p) x[p] = ((uint32_t)p < P && j < n_new) ? xn[(uint64_t)p * n_new + j] : 0.0;
+    for (int p = 0; p < PM; ++p) x[p] = ((uint32_t)p < np && j < n_new) ? xn[(uint64_t)p * n_new + j] : 0.0;
     double acc = 0.0;
     for (uint32_t t0 = i_lo; t0 < i_hi; t0 += 256u) {
         const uint32_t i = t0 + tid;
-        for (uint32_t p = 0; p < P; ++p) pmc_lds[p * 256u + tid] = i < i_hi ? xo[(uint64_t)p * n_old + i] : 0.0;
+        for (uint32_t p = 0; p < np; ++p) sx[p * 256u + tid] = i < i_hi ? xo[(uint64_t)p * n_old + i] : 0.0;
         shw[tid] = i < i_hi ? w_old[i] : 0.0;
         __syncthreads();
         const uint32_t m = min(256u, i_hi - t0);
         for (uint32_t ii = 0; ii < m; ++ii) {
             double s2 = 0.0;
 #pragma unroll
-            for (int p = 0; p < PMAX; ++p) {
-                if ((uint32_t)p < P) {
-                    const double d = x[p] - pmc_lds[p * 256u + ii];
+            for (int p = 0; p < PM; ++p) {
+                if (P > 0 || (uint32_t)p < np) {
+                    const double d = x[p] - sx[p * 256u + ii];
                     s2 = __builtin_fma(d, d, s2);
                 }
             }
-            acc = __builtin_fma(shw[ii], exp_nonpos(-0.5 * s2), acc);
+            acc = __builtin_fma(shw[ii], exp_nonpos(-0.5 * s2, tab), acc);
         }
         __syncthreads();
     }
@@ -1759,10 +1791,19 @@ hipError_t launch_pmc_sums(const double* xn, uint32_t n_new, const double* xo, c
     if (P == 0 || P > (uint32_t)kPmcMaxP) return hipErrorInvalidValue;
     const uint32_t chunks = pmc_chunks(n_old);
     const dim3 g((n_new + 255u) / 256u, chunks);
-    const size_t lds = ((size_t)P + 1u) * 256u * sizeof(double);
-    if (P <= 4) hipLaunchKernelGGL(pmc_sums_kernel<4>, g, dim3(256), lds, s, xn, n_new, xo, w_old, n_old, P, part);
-    else if (P <= 8) hipLaunchKernelGGL(pmc_sums_kernel<8>, g, dim3(256), lds, s, xn, n_new, xo, w_old, n_old, P, part);
-    else hipLaunchKernelGGL(pmc_sums_kernel<kPmcMaxP>, g, dim3(256), lds, s, xn, n_new, xo, w_old, n_old, P, part);
+    const size_t lds = (64u + ((size_t)P + 1u) * 256u) * sizeof(double);
+    const dim3 b(256);
+    switch (P) {
+    case 1: hipLaunchKernelGGL(pmc_sums_kernel<1>, g, b, lds, s, xn, n_new, xo, w_old, n_old, P, part); break;
+    case 2: hipLaunchKernelGGL(pmc_sums_kernel<2>, g, b, lds, s, xn, n_new, xo, w_old, n_old, P, part); break;
+    case 3: hipLaunchKernelGGL(pmc_sums_kernel<3>, g, b, lds, s, xn, n_new, xo, w_old, n_old, P, part); break;
+    case 4: hipLaunchKernelGGL(pmc_sums_kernel<4>, g, b, lds, s, xn, n_new, xo, w_old, n_old, P, part); break;
+    case 5: hipLaunchKernelGGL(pmc_sums_kernel<5>, g, b, lds, s, xn, n_new, xo, w_old, n_old, P, part); break;
+    case 6: hipLaunchKernelGGL(pmc_sums_kernel<6>, g, b, lds, s, xn, n_new, xo, w_old, n_old, P, part); break;
+    case 7: hipLaunchKernelGGL(pmc_sums_kernel<7>, g, b, lds, s, xn, n_new, xo, w_old, n_old, P, part); break;
+    case 8: hipLaunchKernelGGL(pmc_sums_kernel<8>, g, b, lds, s, xn, n_new, xo, w_old, n_old, P, part); break;
+    default: hipLaunchKernelGGL(pmc_sums_kernel<0>, g, b, lds, s, xn, n_new, xo, w_old, n_old, P, part); break;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(pmc_reduce_kernel, dim3((n_new + 255u) / 256u), dim3(256), 0, s, part, n_new, chunks, wsum);

@@ -424,15 +424,20 @@ def test_generate_tracks_matches_oracle(gpu_lib):
         assert got == want, variant
 
 
-def test_pmc_weights_on_device(gpu_lib):
+@pytest.mark.parametrize("P", [1, 4, 5, 8, 9, 15])
+def test_pmc_weights_on_device(gpu_lib, P):
     """Device pmc-standard weights vs the formula in FP64 numpy
-    (normpdf products with their constants, then normalisation)."""
+    (normpdf products with their constants, then normalisation), for
+    parameter counts with their own kernel (1..8) and the general one (9,
+    15), with old members from two chunks, and one new member twenty
+    standard deviations out on its first row (kernel terms near e^-100)."""
     import ctypes as C
 
-    rng = np.random.default_rng(8)
-    P, n_old, n_new = 5, 700, 900
-    old = rng.normal(0.0, 1.0, (P, n_old)) * np.array([[0.5], [2.0], [1.0], [0.3], [3.0]])
+    rng = np.random.default_rng(8 + P)
+    n_old, n_new = 4500, 500
+    old = rng.normal(0.0, 1.0, (P, n_old)) * rng.choice([0.3, 0.5, 1.0, 2.0, 3.0], (P, 1))
     new = old[:, rng.integers(0, n_old, n_new)] + rng.normal(0.0, 0.2, (P, n_new))
+    new[0, 7] += 20.0 * old[0].std()
     w_old = rng.random(n_old)
     w_old /= w_old.sum()
     tau = np.sqrt(2.0 * old.var(axis=1))
