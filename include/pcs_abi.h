@@ -264,7 +264,8 @@ int pcs_get_tuning(pcs_ctx* ctx, pcs_tuning* t);
  * launch shape - particles per lane {1,2} x waves per group {2,4} x groups
  * per CU {2,4,6,8,12,16} (those the hardware can co-schedule) - and keeps
  * the fastest in the context. best/ms (nullable) receive the winner and its
- * walk time (best of 2 runs after a warmup). Results never depend on the shape. */
+ * walk time (the faster of 2 runs; one untimed run warms each kernel first).
+ * Results never depend on the shape. */
 int pcs_autotune(pcs_ctx* ctx, const pcs_proposal* props, uint32_t n, const uint32_t* video_frames,
                  const uint32_t* video_particles, uint32_t n_videos, pcs_tuning* best, float* ms);
 

@@ -264,8 +264,15 @@ void Estimator::autotune_once(const std::vector<pcs_proposal>& props, const std:
 {
     if (tuned_ || !cfg_.autotune_min || props.size() < cfg_.autotune_min) return;
     tuned_ = true;
-    // the first proposals hold videos [0, k J) in order
-    const std::size_t k = std::min<std::size_t>(props.size(), 4000);
+    // The first proposals hold videos [0, k J) in order. Enough of them for
+    // about 5e9 particle-steps (some 10 ms a run: the tuner times 20 launch
+    // shapes twice), at most 4000.
+    std::size_t k = 0;
+    double steps = 0.0;
+    while (k < std::min<std::size_t>(props.size(), 4000) && steps < 5e9) {
+        for (std::size_t v = 0; v < J; ++v) steps += (double)parts[k * J + v] * (double)frames[k * J + v];
+        ++k;
+    }
     pcs_tuning best{};
     float ms = 0.0f;
     check(pcs_autotune(ctx_, props.data(), (uint32_t)k, frames.data(), parts.data(), (uint32_t)(k * J), &best, &ms),

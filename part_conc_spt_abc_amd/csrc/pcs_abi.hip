@@ -1352,21 +1352,25 @@ int pcs_autotune(pcs_ctx* c, const pcs_proposal* props, uint32_t n, const uint32
     float best_ms = -1.0f;
     int r = pcs_batch_load(b, props, n, vframes, vparticles, n_videos);
     for (uint32_t ppl : {1u, 2u}) {
+        bool warm = false;  // one untimed run per kernel (particles per lane)
         for (uint32_t wpg : {2u, 4u}) {
             for (uint32_t gpc : {2u, 4u, 6u, 8u, 12u, 16u}) {
                 if (r != PCS_OK) break;
                 if (wpg * gpc > 32u) continue;  // more waves than a CU holds (8 per SIMD)
                 pcs_tuning t{ppl, wpg, gpc, saved.stage_records};
                 c->tune = t;
-                float times[3];
-                for (int k = 0; k < 3 && r == PCS_OK; ++k) {
+                if (!warm) {
+                    r = pcs_batch_run(b, 0x5eedull, PCS_MODE_BATCHED_DIRECT);
+                    warm = true;
+                }
+                float t_ms = -1.0f;  // the faster of two timed runs
+                for (int k = 1; k <= 2 && r == PCS_OK; ++k) {
                     r = pcs_batch_run(b, 0x5eedull + (uint64_t)k, PCS_MODE_BATCHED_DIRECT);
                     float tm[3];
                     if (r == PCS_OK) r = pcs_batch_timing(b, tm);
-                    times[k] = tm[1];
+                    if (r == PCS_OK && (t_ms < 0.0f || tm[1] < t_ms)) t_ms = tm[1];
                 }
                 if (r != PCS_OK) break;
-                const float t_ms = std::min(times[1], times[2]);  // run 0 is the warmup
                 if (best_ms < 0.0f || t_ms < best_ms) {
                     best_ms = t_ms;
                     best = t;
