@@ -33,7 +33,7 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
 
     structs = {n: getattr(_abi, n) for n in ("pcs_system", "pcs_proposal", "pcs_tuning", "pcs_server_config",
-                                             "pcs_request", "pcs_completion")}
+                                             "pcs_request", "pcs_completion", "pcs_prior")}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pcs_abi.h"', "int main(void){"]
     for n, cls in structs.items():
         lines.append(f'printf("{n} %zu\\n", sizeof({n}));')
@@ -63,6 +63,19 @@ def test_host_only_calls_without_gpu():
     h = C.c_void_p()
     assert lib.pcs_open(C.byref(m.to_abi()), None, 0, C.byref(h)) == _abi.PCS_EINVAL
     assert b"kmin" in lib.pcs_last_error()
+    # NULL handles of the proposal step, the resident population, the
+    # allocation-free entry point and the server are refused before any HIP call
+    pr = _abi.pcs_prior()
+    one = (C.c_double * 4)()
+    out = (C.c_double * 4)()
+    assert lib.pcs_batch_propose(None, C.byref(pr), one, one, one, 1, 1, None, 1, 0, 0, out) == _abi.PCS_EINVAL
+    assert lib.pcs_batch_set_population(None, one, one, 1, 1) == _abi.PCS_EINVAL
+    assert lib.pcs_batch_pmc_weights(None, one, out) == _abi.PCS_EINVAL
+    assert lib.pcs_batch_read_proposals(None, None, None) == _abi.PCS_EINVAL
+    assert lib.pcs_simulate_static(None, None, 0, None, None, 0, 0, None, None, None) == _abi.PCS_EINVAL
+    us = C.c_double()
+    assert lib.pcs_server_now_us(None, C.byref(us)) == _abi.PCS_EINVAL
+    assert lib.pcs_server_set_record_capacity_factor(None, 1.0) == _abi.PCS_EINVAL
 
 
 def test_reference_hist_matches_oracle():
