@@ -801,12 +801,23 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
 #pragma unroll
     for (int d = 0; d < PF; ++d) load_row(rb[d], min((uint32_t)d, maxrow_obs));
 
+    // This lane's bin of row r when the row has 1..kDistSparseRow records,
+    // else 0. It is read one row ahead of its use, so that a row's bins in
+    // HBM arrive while the row before is summed.
+    auto sparse_bin = [&](uint32_t r) -> uint32_t {
+        if (r > last) return 0u;
+        const uint32_t a = (r == 0u) ? 0u : rowc[r - 1u], n = rowc[r] - a;
+        return (n > 0u && n <= (uint32_t)kDistSparseRow && lane < n) ? bin_at(a + lane) : 0u;
+    };
+    uint32_t next_bin = sparse_bin(0u);
     uint64_t racc_prev = 0ull, racc_prev_hi = 0ull;
     for (uint32_t i0 = 0; i0 <= last; i0 += PF) {
 #pragma unroll
         for (int d = 0; d < PF; ++d) {
             const uint32_t i = i0 + (uint32_t)d;
             if (i > last) break;
+            const uint32_t mine = next_bin;
+            next_bin = sparse_bin(i + 1u);
             const uint32_t s0 = (i == 0u) ? 0u : rowc[i - 1u];
             const uint32_t s1 = rowc[i];
             // Past the deepest observed row R repeats its row; a row without
@@ -816,9 +827,8 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
             } else {
                 const uint32_t nr = s1 - s0;
                 if (nr > 0u && nr <= (uint32_t)kDistSparseRow) {
-                    // A few records: every wave reads them once (one per lane)
-                    // and each thread adds those at or left of its columns.
-                    const uint32_t mine = (lane < nr) ? bin_at(s0 + lane) : 0u;
+                    // A few records: every wave holds them (one per lane) and
+                    // each thread adds those at or left of its columns.
                     for (uint32_t q = 0u; q < nr; ++q) {
                         const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)q);
 #pragma unroll
