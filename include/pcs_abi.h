@@ -172,7 +172,9 @@ int pcs_simulate_static(pcs_ctx* ctx, const pcs_proposal* props, uint32_t n,
 
 /* A batch holds proposals resident in HBM; load uploads and plans it,
  * run enqueues the walk + distance asynchronously on the batch stream,
- * sync waits, read copies results back. */
+ * sync waits, read copies results back. After a load (or pcs_batch_propose)
+ * that failed, the batch holds nothing runnable: run and read return
+ * PCS_ESTATE until a load or propose succeeds. */
 int pcs_batch_create(pcs_ctx* ctx, pcs_batch** out);
 int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n,
                    const uint32_t* video_frames, const uint32_t* video_particles, uint32_t n_videos);

@@ -179,6 +179,10 @@ struct pcs_batch {
     double* d_wpop = nullptr;
     bool pop_resident = false;
     uint32_t pop_n = 0, pop_P = 0, new_P = 0;
+    // The last pcs_batch_load / pcs_batch_propose completed: the tables,
+    // plan and buffers describe n_props simulations. Cleared as soon as
+    // either starts changing them, so a failed call leaves nothing runnable.
+    bool ready = false;
     // pcs_batch_pmc_weights workspace: scaled rows, kernel sums, chunk sums
     double *d_xn = nullptr, *d_xo = nullptr, *d_wsum = nullptr, *d_part = nullptr;
     size_t cap_xn = 0, cap_xo = 0, cap_wsum = 0, cap_part = 0;
@@ -782,6 +786,7 @@ int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const ui
     PCS_HIP(hipStreamSynchronize(b->stream));
     PCS_TRY(restore_slices(b));
     destroy_graph(b);
+    b->ready = false;
     b->ran = false;
     b->dev_planned = false;
     b->host_vtile = false;
@@ -826,7 +831,10 @@ int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const ui
     b->arena_records = arena;
     plan_tiles(b, (int)c->tune.particles_per_lane);
 
+    // A capacity is zeroed before its buffers are replaced, so that an
+    // allocation failing part-way makes the next call allocate them all.
     if (n > b->cap_props) {
+        b->cap_props = 0;
         PCS_TRY(dev_alloc(&b->d_props, n));
         PCS_TRY(dev_alloc(&b->d_count, n));
         PCS_TRY(dev_alloc(&b->d_tracks, n));
@@ -836,6 +844,7 @@ int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const ui
         b->cap_props = n;
     }
     if (n_videos + 1 > b->cap_videos) {
+        b->cap_videos = 0;
         PCS_TRY(dev_alloc(&b->d_vframes, n_videos + 1));
         PCS_TRY(dev_alloc(&b->d_vparticles, n_videos + 1));
         PCS_TRY(dev_alloc(&b->d_vprop, n_videos + 1));
@@ -843,6 +852,7 @@ int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const ui
         b->cap_videos = n_videos + 1;
     }
     if (arena > b->cap_arena) {
+        b->cap_arena = 0;
         PCS_TRY(dev_alloc(&b->d_rec, arena));
         PCS_TRY(dev_alloc(&b->d_srt, arena));
         b->cap_arena = arena;
@@ -865,6 +875,7 @@ int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const ui
     }
     PCS_TRY(upload_plan(b));
     PCS_HIP(hipStreamSynchronize(b->stream));
+    b->ready = true;
     return PCS_OK;
 }
 
@@ -874,6 +885,8 @@ int pcs_batch_run(pcs_batch* b, uint64_t seed, int mode)
     if (mode != PCS_MODE_PLAIN && mode != PCS_MODE_BATCHED && mode != PCS_MODE_BATCHED_DIRECT &&
         mode != PCS_MODE_PLAIN_STREAMS)
         return set_error(PCS_EINVAL, "mode %d unknown", mode);
+    if (!b->ready)
+        return set_error(PCS_ESTATE, "batch holds no simulations: no pcs_batch_load / pcs_batch_propose, or the last one failed");
     pcs_ctx* c = b->ctx;
     PCS_HIP(hipSetDevice(c->device));
     if (b->planned_ppl != (int)c->tune.particles_per_lane) {
@@ -1044,9 +1057,12 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
     PCS_HIP(hipSetDevice(c->device));
     PCS_HIP(hipStreamSynchronize(b->stream));
     PCS_TRY(restore_slices(b, false));
+    b->ready = false;
+    b->ran = false;
     const uint32_t nv = n_trials * J;
     bool relayout = !b->dev_planned || n_trials != b->n_props || J != b->J;
     if (n_trials > b->cap_props) {
+        b->cap_props = 0;
         PCS_TRY(dev_alloc(&b->d_props, n_trials));
         PCS_TRY(dev_alloc(&b->d_count, n_trials));
         PCS_TRY(dev_alloc(&b->d_tracks, n_trials));
@@ -1057,11 +1073,13 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
         relayout = true;
     }
     if (n_trials > b->cap_in || !b->d_in) {
+        b->cap_in = b->cap_new = 0;
         PCS_TRY(dev_alloc(&b->d_in, n_trials));
         PCS_TRY(dev_alloc(&b->d_theta_new, (size_t)3u * PCS_MAX_SPECIES * n_trials));
         b->cap_in = b->cap_new = n_trials;
     }
     if ((size_t)nv + 1u > b->cap_videos) {
+        b->cap_videos = 0;
         PCS_TRY(dev_alloc(&b->d_vframes, (size_t)nv + 1u));
         PCS_TRY(dev_alloc(&b->d_vparticles, (size_t)nv + 1u));
         PCS_TRY(dev_alloc(&b->d_vprop, (size_t)nv + 1u));
@@ -1069,13 +1087,14 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
         b->cap_videos = (size_t)nv + 1u;
         relayout = true;
     }
+    if (!resident) b->pop_resident = false;  // replaced by an uploaded population without weights
     if (!resident && (n_pop > b->cap_pop || !b->d_theta_pop)) {
+        b->cap_pop = 0;
         PCS_TRY(dev_alloc(&b->d_theta_pop, (size_t)3u * PCS_MAX_SPECIES * n_pop));
         PCS_TRY(dev_alloc(&b->d_cumw, n_pop));
         PCS_TRY(dev_alloc(&b->d_wpop, n_pop));
         b->cap_pop = n_pop;
     }
-    if (!resident) b->pop_resident = false;  // replaced by an uploaded population without weights
     b->new_P = P;
     if (!b->d_totals) PCS_TRY(dev_alloc(&b->d_totals, 2));
     b->dev_planned = true;
@@ -1129,6 +1148,7 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
     PCS_HIP(hipStreamSynchronize(st));
     if (tot[0] > b->cap_arena || !b->d_rec) {
         const uint64_t cap = tot[0] + tot[0] / 8u + 1024u;
+        b->cap_arena = 0;
         PCS_TRY(dev_alloc(&b->d_rec, cap));
         PCS_TRY(dev_alloc(&b->d_srt, cap));
         b->cap_arena = cap;
@@ -1144,6 +1164,7 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
     }
     PCS_HIP(hipStreamSynchronize(st));
     if (relayout) destroy_graph(b);
+    b->ready = true;
     return PCS_OK;
 }
 
@@ -1158,7 +1179,9 @@ int pcs_batch_set_population(pcs_batch* b, const double* theta, const double* w,
         if (!(w[i] >= 0.0) || !std::isfinite(w[i])) return set_error(PCS_EINVAL, "w[%u] negative or not finite", i);
     PCS_HIP(hipSetDevice(b->ctx->device));
     PCS_HIP(hipStreamSynchronize(b->stream));
+    b->pop_resident = false;  // until the new population is in place
     if (n_pop > b->cap_pop || !b->d_theta_pop || !b->d_wpop) {
+        b->cap_pop = 0;
         PCS_TRY(dev_alloc(&b->d_theta_pop, (size_t)3u * PCS_MAX_SPECIES * n_pop));
         PCS_TRY(dev_alloc(&b->d_cumw, n_pop));
         PCS_TRY(dev_alloc(&b->d_wpop, n_pop));
@@ -1179,7 +1202,7 @@ int pcs_batch_pmc_weights(pcs_batch* b, const double* tau, double* w_out)
 {
     if (!b || !tau || !w_out) return set_error(PCS_EINVAL, "NULL argument");
     if (!b->pop_resident) return set_error(PCS_ESTATE, "no resident population (pcs_batch_set_population)");
-    if (!b->dev_planned || b->new_P != b->pop_P)
+    if (!b->ready || !b->dev_planned || b->new_P != b->pop_P)
         return set_error(PCS_ESTATE, "no trials proposed from the resident population (pcs_batch_propose)");
     const uint32_t P = b->pop_P, n_new = b->n_props, n_old = b->pop_n;
     ScaleRows sr{};
@@ -1222,6 +1245,7 @@ int pcs_batch_pmc_weights(pcs_batch* b, const double* tau, double* w_out)
 int pcs_batch_read_proposals(pcs_batch* b, pcs_proposal* props_out, uint32_t* particles_out)
 {
     if (!b) return set_error(PCS_EINVAL, "batch is NULL");
+    if (!b->ready) return set_error(PCS_ESTATE, "batch holds no simulations (the last load or propose failed)");
     PCS_HIP(hipSetDevice(b->ctx->device));
     PCS_HIP(hipStreamSynchronize(b->stream));
     if (b->dev_planned) {

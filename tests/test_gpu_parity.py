@@ -305,6 +305,40 @@ def test_invalid_arguments_fail_loudly(gpu_lib):
             b.load([p], f, n)
 
 
+def test_failed_load_leaves_nothing_to_run(gpu_lib):
+    """A batch whose last load failed part-way (here its second proposal is
+    invalid, after the batch held a smaller valid load and ran it) refuses to
+    run or read (PCS_ESTATE, on the host: no kernel is launched), and a
+    following valid load runs and reads as a fresh batch would."""
+    model = cases.cpu_model()
+    obs = observed(model)
+    good, f1, n1 = cases.fixed_proposal(model, [300], [200], sim_id=1)
+    with Context(model, obs) as ctx:
+        b = ctx.batch()
+        b.load([good], f1, n1)
+        b.run(5, _abi.PCS_MODE_BATCHED)
+        d_first, t_first, _ = b.read()
+        bad = _abi.pcs_proposal.from_buffer_copy(good)
+        bad.first_video = 1
+        bad.cum_prob[1] = 0.5
+        frames = np.array([200, 200, 200], np.uint32)
+        parts = np.array([300, 300, 300], np.uint32)
+        with pytest.raises(_abi.PcsError):
+            b.load([good, bad, good], frames, parts)
+        for mode in (_abi.PCS_MODE_BATCHED, _abi.PCS_MODE_PLAIN):
+            with pytest.raises(_abi.PcsError) as e:
+                b.run(5, mode)
+            assert e.value.code == _abi.PCS_ESTATE
+        with pytest.raises(_abi.PcsError) as e:
+            b.read()
+        assert e.value.code == _abi.PCS_ESTATE
+        b.load([good], f1, n1)
+        b.run(5, _abi.PCS_MODE_BATCHED)
+        d_again, t_again, _ = b.read()
+        b.close()
+    assert np.array_equal(t_again, t_first) and np.array_equal(d_again.view(np.uint64), d_first.view(np.uint64))
+
+
 def test_batch_outlives_its_context(gpu_lib):
     # pcs_close releases the caller's handle only: a live batch keeps the
     # context, runs after the close, and destroying it frees both.
