@@ -826,8 +826,14 @@ int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const ui
             d.sigma[s] = p.sigma[s];
         }
     }
-    for (uint32_t v = 0; v < n_videos; ++v)
+    uint64_t tiles = 0;
+    for (uint32_t v = 0; v < n_videos; ++v) {
         if (b->vprop[v] == 0xffffffffu) return set_error(PCS_EINVAL, "video %u belongs to no proposal", v);
+        if (vparticles[v] > (1u << 31)) return set_error(PCS_EINVAL, "video %u: more than 2^31 particles", v);
+        tiles += (vparticles[v] + 63u) / 64u;
+    }
+    if (tiles > 0xffffffffull) return set_error(PCS_EINVAL, "the batch needs %llu tiles, more than 2^32 - 1",
+                                                (unsigned long long)tiles);
     b->arena_records = arena;
     plan_tiles(b, (int)c->tune.particles_per_lane);
 
@@ -1146,6 +1152,9 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
     uint64_t tot[2] = {0, 0};
     PCS_HIP(hipMemcpyAsync(tot, b->d_totals, sizeof(tot), hipMemcpyDeviceToHost, st));
     PCS_HIP(hipStreamSynchronize(st));
+    if (tot[1] > 0xffffffffull)  // tile indices are 32-bit
+        return set_error(PCS_EINVAL, "the proposed trials need %llu tiles, more than 2^32 - 1",
+                         (unsigned long long)tot[1]);
     if (tot[0] > b->cap_arena || !b->d_rec) {
         const uint64_t cap = tot[0] + tot[0] / 8u + 1024u;
         b->cap_arena = 0;

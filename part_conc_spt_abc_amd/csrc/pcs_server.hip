@@ -810,6 +810,8 @@ int pcs_server_submit(pcs_server* s, const pcs_request* req)
     if (req->n_videos == 0 || req->n_videos > s->cfg.max_videos)
         return set_error(PCS_EINVAL, "n_videos %u outside [1, %u]", req->n_videos, s->cfg.max_videos);
     if (!req->frames || !req->particles) return set_error(PCS_EINVAL, "NULL video arrays");
+    for (uint32_t v = 0; v < req->n_videos; ++v)
+        if (req->particles[v] > (1u << 31)) return set_error(PCS_EINVAL, "video %u: more than 2^31 particles", v);
     Pending p;
     p.req = *req;
     p.frames.assign(req->frames, req->frames + req->n_videos);

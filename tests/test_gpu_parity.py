@@ -303,6 +303,20 @@ def test_invalid_arguments_fail_loudly(gpu_lib):
         with pytest.raises(_abi.PcsError):
             b = ctx.batch()
             b.load([p], f, n)
+        # more than 2^31 particles in a video (tile indices are 32-bit)
+        q, f, _ = cases.fixed_proposal(model, [10], [10])
+        with pytest.raises(_abi.PcsError) as e:
+            ctx.batch().load([q], f, np.array([(1 << 31) + 1], np.uint32))
+        assert e.value.code == _abi.PCS_EINVAL
+    from part_conc_spt_abc_amd.gpu import Server
+
+    srv = Server(device=0, arena_bytes=1 << 24, max_inflight=8)
+    sid = srv.add_system(model)
+    q, f, _ = cases.fixed_proposal(model, [10], [10])
+    with pytest.raises(_abi.PcsError) as e:
+        srv.submit(0, sid, q, f, np.array([(1 << 31) + 1], np.uint32), 1)
+    assert e.value.code == _abi.PCS_EINVAL
+    srv.close()
 
 
 def test_failed_load_leaves_nothing_to_run(gpu_lib):
