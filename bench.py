@@ -403,7 +403,7 @@ def generation_population(model, n, rng, frames_choices=(500, 1000, 2000)):
     return theta, w, tau, frames, prior
 
 
-def run_generations(batch, pop, steps, warmup, seed0=1000):
+def run_generations(batch, pop, steps, warmup, seed0=1000, overlap=True):
     """Timed ABC-PMC generations on the GPU, from a population resident in
     HBM before timing (pcs_batch_set_population). One step: the proposal
     step of all the trials on the GPU (parents, displacements, Poisson
@@ -412,8 +412,10 @@ def run_generations(batch, pop, steps, warmup, seed0=1000):
     distances, and the trials' pmc-standard importance weights against the
     population on the GPU (n x n kernel sums over the parameters with
     spread), read back. The population is the same at every step, so every
-    step does the same work. Returns the wall times, per-step (propose,
-    run + read, weights) ms and particle-steps."""
+    step does the same work. With overlap the weights are enqueued right
+    after the walk (pcs_batch_pmc_weights_async) and run beside its tail and
+    the distance; the step still ends only when they are back. Returns the
+    wall times, per-step (propose, run + read, weights) ms and particle-steps."""
     theta, w, tau, frames, prior = pop
     n = theta.shape[1]
     batch.set_population(theta, w)
@@ -423,9 +425,14 @@ def run_generations(batch, pop, steps, warmup, seed0=1000):
         batch.propose(prior, None, None, tau, n, frames, seed0 + i)
         t1 = time.perf_counter()
         batch.run(seed0 + i, _abi.PCS_MODE_BATCHED)
+        if overlap:
+            batch.pmc_weights_async(tau)  # beside the walk: it needs the trials, not the distances
         batch.read()
         t2 = time.perf_counter()
-        batch.pmc_weights(tau)
+        if overlap:
+            batch.pmc_weights_wait()
+        else:
+            batch.pmc_weights(tau)
         t3 = time.perf_counter()
         if i >= warmup:
             walls.append(t3 - t0)
@@ -454,6 +461,8 @@ def main():
                     help="z walls of the 200x200x30 um box (x and y are periodic); the batched leg also runs "
                          "the other z boundary on the same proposals unless --no-other-z")
     ap.add_argument("--no-other-z", action="store_true")
+    ap.add_argument("--no-weights-overlap", action="store_true",
+                    help="compute the weights after the read-back instead of beside the walk")
     ap.add_argument("--tuning", default="",
                     help="launch shape 'particles_per_lane,waves_per_group,groups_per_cu[,stage_records]' instead "
                          "of the autotuner (profiling runs: no autotuner launches among the profiled ones)")
@@ -495,7 +504,7 @@ def main():
     # everything on the GPU.
     pop = generation_population(model, args.batched_sims, np.random.default_rng(11))
     gb = ctx.batch()
-    walls, gparts, gsteps = run_generations(gb, pop, args.steps, args.warmup)
+    walls, gparts, gsteps = run_generations(gb, pop, args.steps, args.warmup, overlap=not args.no_weights_overlap)
     reruns = gb.overflow_reruns()
     wall = sum(walls)
     sims_s = args.batched_sims * args.steps / wall
@@ -513,8 +522,8 @@ def main():
     walk_s = float(np.mean(walk_ms)) / 1e3 if min(walk_ms) > 0 else float("nan")
     log(f"generation: {sims_s:.0f} sims/s, {psteps_s:.3e} particle-steps/s, {np.mean(walls)*1e3:.1f} ms per "
         f"generation (propose + plan {part_ms[0]:.1f} ms, walk + distance + read {part_ms[1]:.1f} ms, pmc weights "
-        f"{part_ms[2]:.1f} ms), {steps_per_gen:.3e} particle-steps per generation, walk kernel {walk_s*1e3:.1f} ms, "
-        f"reruns {reruns}")
+        f"{'left after the read ' if not args.no_weights_overlap else ''}{part_ms[2]:.1f} ms), "
+        f"{steps_per_gen:.3e} particle-steps per generation, walk kernel {walk_s*1e3:.1f} ms, reruns {reruns}")
 
     # Beside it: the same kind of generation with its proposals packed on the
     # host and resident in HBM before timing (walk + distance + read only).
@@ -635,7 +644,9 @@ def main():
                         f"120-12,000 particles, tau = sqrt(2 var)), frames U{{500,1000,2000}}, box 200x200x30 um "
                         f"{'periodic' if args.z_boundary == 'periodic' else 'periodic in x/y, reflecting in z'}, "
                         f"H 1000x500 uint32; step = proposal step + plan, walk + distance in one hipGraph launch, "
-                        f"distances read back, pmc-standard weights of the trials against the population",
+                        f"distances read back, pmc-standard weights of the trials against the population"
+                        f"{' (enqueued beside the walk)' if not args.no_weights_overlap else ''}",
+            "weights_beside_walk": not args.no_weights_overlap,
             "step_propose_ms": float(part_ms[0]),
             "step_walk_distance_read_ms": float(part_ms[1]),
             "step_weights_ms": float(part_ms[2]),

@@ -247,6 +247,15 @@ int pcs_batch_set_population(pcs_batch* b, const double* theta, const double* w,
  * no spread and give every pair the same factor); normalised to sum 1
  * (n_trials doubles). */
 int pcs_batch_pmc_weights(pcs_batch* b, const double* tau, double* w_out);
+/* The same in two halves, so that the weights overlap the walk: _async
+ * enqueues them on the batch's own weights stream, ordered after the last
+ * pcs_batch_propose only (they need the proposed parameters, not the
+ * distances: called after pcs_batch_run they run beside the walk), and
+ * returns; _wait blocks until they are done and writes w_out as
+ * pcs_batch_pmc_weights does. A load, propose or set_population in between
+ * waits for them and drops them (_wait then returns PCS_ESTATE). */
+int pcs_batch_pmc_weights_async(pcs_batch* b, const double* tau);
+int pcs_batch_pmc_weights_wait(pcs_batch* b, double* w_out);
 /* The loaded or proposed batch's proposal records (n) and video particle
  * counts (n_videos); each nullable. */
 int pcs_batch_read_proposals(pcs_batch* b, pcs_proposal* props_out, uint32_t* video_particles_out);

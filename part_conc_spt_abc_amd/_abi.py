@@ -159,6 +159,8 @@ PROTOTYPES = {
     "pcs_batch_read_proposals": (C.c_int, [P, P, P]),
     "pcs_batch_set_population": (C.c_int, [P, P, P, C.c_uint32, C.c_uint32]),
     "pcs_batch_pmc_weights": (C.c_int, [P, P, P]),
+    "pcs_batch_pmc_weights_async": (C.c_int, [P, P]),
+    "pcs_batch_pmc_weights_wait": (C.c_int, [P, P]),
     "pcs_set_tuning": (C.c_int, [P, C.POINTER(pcs_tuning)]),
     "pcs_get_tuning": (C.c_int, [P, C.POINTER(pcs_tuning)]),
     "pcs_set_record_capacity_factor": (C.c_int, [P, C.c_double]),

@@ -195,6 +195,12 @@ struct pcs_batch {
     hipStream_t pstream[PCS_PLAIN_STREAMS] = {};
     hipEvent_t pfork = nullptr;
     hipEvent_t pjoin[PCS_PLAIN_STREAMS] = {};
+    // pmc weights enqueued by pcs_batch_pmc_weights_async on their own stream
+    // (after the proposals on the batch stream), collected by ..._wait
+    hipStream_t wstream = nullptr;
+    hipEvent_t wfork = nullptr;
+    bool w_pending = false, w_uniform = false;
+    uint32_t w_n = 0;
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int last_mode = -1;
@@ -724,6 +730,16 @@ int pcs_get_tuning(pcs_ctx* c, pcs_tuning* t)
     return PCS_OK;
 }
 
+// Waits for weights enqueued by pcs_batch_pmc_weights_async (they read the
+// trials' parameters and the population) and forgets them: a call that
+// rewrites those buffers comes first.
+static int drop_weights(pcs_batch* b)
+{
+    if (b->wstream) PCS_HIP(hipStreamSynchronize(b->wstream));
+    b->w_pending = false;
+    return PCS_OK;
+}
+
 int pcs_batch_create(pcs_ctx* c, pcs_batch** out)
 {
     if (!c || !out) return set_error(PCS_EINVAL, "NULL argument");
@@ -754,6 +770,7 @@ int pcs_batch_destroy(pcs_batch* b)
     if (b->stream) hipStreamSynchronize(b->stream);
     for (hipStream_t ps : b->pstream)
         if (ps) hipStreamSynchronize(ps);
+    if (b->wstream) hipStreamSynchronize(b->wstream);
     destroy_graph(b);
     void* ptrs[] = {b->d_props, b->d_vframes, b->d_vparticles, b->d_vprop, b->d_vtile, b->d_rec, b->d_srt,
                     b->d_rec2, b->d_srt2, b->d_count, b->d_tracks, b->d_tile_counter, b->d_dist, b->d_exact, b->d_status,
@@ -769,6 +786,8 @@ int pcs_batch_destroy(pcs_batch* b)
         if (b->pjoin[k]) hipEventDestroy(b->pjoin[k]);
     }
     if (b->pfork) hipEventDestroy(b->pfork);
+    if (b->wfork) hipEventDestroy(b->wfork);
+    if (b->wstream) hipStreamDestroy(b->wstream);
     if (b->stream) hipStreamDestroy(b->stream);
     pcs_ctx* c = b->ctx;
     delete b;
@@ -784,6 +803,7 @@ int pcs_batch_load(pcs_batch* b, const pcs_proposal* props, uint32_t n, const ui
     pcs_ctx* c = b->ctx;
     PCS_HIP(hipSetDevice(c->device));
     PCS_HIP(hipStreamSynchronize(b->stream));
+    PCS_TRY(drop_weights(b));
     PCS_TRY(restore_slices(b));
     destroy_graph(b);
     b->ready = false;
@@ -1062,6 +1082,7 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
                 return set_error(PCS_EINVAL, "cum_w must be finite and non-decreasing (member %u)", i);
     PCS_HIP(hipSetDevice(c->device));
     PCS_HIP(hipStreamSynchronize(b->stream));
+    PCS_TRY(drop_weights(b));
     PCS_TRY(restore_slices(b, false));
     b->ready = false;
     b->ran = false;
@@ -1173,6 +1194,12 @@ int pcs_batch_propose(pcs_batch* b, const pcs_prior* pr, const double* theta, co
     }
     PCS_HIP(hipStreamSynchronize(st));
     if (relayout) destroy_graph(b);
+    // the point pcs_batch_pmc_weights_async starts from: the trials exist
+    if (!b->wstream) {
+        PCS_HIP(hipStreamCreateWithFlags(&b->wstream, hipStreamNonBlocking));
+        PCS_HIP(hipEventCreateWithFlags(&b->wfork, hipEventDisableTiming));
+    }
+    PCS_HIP(hipEventRecord(b->wfork, st));
     b->ready = true;
     return PCS_OK;
 }
@@ -1188,6 +1215,7 @@ int pcs_batch_set_population(pcs_batch* b, const double* theta, const double* w,
         if (!(w[i] >= 0.0) || !std::isfinite(w[i])) return set_error(PCS_EINVAL, "w[%u] negative or not finite", i);
     PCS_HIP(hipSetDevice(b->ctx->device));
     PCS_HIP(hipStreamSynchronize(b->stream));
+    PCS_TRY(drop_weights(b));
     b->pop_resident = false;  // until the new population is in place
     if (n_pop > b->cap_pop || !b->d_theta_pop || !b->d_wpop) {
         b->cap_pop = 0;
@@ -1207,9 +1235,9 @@ int pcs_batch_set_population(pcs_batch* b, const double* theta, const double* w,
     return PCS_OK;
 }
 
-int pcs_batch_pmc_weights(pcs_batch* b, const double* tau, double* w_out)
+int pcs_batch_pmc_weights_async(pcs_batch* b, const double* tau)
 {
-    if (!b || !tau || !w_out) return set_error(PCS_EINVAL, "NULL argument");
+    if (!b || !tau) return set_error(PCS_EINVAL, "NULL argument");
     if (!b->pop_resident) return set_error(PCS_ESTATE, "no resident population (pcs_batch_set_population)");
     if (!b->ready || !b->dev_planned || b->new_P != b->pop_P)
         return set_error(PCS_ESTATE, "no trials proposed from the resident population (pcs_batch_propose)");
@@ -1223,14 +1251,17 @@ int pcs_batch_pmc_weights(pcs_batch* b, const double* tau, double* w_out)
             ++sr.n_rows;
         }
     }
-    if (n_new == 0) return PCS_OK;
-    if (sr.n_rows == 0) {  // every member identical: equal weights
-        for (uint32_t j = 0; j < n_new; ++j) w_out[j] = 1.0 / (double)n_new;
+    PCS_HIP(hipSetDevice(b->ctx->device));
+    PCS_TRY(drop_weights(b));
+    b->w_n = n_new;
+    b->w_uniform = sr.n_rows == 0;  // every member identical: equal weights
+    if (n_new == 0 || b->w_uniform) {
+        b->w_pending = true;
         return PCS_OK;
     }
-    PCS_HIP(hipSetDevice(b->ctx->device));
     auto grow = [](double** ptr, size_t& cap, size_t n) -> int {
         if (n <= cap && *ptr) return PCS_OK;
+        cap = 0;
         const int r = dev_alloc(ptr, n);
         cap = r == PCS_OK ? n : 0;
         return r;
@@ -1239,16 +1270,43 @@ int pcs_batch_pmc_weights(pcs_batch* b, const double* tau, double* w_out)
     PCS_TRY(grow(&b->d_xo, b->cap_xo, (size_t)sr.n_rows * n_old));
     PCS_TRY(grow(&b->d_wsum, b->cap_wsum, n_new));
     PCS_TRY(grow(&b->d_part, b->cap_part, (size_t)pmc_chunks(n_old) * n_new));
-    hipStream_t st = b->stream;
+    // after the proposal step (recorded by pcs_batch_propose), not after
+    // whatever the batch stream has taken since: a walk enqueued before this
+    // call runs beside the weights
+    PCS_HIP(hipStreamWaitEvent(b->wstream, b->wfork, 0));
+    hipStream_t st = b->wstream;
     PCS_HIP(launch_scale_rows(b->d_theta_new, n_new, sr, b->d_xn, st));
     PCS_HIP(launch_scale_rows(b->d_theta_pop, n_old, sr, b->d_xo, st));
     PCS_HIP(launch_pmc_sums(b->d_xn, n_new, b->d_xo, b->d_wpop, n_old, sr.n_rows, b->d_part, b->d_wsum, st));
-    PCS_HIP(hipMemcpyAsync(w_out, b->d_wsum, (size_t)n_new * 8u, hipMemcpyDeviceToHost, st));
-    PCS_HIP(hipStreamSynchronize(st));
+    b->w_pending = true;
+    return PCS_OK;
+}
+
+int pcs_batch_pmc_weights_wait(pcs_batch* b, double* w_out)
+{
+    if (!b || !w_out) return set_error(PCS_EINVAL, "NULL argument");
+    if (!b->w_pending) return set_error(PCS_ESTATE, "no weights enqueued (pcs_batch_pmc_weights_async)");
+    PCS_HIP(hipSetDevice(b->ctx->device));
+    b->w_pending = false;
+    const uint32_t n_new = b->w_n;
+    if (n_new == 0) return PCS_OK;
+    if (b->w_uniform) {
+        for (uint32_t j = 0; j < n_new; ++j) w_out[j] = 1.0 / (double)n_new;
+        return PCS_OK;
+    }
+    PCS_HIP(hipMemcpyAsync(w_out, b->d_wsum, (size_t)n_new * 8u, hipMemcpyDeviceToHost, b->wstream));
+    PCS_HIP(hipStreamSynchronize(b->wstream));
     double sum = 0.0;
     for (uint32_t j = 0; j < n_new; ++j) sum += (w_out[j] = 1.0 / w_out[j]);
     for (uint32_t j = 0; j < n_new; ++j) w_out[j] /= sum;
     return PCS_OK;
+}
+
+int pcs_batch_pmc_weights(pcs_batch* b, const double* tau, double* w_out)
+{
+    if (!b || !tau || !w_out) return set_error(PCS_EINVAL, "NULL argument");
+    PCS_TRY(pcs_batch_pmc_weights_async(b, tau));
+    return pcs_batch_pmc_weights_wait(b, w_out);
 }
 
 int pcs_batch_read_proposals(pcs_batch* b, pcs_proposal* props_out, uint32_t* particles_out)

@@ -149,6 +149,17 @@ class Batch:
         _abi.check(self.lib.pcs_batch_pmc_weights(self.h, _ptr(ta), _ptr(out)), "pcs_batch_pmc_weights")
         return out
 
+    def pmc_weights_async(self, tau: np.ndarray) -> None:
+        """Enqueue the weights on the batch's weights stream
+        (pcs_batch_pmc_weights_async); collect them with pmc_weights_wait."""
+        ta = np.ascontiguousarray(tau, dtype=np.float64)
+        _abi.check(self.lib.pcs_batch_pmc_weights_async(self.h, _ptr(ta)), "pcs_batch_pmc_weights_async")
+
+    def pmc_weights_wait(self) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        _abi.check(self.lib.pcs_batch_pmc_weights_wait(self.h, _ptr(out)), "pcs_batch_pmc_weights_wait")
+        return out
+
     def read_proposals(self):
         """(pcs_proposal records, video particle counts) of the loaded or
         proposed batch."""

@@ -188,3 +188,40 @@ def test_resident_population_equals_host_arrays(gpu_lib):
     w_ref = pmc_weights(th_res[keep], theta[keep], w, tau[keep])
     assert np.array_equal(w_res.view(np.uint64), w_ref.view(np.uint64))
     assert abs(w_res.sum() - 1.0) < 1e-12
+
+
+def test_weights_beside_the_walk_equal_the_synchronous_ones(gpu_lib):
+    """pcs_batch_pmc_weights_async enqueued after the walk of the same
+    trials (so the two run side by side) and collected after the read-back
+    gives the synchronous weights bit for bit, and leaves the walk's results
+    unchanged; _wait without a pending computation, or after a new proposal
+    step dropped it, returns PCS_ESTATE."""
+    model = cases.headline_model(n_k=300, n_de=100, n_species=2)
+    rng = np.random.default_rng(23)
+    theta, _, tau = population(rng, 2, 1, 5000)
+    w = rng.dirichlet(np.ones(theta.shape[1]))
+    n = 6000
+    frames = np.full((n, 1), 300, np.uint32)
+    pr = prior_record(model, M_B, C_B, AZ_B)
+    with Context(model) as ctx:
+        b = ctx.batch()
+        b.set_population(theta, w)
+        b.propose(pr, None, None, tau, n, frames, 7)
+        b.run(3, _abi.PCS_MODE_BATCHED)
+        d_sync, t_sync, _ = b.read()
+        w_sync = b.pmc_weights(tau)
+        b.run(3, _abi.PCS_MODE_BATCHED)
+        b.pmc_weights_async(tau)
+        d_async, t_async, _ = b.read()
+        w_async = b.pmc_weights_wait()
+        with pytest.raises(_abi.PcsError) as e:
+            b.pmc_weights_wait()
+        assert e.value.code == _abi.PCS_ESTATE
+        b.pmc_weights_async(tau)
+        b.propose(pr, None, None, tau, n, frames, 8)
+        with pytest.raises(_abi.PcsError) as e:
+            b.pmc_weights_wait()
+        assert e.value.code == _abi.PCS_ESTATE
+        b.close()
+    assert np.array_equal(w_async.view(np.uint64), w_sync.view(np.uint64))
+    assert np.array_equal(t_async, t_sync) and np.array_equal(d_async.view(np.uint64), d_sync.view(np.uint64))
