@@ -161,15 +161,17 @@ def precise_timers():
             libc.prctl(29, ctypes.c_ulong(0), 0, 0, 0)
 
 
-def serving_latency(n_requests, rate, seed=7, device_loop=False):
+def serving_latency(n_requests, rate, seed=7, device_loop=False, one_wave_tiles=False):
     """Poisson arrivals at `rate` req/s from 32 chains, each with its own box
     (lateral 100-400 um, depth 10-50 um) and 2-4 species (100-600 nm);
-    host-driven launch groups or the device-resident loop. A request's
+    host-driven launch groups or the device-resident loop (one_wave_tiles:
+    small requests' tiles walked by one wave, not a group, each). A request's
     latency runs from its scheduled arrival (t0 + the sum of the gaps before
     it, on the server's clock) to the moment its result is seen, so time it
     waits behind a busy submit loop or a full queue counts."""
     rng = np.random.default_rng(seed)
-    srv = Server(device=0, arena_bytes=4 << 30, max_inflight=4096, device_loop=device_loop)
+    srv = Server(device=0, arena_bytes=4 << 30, max_inflight=4096, device_loop=device_loop,
+                 one_wave_tiles=one_wave_tiles)
     chains = []
     for c in range(32):
         L = float(rng.uniform(100, 400))
@@ -196,6 +198,7 @@ def serving_latency(n_requests, rate, seed=7, device_loop=False):
     warm = 0
     while warm < 64:
         warm += len(srv.poll(64, wait_us=2000))
+    walked0 = srv.group_walked()
     gaps = rng.exponential(1.0 / rate, n_requests)
     # scheduled arrivals in perf_counter seconds, and the offset of the
     # server's clock (both CLOCK_MONOTONIC): server_us = perf_s * 1e6 + off
@@ -220,8 +223,10 @@ def serving_latency(n_requests, rate, seed=7, device_loop=False):
                 service[c.tag] = c.done_us - c.submit_us
                 done += 1
     wall = time.perf_counter() - t0
+    walked = srv.group_walked() - walked0
     srv.close()
     return {"rate_req_s": rate, "requests": n_requests, "throughput_req_s": n_requests / wall,
+            "group_walked_requests": walked,
             "generator_timer_slack_1ns": bool(precise),
             "p50_ms": float(np.percentile(lat, 50) / 1e3), "p99_ms": float(np.percentile(lat, 99) / 1e3),
             "p50_from_submit_ms": float(np.percentile(service, 50) / 1e3),
@@ -667,7 +672,8 @@ def main():
             # default), from each request's scheduled arrival
             "p50_ms": pick(serve_dev, "p50_ms"),
             "p99_ms": pick(serve_dev, "p99_ms"),
-            "latency_form": "device-resident loop, from scheduled arrival" if serve_dev else None,
+            "latency_form": ("device-resident loop (requests of <= 1 tile per CU admitted behind <= 12 "
+                             "unfinished ones walked a group per tile), from scheduled arrival") if serve_dev else None,
             "host_driven_p50_ms": pick(serve, "p50_ms"),
             "host_driven_p99_ms": pick(serve, "p99_ms"),
             "large_polydisperse_sims_per_s": large_dev["sims_per_s"] if large_dev else None,

@@ -339,14 +339,24 @@ int pcs_pmc_weights(int device, const double* theta_new, uint32_t n_new, const d
  *                          only admits (record arena, ring slot) and collects.
  *                          The kernel leaves after loop_idle_us without work
  *                          and is launched again when requests arrive.
- * Both give the same result for the same request. */
+ *                          A request of at most one tile (64 particles) per
+ *                          CU, admitted while at most 12 earlier requests are
+ *                          unfinished, is claimed a tile at a time and each
+ *                          tile walked by a whole group of 4 waves, three
+ *                          producing its random normals, one stepping (a lone
+ *                          wave's instruction stream is its latency); other
+ *                          requests have a wave walk each tile.
+ *   PCS_SERVE_LOOP_ONE_WAVE_TILES  with PCS_SERVE_DEVICE_LOOP: every tile is
+ *                          walked by one wave, as in a large request.
+ * All give the same result for the same request. */
 #define PCS_SERVE_DEVICE_LOOP 1u
+#define PCS_SERVE_LOOP_ONE_WAVE_TILES 2u
 
 typedef struct pcs_server_config {
     uint64_t arena_bytes;    /* HBM budget for track records (admission), 0 = 1 GiB */
     uint32_t max_inflight;   /* simulations on the GPU at once, 0 = 4096 */
     uint32_t max_videos;     /* videos per request, 0 = 64 */
-    uint32_t flags;          /* 0 or PCS_SERVE_DEVICE_LOOP */
+    uint32_t flags;          /* 0, or PCS_SERVE_DEVICE_LOOP (| PCS_SERVE_LOOP_ONE_WAVE_TILES) */
     uint32_t loop_idle_us;   /* device loop: idle time before it leaves, 0 = 20000 */
 } pcs_server_config;
 
@@ -385,6 +395,11 @@ int pcs_server_pending(pcs_server* s, uint32_t* n);
 int pcs_server_set_record_capacity_factor(pcs_server* s, double factor);
 /* Requests rerun so far after a record slice overflow (informational). */
 int pcs_server_overflow_reruns(pcs_server* s, uint64_t* n);
+/* Device loop: requests completed so far whose tiles were each walked by a
+ * whole group (informational; 0 in the host-driven form). No counterpart in
+ * the reference (its runs have no request loop): for a serving caller's
+ * monitoring. */
+int pcs_server_group_walked(pcs_server* s, uint64_t* n);
 /* The server's clock (us since server start; CLOCK_MONOTONIC), the clock of
  * submit_us and done_us: lets a load generator stamp its own scheduled
  * arrival times in the same frame. */

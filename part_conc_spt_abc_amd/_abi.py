@@ -34,6 +34,7 @@ PCS_MODE_BATCHED_DIRECT = 2
 PCS_MODE_PLAIN_STREAMS = 3
 PCS_PLAIN_STREAMS = 4
 PCS_SERVE_DEVICE_LOOP = 1
+PCS_SERVE_LOOP_ONE_WAVE_TILES = 2
 PCS_STATIC_SCRATCH_BYTES = 128 << 20
 PCS_STATIC_MAX_SIMS = 8192
 PCS_STATIC_MAX_VIDEOS = 65536
@@ -179,6 +180,7 @@ PROTOTYPES = {
     "pcs_server_now_us": (C.c_int, [P, F64P]),
     "pcs_server_set_record_capacity_factor": (C.c_int, [P, C.c_double]),
     "pcs_server_overflow_reruns": (C.c_int, [P, U64P]),
+    "pcs_server_group_walked": (C.c_int, [P, U64P]),
     "pcs_server_destroy": (C.c_int, [P]),
 }
 

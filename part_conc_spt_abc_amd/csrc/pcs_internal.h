@@ -145,16 +145,19 @@ struct LoopDone {
     uint64_t exact;
     uint32_t tracks;
     int32_t status;
+    uint32_t group_walked;  // the request's tiles were claimed a tile per group
+    uint32_t pad;
 };
-static_assert(sizeof(LoopDone) == 32, "LoopDone layout");
+static_assert(sizeof(LoopDone) == 40, "LoopDone layout");
 
 struct LoopSlot {
     uint64_t state;
-    uint32_t tiles, tiles_done, rec_count, n_videos;
+    uint32_t tiles, coop;  // one 8-byte word: a worker reads both at once; coop: a tile per group
+    uint32_t tiles_done, rec_count, n_videos, pad;
     uint64_t claim;  // (r mod 2^32) << 32 | next unclaimed tile
     DevProp prop;
 };
-static_assert(sizeof(LoopSlot) == 176, "LoopSlot layout");
+static_assert(sizeof(LoopSlot) == 184, "LoopSlot layout");
 
 // CU index of a group: XCC id << 8 | SE/SH/CU id of HW_ID, below 2048.
 constexpr uint32_t kLoopCuSlots = 2048;
@@ -163,6 +166,7 @@ struct LoopCtl {
     uint64_t cursor;      // oldest request that may have unclaimed tiles
     uint64_t last_claim;  // wall clock of the latest tile claim, any group
     uint64_t admit_next;  // next request the admitter copies in (kept across launches)
+    uint64_t finished;    // requests whose result is written (kept across launches)
     uint32_t quit;        // set when the loop should end: every group leaves
     uint32_t error;       // non-zero: a claim could not be resolved (the loop ended)
 };
@@ -185,6 +189,9 @@ struct LoopArgs {
     uint32_t stage_cap;     // LDS records per wave
     uint32_t nk_lds, nde_lds;
     uint32_t srtl_cap;      // sorted 16-bit bins of a distance that fit in LDS (after the column histogram)
+    uint32_t coop_max_tiles;  // requests of at most this many tiles are claimed a tile at a time and each
+                              // tile walked by a whole group (0: never) when admitted while at most
+    uint32_t coop_max_open;   // this many earlier requests are unfinished
     uint64_t idle_ticks;    // wall-clock ticks without work before the loop ends
 };
 
@@ -256,6 +263,9 @@ hipError_t serve_loop_occupancy(const LoopArgs& a, uint32_t n_de_max, int* group
 // groups leaves on a CU without costing a group per CU, then reports the
 // groups per CU at that size.
 hipError_t serve_loop_fit(LoopArgs& a, uint32_t n_de_max, int* groups_per_cu);
+// Whether a group's LDS past the angle table holds the cooperative walk's
+// exchange buffers (the distance's working set shares that room).
+bool serve_loop_coop_fits(const LoopArgs& a);
 hipError_t launch_prepare(const PrepArgs& a, hipStream_t s);
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t s);
 // Sequential prefix sum of w (one lane, member order) into cum.

@@ -336,6 +336,168 @@ struct Particle {
 
 __device__ __forceinline__ float track_de(const Particle& q) { return q.sig2 * q.de; }
 
+// Per-tile constants of the walk: box and window in the units the walk uses
+// (z in box units when periodic: measured from the window's lower edge for
+// the slab, from the box floor for Gaussian detection (centred distance
+// q - 1/2); in um, centred, between reflecting walls).
+struct WalkConsts {
+    float hz, Lx, Ly, Lz, ox, oy, wx, wy, cg, four_dt;
+    uint32_t kmin, C;
+    BinSpec bs;
+};
+
+template <bool GAUSS, bool ZREF>
+__device__ __forceinline__ WalkConsts walk_consts(const DevSys& sy)
+{
+    constexpr bool ZBOX = !ZREF;
+    WalkConsts K;
+    K.hz = sy.half_l[2];
+    K.Lx = 2.0f * sy.half_l[0];
+    K.Ly = 2.0f * sy.half_l[1];
+    K.Lz = 2.0f * K.hz;
+    const float hax = sy.half_a[0], hay = sy.half_a[1];
+    K.ox = hax / K.Lx - 0.5f;  // window origin offsets
+    K.oy = hay / K.Ly - 0.5f;
+    K.wx = (hax + hax) / K.Lx;  // window widths
+    K.wy = (hay + hay) / K.Ly;
+    const float sdz = ZBOX ? sy.detect_sigma / K.Lz : sy.detect_sigma;  // Gaussian detection sigma, z's unit
+    K.cg = gauss_c(sdz * sdz);
+    K.kmin = sy.kmin;
+    K.C = sy.n_species;
+    K.four_dt = sy.four_dt;
+    K.bs = BinSpec{sy.four_dt, sy.de_bin_width, sy.n_k, sy.n_de};
+    return K;
+}
+
+// Particle q.id of the tile: start position, species (sigma, window depth)
+// and whether it starts inside the window, from Philox block 0 (and block
+// 0xffffffff for Gaussian detection).
+template <bool GAUSS, bool ZREF>
+__device__ __forceinline__ void init_particle(Particle& q, uint32_t N, uint32_t video, uint32_t sim, uint32_t k0,
+                                              uint32_t k1, const DevProp& pr, const WalkConsts& K)
+{
+    constexpr bool ZBOX = !ZREF;
+    q.active = q.id < N;
+    const U4 w = philox10(0u, q.id, video, sim, k0, k1);
+    q.x = fract_box(u24(w.x) + K.ox);
+    q.y = fract_box(u24(w.y) + K.oy);
+    const float us = u24(w.w);
+    uint32_t s = 0u;
+#pragma unroll
+    for (uint32_t c = 0; c + 1u < PCS_MAX_SPECIES; ++c) s += (c + 1u < K.C && us >= pr.cum_prob[c]) ? 1u : 0u;
+    float sg = pr.sigma[0], az = pr.half_az[0];
+#pragma unroll
+    for (uint32_t c = 1; c < PCS_MAX_SPECIES; ++c) {
+        sg = (s == c) ? pr.sigma[c] : sg;
+        az = (s == c) ? pr.half_az[c] : az;
+    }
+    q.sx = sg / K.Lx;
+    q.sy = sg / K.Ly;
+    q.sig2 = sg * sg;
+    if (ZBOX && GAUSS) {
+        q.sz = sg / K.Lz;
+        q.wz = 0.0f;
+        q.z = u24(w.z);
+    } else if (ZBOX) {
+        q.sz = sg / K.Lz;
+        q.wz = (az + az) / K.Lz;
+        q.z = fract_box(u24(w.z) + (az / K.Lz - 0.5f));
+    } else {
+        q.sz = sg;
+        q.wz = az;
+        q.z = K.hz * (2.0f * u24(w.z) - 1.0f);
+    }
+    bool det;
+    if (GAUSS) {
+        const U4 wd = philox10(0xffffffffu, q.id, video, sim, k0, k1);
+        det = q.x <= K.wx && q.y <= K.wy && gauss_detect(ZBOX ? q.z - 0.5f : q.z, K.cg, wd.x);
+    } else if (ZBOX) {
+        det = q.x <= K.wx && q.y <= K.wy && q.z <= q.wz;
+    } else {
+        det = q.x <= K.wx && q.y <= K.wy && __builtin_fabsf(q.z) <= q.wz;
+    }
+    q.ts = 0;
+    q.det0 = det;
+    q.de = 0.0f;
+}
+
+// Box-Muller pair p of a chunk from its radius word wr and the word wa that
+// holds its angle's low byte (byte p % 4): d0 = r sin, d1 = r cos, y = r^2.
+__device__ __forceinline__ void bm_pair(uint32_t wr, uint32_t wa, int p, const float2* tab, float& d0, float& d1,
+                                        float& y)
+{
+    y = neg2log_u24(wr);
+    const float r = sqrt_rn_domain(y);
+    float sn, cs;
+    sincos_tab(tab, wr & 0xffu, (wa >> (8 * (p & 3))) & 0xffu, sn, cs);
+    d0 = r * sn;
+    d1 = r * cs;
+}
+
+// Step s of a chunk (to frame s + 1 of the chunk's numbering) of particle q
+// with the unit normals (nx, ny, nz), its pair's r^2 and (Gaussian
+// detection) the word wdet: move, detect, close and stage a finished track,
+// update the open track. m_prev holds the lanes inside the window at the
+// frame before.
+template <bool GAUSS, bool ZREF, bool TRACKS>
+__device__ __forceinline__ void step_particle(Particle& q, uint64_t m_active, uint64_t& m_prev, float nx, float ny,
+                                              float nz, float r2, uint32_t wdet, uint32_t s, const WalkConsts& K,
+                                              Stage& st)
+{
+    constexpr bool ZBOX = !ZREF;
+    q.x = fract_box(__builtin_fmaf(q.sx, nx, q.x));
+    q.y = fract_box(__builtin_fmaf(q.sy, ny, q.y));
+    if (ZBOX) {
+        q.z = fract_box(__builtin_fmaf(q.sz, nz, q.z));
+    } else {
+        const float zn = __builtin_fmaf(q.sz, nz, q.z);
+        q.z = wrap_reflect(zn, K.hz);
+    }
+    const bool cx = q.x <= K.wx, cy = q.y <= K.wy;
+    const bool cz = GAUSS ? gauss_detect(ZBOX ? q.z - 0.5f : q.z, K.cg, wdet)
+                          : (ZBOX ? q.z <= q.wz : __builtin_fabsf(q.z) <= q.wz);
+    // Lanes inside the window and lanes closing a track: wave masks formed
+    // from the masks of the single comparisons (no per-lane boolean is
+    // materialised). This step reaches frame s + 1; the open track ends at
+    // frame s with k = s + 1 - ts, at least kmin iff ts <= s + 1 - kmin. The
+    // length test runs only in steps where some lane's track ends (about a
+    // quarter of them).
+    const int32_t f1 = (int32_t)s + 1;
+    const uint64_t m_det = ballot(cx) & ballot(cy) & ballot(cz);
+    const uint64_t m_end = m_active & m_prev & ~m_det;
+    if (m_end != 0ull) {
+        const uint64_t m_closing = m_end & ballot(q.ts <= f1 - (int32_t)K.kmin);
+        if (m_closing != 0ull) {
+            const bool closing = __builtin_amdgcn_inverse_ballot_w64(m_closing);
+            const uint32_t k = (uint32_t)(f1 - q.ts);
+            float v = track_de(q);
+            if (TRACKS && closing) v = v / ((float)(k - 1u) * K.four_dt);
+            stage_push2<!TRACKS>(st, m_closing, k, __float_as_uint(v), K.bs);
+        }
+    }
+    // nx^2 + ny^2 of this step is pair s's r^2, the log's result
+    const float de_n = q.de + r2;
+    q.de = __builtin_amdgcn_inverse_ballot_w64(m_det & m_prev) ? de_n : 0.0f;
+    q.ts = __builtin_amdgcn_inverse_ballot_w64(m_det & ~m_prev) ? f1 : q.ts;
+    m_prev = m_det;
+}
+
+// Tracks still open in the last frame (src/cpu/simulate.jl:114-121), which is
+// frame 0 after the last chunk's rebase: k = 1 - ts.
+template <bool TRACKS>
+__device__ __forceinline__ void close_open_tracks(Particle& q, uint64_t m_active, uint64_t m_prev, const WalkConsts& K,
+                                                  Stage& st)
+{
+    const uint64_t m_closing = m_active & m_prev & ballot(q.ts <= 1 - (int32_t)K.kmin);
+    if (m_closing != 0ull) {
+        const bool closing = __builtin_amdgcn_inverse_ballot_w64(m_closing);
+        const uint32_t k = (uint32_t)(1 - q.ts);
+        float v = track_de(q);
+        if (TRACKS && closing) v = v / ((float)(k - 1u) * K.four_dt);
+        stage_push2<!TRACKS>(st, m_closing, k, __float_as_uint(v), K.bs);
+    }
+}
+
 // TRACKS = true: the track-list form of src/cpu/generate_experiment.jl -
 // every track with k >= kmin is emitted as (k, de / ((k-1) 4 dt)) without
 // binning; otherwise tracks are binned records (row << 16 | bin).
@@ -348,27 +510,13 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
                                           uint32_t s0, uint32_t s1)
 {
     if (p0 >= N) return;  // empty tile (a request without particles)
-    // z in box units when periodic: measured from the window's lower edge for
-    // the slab, from the box floor for Gaussian detection (centred distance
-    // q - 1/2); in um, centred, between reflecting walls.
-    constexpr bool ZBOX = !ZREF;
     // The counter words other than the particle and the key are the same on
     // every lane: held in SGPRs, the Philox rounds that see only them (round 1,
     // and half of rounds 2 and 3) run on the scalar unit.
     video = uniform(video);
     const uint32_t sim = uniform(pr.sim_id);
     const uint32_t k0 = uniform(s0 ^ pr.key0), k1 = uniform(s1 ^ pr.key1);
-
-    const float hz = sy.half_l[2];
-    const float Lx = 2.0f * sy.half_l[0], Ly = 2.0f * sy.half_l[1], Lz = 2.0f * hz;
-    const float hax = sy.half_a[0], hay = sy.half_a[1];
-    const float ox = hax / Lx - 0.5f, oy = hay / Ly - 0.5f;  // window origin offsets
-    const float wx = (hax + hax) / Lx, wy = (hay + hay) / Ly;  // window widths
-    const float sdz = ZBOX ? sy.detect_sigma / Lz : sy.detect_sigma;  // Gaussian detection sigma, z's unit
-    const float cg = gauss_c(sdz * sdz);
-    const uint32_t kmin = sy.kmin, C = sy.n_species;
-    const float four_dt = sy.four_dt;
-    const BinSpec bs{four_dt, sy.de_bin_width, sy.n_k, sy.n_de};
+    const WalkConsts K = walk_consts<GAUSS, ZREF>(sy);
     // Philox blocks per 4-step chunk: 2 for the 12 unit normals (+1 with
     // Gaussian detection for its 4 uniforms).
     constexpr uint32_t B = GAUSS ? 3u : 2u;
@@ -376,50 +524,8 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
     Particle pt[PPL];
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
-        Particle& q = pt[i];
-        q.id = p0 + (uint32_t)(i * kWave) + lane_id();
-        q.active = q.id < N;
-        const U4 w = philox10(0u, q.id, video, sim, k0, k1);
-        q.x = fract_box(u24(w.x) + ox);
-        q.y = fract_box(u24(w.y) + oy);
-        const float us = u24(w.w);
-        uint32_t s = 0u;
-#pragma unroll
-        for (uint32_t c = 0; c + 1u < PCS_MAX_SPECIES; ++c) s += (c + 1u < C && us >= pr.cum_prob[c]) ? 1u : 0u;
-        float sg = pr.sigma[0], az = pr.half_az[0];
-#pragma unroll
-        for (uint32_t c = 1; c < PCS_MAX_SPECIES; ++c) {
-            sg = (s == c) ? pr.sigma[c] : sg;
-            az = (s == c) ? pr.half_az[c] : az;
-        }
-        q.sx = sg / Lx;
-        q.sy = sg / Ly;
-        q.sig2 = sg * sg;
-        if (ZBOX && GAUSS) {
-            q.sz = sg / Lz;
-            q.wz = 0.0f;
-            q.z = u24(w.z);
-        } else if (ZBOX) {
-            q.sz = sg / Lz;
-            q.wz = (az + az) / Lz;
-            q.z = fract_box(u24(w.z) + (az / Lz - 0.5f));
-        } else {
-            q.sz = sg;
-            q.wz = az;
-            q.z = hz * (2.0f * u24(w.z) - 1.0f);
-        }
-        bool det;
-        if (GAUSS) {
-            const U4 wd = philox10(0xffffffffu, q.id, video, sim, k0, k1);
-            det = q.x <= wx && q.y <= wy && gauss_detect(ZBOX ? q.z - 0.5f : q.z, cg, wd.x);
-        } else if (ZBOX) {
-            det = q.x <= wx && q.y <= wy && q.z <= q.wz;
-        } else {
-            det = q.x <= wx && q.y <= wy && __builtin_fabsf(q.z) <= q.wz;
-        }
-        q.ts = 0;
-        q.det0 = det;
-        q.de = 0.0f;
+        pt[i].id = p0 + (uint32_t)(i * kWave) + lane_id();
+        init_particle<GAUSS, ZREF>(pt[i], N, video, sim, k0, k1, pr, K);
     }
 
     // Per particle slot, wave masks of the lanes holding a particle and of the
@@ -447,13 +553,9 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
             const uint32_t w[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
             for (int p = 0; p < 6; ++p) {
-                const float y = neg2log_u24(w[p]);
-                const float r = sqrt_rn_domain(y);
+                float y;
+                bm_pair(w[p], w[6 + p / 4], p, tab, d[i][2 * p], d[i][2 * p + 1], y);
                 if (p < 4) r2[i][p] = y;
-                float sn, cs;
-                sincos_tab(tab, w[p] & 0xffu, (w[6 + p / 4] >> (8 * (p & 3))) & 0xffu, sn, cs);
-                d[i][2 * p] = r * sn;
-                d[i][2 * p + 1] = r * cs;
             }
             if (GAUSS) {
                 const U4 b2 = philox10(B * c + 3u, pt[i].id, video, sim, k0, k1);
@@ -468,65 +570,120 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
             if (s >= nst) break;
 #pragma unroll
             for (int i = 0; i < PPL; ++i) {
-                Particle& q = pt[i];
                 // Unit normals of this step (x and y from pair s, z from pairs 4
                 // and 5); the particle moves by sigma * n.
-                const float nx = d[i][2 * s], ny = d[i][2 * s + 1], nz = d[i][8 + s];
-                q.x = fract_box(__builtin_fmaf(q.sx, nx, q.x));
-                q.y = fract_box(__builtin_fmaf(q.sy, ny, q.y));
-                if (ZBOX) {
-                    q.z = fract_box(__builtin_fmaf(q.sz, nz, q.z));
-                } else {
-                    const float zn = __builtin_fmaf(q.sz, nz, q.z);
-                    q.z = wrap_reflect(zn, hz);
-                }
-                const bool cx = q.x <= wx, cy = q.y <= wy;
-                const bool cz = GAUSS ? gauss_detect(ZBOX ? q.z - 0.5f : q.z, cg, wdet[i][s])
-                                      : (ZBOX ? q.z <= q.wz : __builtin_fabsf(q.z) <= q.wz);
-                // Lanes inside the window and lanes closing a track: wave masks
-                // formed from the masks of the single comparisons (no per-lane
-                // boolean is materialised). This step reaches frame s + 1; the
-                // open track ends at frame s with k = s + 1 - ts, at least kmin
-                // iff ts <= s + 1 - kmin. The length test runs only in steps
-                // where some lane's track ends (about a quarter of them).
-                const int32_t f1 = (int32_t)s + 1;
-                const uint64_t m_det = ballot(cx) & ballot(cy) & ballot(cz);
-                const uint64_t m_end = m_active[i] & m_prev[i] & ~m_det;
-                if (m_end != 0ull) {
-                    const uint64_t m_closing = m_end & ballot(q.ts <= f1 - (int32_t)kmin);
-                    if (m_closing != 0ull) {
-                        const bool closing = __builtin_amdgcn_inverse_ballot_w64(m_closing);
-                        const uint32_t k = (uint32_t)(f1 - q.ts);
-                        float v = track_de(q);
-                        if (TRACKS && closing) v = v / ((float)(k - 1u) * four_dt);
-                        stage_push2<!TRACKS>(st, m_closing, k, __float_as_uint(v), bs);
-                    }
-                }
-                // nx^2 + ny^2 of this step is pair s's r^2, the log's result
-                const float de_n = q.de + r2[i][s];
-                q.de = __builtin_amdgcn_inverse_ballot_w64(m_det & m_prev[i]) ? de_n : 0.0f;
-                q.ts = __builtin_amdgcn_inverse_ballot_w64(m_det & ~m_prev[i]) ? f1 : q.ts;
-                m_prev[i] = m_det;
+                step_particle<GAUSS, ZREF, TRACKS>(pt[i], m_active[i], m_prev[i], d[i][2 * s], d[i][2 * s + 1],
+                                                   d[i][8 + s], r2[i][s], GAUSS ? wdet[i][s] : 0u, s, K, st);
             }
         }
 #pragma unroll
         for (int i = 0; i < PPL; ++i) pt[i].ts -= (int32_t)nst;  // the chunk's last frame becomes frame 0
     }
-    // Tracks still open in the last frame (src/cpu/simulate.jl:114-121), which
-    // is frame 0 after the rebase: k = 1 - ts.
 #pragma unroll
-    for (int i = 0; i < PPL; ++i) {
-        const uint64_t m_closing = m_active[i] & m_prev[i] & ballot(pt[i].ts <= 1 - (int32_t)kmin);
-        if (m_closing != 0ull) {
-            const bool closing = __builtin_amdgcn_inverse_ballot_w64(m_closing);
-            const uint32_t k = (uint32_t)(1 - pt[i].ts);
-            float v = track_de(pt[i]);
-            if (TRACKS && closing) v = v / ((float)(k - 1u) * four_dt);
-            stage_push2<!TRACKS>(st, m_closing, k, __float_as_uint(v), bs);
-        }
-    }
+    for (int i = 0; i < PPL; ++i) close_open_tracks<TRACKS>(pt[i], m_active[i], m_prev[i], K, st);
     if (TRACKS) stage_flush(st);
-    else stage_flush_binned(st, bs);
+    else stage_flush_binned(st, K.bs);
+}
+
+// Cooperative walk of one tile by the four waves of a serving group, for a
+// request's latency: a lone wave issues one instruction per few cycles, so
+// its tile takes as long as its whole instruction stream. Waves 0..2 produce
+// each 4-step chunk's unit normals - wave w the Box-Muller pairs 2w and 2w + 1
+// from the Philox blocks they need, wave 2 also the Gaussian-detection words -
+// into an LDS double buffer; wave 3 holds the particles, steps them and
+// stages their tracks. Round i produces chunk i into buffer i % 2 while chunk
+// i - 1 is stepped from the other; the group's barrier after each round hands
+// the buffers over. The values and their order are walk_tile's, so the
+// results are the same. xbuf: 2 * kCoopWords * 64 words (lane-minor, so a
+// wave's accesses to one word hit 64 consecutive words).
+constexpr int kCoopWords = 20;  // per lane and chunk: d[12], r^2 of pairs 0..3, detection words 0..3
+
+template <bool GAUSS, bool ZREF>
+__device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& sy, uint32_t F, uint32_t N,
+                                               uint32_t video, uint32_t p0, Stage& st, const float2* tab,
+                                               uint32_t* xbuf)
+{
+    if (p0 >= N) return;  // the same tile on every wave: all return together
+    const uint32_t wave = uniform(threadIdx.x / kWave), lane = lane_id();  // branches on it are uniform
+    video = uniform(video);
+    const uint32_t sim = uniform(pr.sim_id);
+    const uint32_t k0 = uniform(pr.key0), k1 = uniform(pr.key1);
+    const WalkConsts K = walk_consts<GAUSS, ZREF>(sy);
+    constexpr uint32_t B = GAUSS ? 3u : 2u;
+    const uint32_t id = p0 + lane;
+    const uint32_t steps = F > 0u ? F - 1u : 0u;
+    const uint32_t nc = (steps + 3u) / 4u;
+
+    Particle q;
+    uint64_t m_active = 0ull, m_prev = 0ull;
+    if (wave == 3u) {
+        q.id = id;
+        init_particle<GAUSS, ZREF>(q, N, video, sim, k0, k1, pr, K);
+        m_active = ballot(q.active);
+        m_prev = ballot(q.det0);
+    }
+    for (uint32_t it = 0; it <= nc; ++it) {
+        if (wave < 3u && it < nc) {
+            const uint32_t c = it;
+            float* xb = reinterpret_cast<float*>(xbuf) + (c & 1u) * (uint32_t)(kCoopWords * kWave) + lane;
+            const U4 b1 = philox10(B * c + 2u, id, video, sim, k0, k1);
+            uint32_t wr0, wr1, wa;
+            if (wave < 2u) {
+                const U4 b0 = philox10(B * c + 1u, id, video, sim, k0, k1);
+                wr0 = wave == 0u ? b0.x : b0.z;
+                wr1 = wave == 0u ? b0.y : b0.w;
+                wa = b1.z;
+            } else {
+                wr0 = b1.x;
+                wr1 = b1.y;
+                wa = b1.w;
+            }
+            const int pa = 2 * (int)wave;
+            float d0, d1, y0, d2, d3, y1;
+            bm_pair(wr0, wa, pa, tab, d0, d1, y0);
+            bm_pair(wr1, wa, pa + 1, tab, d2, d3, y1);
+            xb[(2 * pa) * kWave] = d0;
+            xb[(2 * pa + 1) * kWave] = d1;
+            xb[(2 * pa + 2) * kWave] = d2;
+            xb[(2 * pa + 3) * kWave] = d3;
+            if (wave < 2u) {
+                xb[(12 + pa) * kWave] = y0;
+                xb[(13 + pa) * kWave] = y1;
+            }
+            if (GAUSS && wave == 2u) {
+                const U4 b2 = philox10(B * c + 3u, id, video, sim, k0, k1);
+                xb[16 * kWave] = __uint_as_float(b2.x);
+                xb[17 * kWave] = __uint_as_float(b2.y);
+                xb[18 * kWave] = __uint_as_float(b2.z);
+                xb[19 * kWave] = __uint_as_float(b2.w);
+            }
+        } else if (wave == 3u && it > 0u) {
+            const uint32_t c = it - 1u;
+            const uint32_t nst = min(4u, steps - 4u * c);
+            const float* xb = reinterpret_cast<const float*>(xbuf) + (c & 1u) * (uint32_t)(kCoopWords * kWave) + lane;
+            float d[12], r2[4];
+            uint32_t wdet[4];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) d[k] = xb[k * kWave];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                r2[k] = xb[(12 + k) * kWave];
+                wdet[k] = GAUSS ? __float_as_uint(xb[(16 + k) * kWave]) : 0u;
+            }
+#pragma unroll
+            for (uint32_t s = 0; s < 4u; ++s) {
+                if (s >= nst) break;
+                step_particle<GAUSS, ZREF, false>(q, m_active, m_prev, d[2 * s], d[2 * s + 1], d[8 + s], r2[s],
+                                                  wdet[s], s, K, st);
+            }
+            q.ts -= (int32_t)nst;  // the chunk's last frame becomes frame 0
+        }
+        __syncthreads();
+    }
+    if (wave == 3u) {
+        close_open_tracks<false>(q, m_active, m_prev, K, st);
+        stage_flush_binned(st, K.bs);
+    }
 }
 
 // Tile t of a batch plan: the video owning it is the largest v with
@@ -1028,8 +1185,16 @@ __device__ bool loop_admit(const LoopArgs& a, uint64_t q)
     const uint32_t* vs = a.ring_vid + (uint64_t)slot * 3u * (a.maxv + 1u);
     uint32_t* vd = a.slot_vid + (uint64_t)slot * 3u * (a.maxv + 1u);
     for (uint32_t i = lane; i < vw; i += kWave) vd[i] = vs[i];
+    // A request of few tiles admitted onto a quiet loop has each tile walked by
+    // a whole group: a lone wave's walk is bound by its own instruction
+    // stream, and the group's walk issues more instructions per tile, which a
+    // loaded GPU pays for in throughput.
+    uint32_t coop = 0u;
+    if (l0 && a.coop_max_tiles != 0u && tiles <= a.coop_max_tiles)
+        coop = q - ld_agent64(&a.ctl->finished) <= (uint64_t)a.coop_max_open ? 1u : 0u;
     if (l0) {
         S.tiles = tiles;
+        S.coop = coop;
         S.n_videos = nv;
         S.tiles_done = 0u;
         S.rec_count = 0u;
@@ -1072,7 +1237,7 @@ __device__ void loop_admitter(const LoopArgs& a)
 // [t, t+4) of r when t < T. If the slot has moved on to a newer request q,
 // the increment landed on q's word and claims q's tiles: those are resolved
 // against q once the admitter has opened q, so that no tile is skipped.
-__device__ uint32_t loop_resolve(const LoopArgs& a, LoopSlot& S, uint64_t r, uint32_t T, uint64_t old,
+__device__ uint32_t loop_resolve(const LoopArgs& a, LoopSlot& S, uint64_t r, uint32_t T, uint64_t old, uint32_t k,
                                  uint64_t& r_out, uint32_t& t0, uint32_t& nt)
 {
     const bool l0 = lane_id() == 0u;
@@ -1081,7 +1246,7 @@ __device__ uint32_t loop_resolve(const LoopArgs& a, LoopSlot& S, uint64_t r, uin
         if (t >= T) return kLoopIdle;
         r_out = r;
         t0 = t;
-        nt = min((uint32_t)kLoopWaves, T - t);
+        nt = min(k, T - t);
         return kLoopWork;
     }
     for (uint32_t spin = 0; spin < (1u << 20); ++spin) {
@@ -1094,7 +1259,7 @@ __device__ uint32_t loop_resolve(const LoopArgs& a, LoopSlot& S, uint64_t r, uin
             if (t >= Tq) return kLoopIdle;
             r_out = q;
             t0 = t;
-            nt = min((uint32_t)kLoopWaves, Tq - t);
+            nt = min(k, Tq - t);
             return kLoopWork;
         }
         __builtin_amdgcn_s_sleep(2);  // the admitter is copying q in
@@ -1114,9 +1279,9 @@ __device__ __forceinline__ uint32_t hw_cu_slot()
     return (((xcc & 7u) << 8) | ((hw >> 8) & 0xffu)) & (kLoopCuSlots - 1u);
 }
 
-// Worker decision (wave 0): kLoopWork with (r, t0, nt) claimed, kLoopIdle or
-// kLoopExit.
-__device__ uint32_t loop_decide(const LoopArgs& a, uint64_t& r_out, uint32_t& t0, uint32_t& nt)
+// Worker decision (wave 0): kLoopWork with (r, t0, nt) claimed (coop: one
+// tile, walked by the whole group), kLoopIdle or kLoopExit.
+__device__ uint32_t loop_decide(const LoopArgs& a, uint64_t& r_out, uint32_t& t0, uint32_t& nt, bool& coop)
 {
     const bool l0 = lane_id() == 0u;
     for (int attempt = 0; attempt < 8; ++attempt) {
@@ -1125,15 +1290,20 @@ __device__ uint32_t loop_decide(const LoopArgs& a, uint64_t& r_out, uint32_t& t0
         LoopSlot& S = a.slots[r % a.R];
         const uint64_t st = bcast64(l0 ? ld_agent64(&S.state) : 0ull);
         if (st == 4ull * r + 2ull) {
-            const uint32_t T = uniform(l0 ? ld_agent32(&S.tiles) : 0u);
+            const uint64_t tc = bcast64(l0 ? ld_agent64(reinterpret_cast<uint64_t*>(&S.tiles)) : 0ull);
+            const uint32_t T = (uint32_t)tc;
             const uint64_t w = bcast64(l0 ? ld_agent64(&S.claim) : 0ull);
             if ((uint32_t)(w >> 32) == (uint32_t)r && (uint32_t)w < T) {
+                // A request the admitter marked is claimed a tile at a time,
+                // each walked by the whole group; others kLoopWaves tiles at a
+                // time, a wave each.
+                const uint32_t k = (tc >> 32) != 0ull ? 1u : (uint32_t)kLoopWaves;
+                coop = k == 1u;
                 uint64_t old = 0ull;
-                if (l0) old = __hip_atomic_fetch_add(&S.claim, (uint64_t)kLoopWaves, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
+                if (l0) old = __hip_atomic_fetch_add(&S.claim, (uint64_t)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 old = bcast64(old);
-                const uint32_t k = loop_resolve(a, S, r, T, old, r_out, t0, nt);
-                if (k != kLoopIdle) return k;
+                const uint32_t kind = loop_resolve(a, S, r, T, old, k, r_out, t0, nt);
+                if (kind != kLoopIdle) return kind;
                 continue;
             }
         } else if (st != 4ull * r + 3ull) {
@@ -1147,6 +1317,29 @@ __device__ uint32_t loop_decide(const LoopArgs& a, uint64_t& r_out, uint32_t& t0
     return kLoopIdle;
 }
 
+// The cooperative walk as a function of its own, called from the serving
+// loop: its registers are allocated apart from the loop's (inlined, its four
+// model classes raised the loop's register pressure and spill traffic on the
+// path every request takes). The proposal and the model are read from their
+// HBM copies and the stage is passed by value (empty on entry and return), so
+// that nothing of the caller's has its address taken.
+__attribute__((noinline)) __device__ void walk_tile_coop_call(const DevProp* gp, const DevSys* gs, uint32_t F,
+                                                              uint32_t N, uint32_t video, uint32_t p0, Stage st,
+                                                              const float2* tab, uint32_t* xbuf)
+{
+    const DevProp pr = load_uniform(gp);
+    const DevSys sy = load_uniform(gs);
+    const bool gauss = sy.detect_model == PCS_DETECT_GAUSS_Z;
+    const bool zref = sy.z_boundary == PCS_Z_REFLECT;
+    if (gauss) {
+        if (zref) walk_tile_coop<true, true>(pr, sy, F, N, video, p0, st, tab, xbuf);
+        else walk_tile_coop<true, false>(pr, sy, F, N, video, p0, st, tab, xbuf);
+    } else {
+        if (zref) walk_tile_coop<false, true>(pr, sy, F, N, video, p0, st, tab, xbuf);
+        else walk_tile_coop<false, false>(pr, sy, F, N, video, p0, st, tab, xbuf);
+    }
+}
+
 template <int SEG>
 __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per_eu(6, 6))) void serve_loop_kernel(LoopArgs a)
 {
@@ -1155,7 +1348,7 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
     __shared__ uint32_t smax;
     __shared__ uint64_t red[kDistRed];
     __shared__ uint64_t cmd_r;
-    __shared__ uint32_t cmd[5];  // kind, t0, nt, last, sleep units
+    __shared__ uint32_t cmd[6];  // kind, t0, nt, last, sleep units, coop
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid / kWave;
@@ -1191,7 +1384,8 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
         if (wave == 0u) {
             uint64_t r = 0ull;
             uint32_t t0 = 0u, nt = 0u;
-            uint32_t kind = loop_decide(a, r, t0, nt);
+            bool coop = false;
+            uint32_t kind = loop_decide(a, r, t0, nt, coop);
             const uint64_t now = wall_clock64();
             uint32_t units = 1u;
             if (kind == kLoopWork) {
@@ -1234,6 +1428,7 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
                 cmd[1] = t0;
                 cmd[2] = nt;
                 cmd[4] = units;
+                cmd[5] = coop ? 1u : 0u;
                 cmd_r = r;
             }
         }
@@ -1248,10 +1443,11 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
         }
         const uint64_t r = bcast64(cmd_r);
         const uint32_t t0 = uniform(cmd[1]), nt = uniform(cmd[2]);
+        const bool coop = uniform(cmd[5]) != 0u;
         const uint32_t slot = (uint32_t)(r % a.R);
         LoopSlot& S = a.slots[slot];
-        if (wave < nt) {
-            const uint32_t t = t0 + wave;
+        if (coop || wave < nt) {
+            const uint32_t t = t0 + (coop ? 0u : wave);
             const uint32_t* vid = a.slot_vid + (uint64_t)slot * vw;
             uint32_t lo = 0u, hi = uniform(S.n_videos);
             while (hi - lo > 1u) {
@@ -1267,7 +1463,9 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
             const uint32_t p0 = (t - uniform(vid[3u * lo + 2u])) * (uint32_t)kWave;
             const bool gauss = sy.detect_model == PCS_DETECT_GAUSS_Z;
             const bool zref = sy.z_boundary == PCS_Z_REFLECT;
-            if (gauss) {
+            if (coop) {  // exchange buffers in the distance's room, idle while a tile is walked
+                walk_tile_coop_call(&S.prop, &a.sys[pr.sys_id], F, N, lo, p0, st, tab, rowc);
+            } else if (gauss) {
                 if (zref) walk_tile<1, true, true, false>(pr, sy, F, N, lo, p0, st, tab, 0u, 0u);
                 else walk_tile<1, true, false, false>(pr, sy, F, N, lo, p0, st, tab, 0u, 0u);
             } else {
@@ -1293,8 +1491,10 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
                 d.dist = dist_log10(o);
                 d.tracks = o.tracks;
                 d.status = o.status;
+                d.group_walked = S.coop;
                 __hip_atomic_store(&S.state, 4ull * r + 3ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(&d.seq, r + 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_fetch_add(&a.ctl->finished, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         __syncthreads();
@@ -1976,6 +2176,11 @@ hipError_t launch_distance(const DistArgs& args, uint32_t n_k_max, uint32_t n_de
     else if (n_de_max <= 2048) launch_distance_t<8>(a, base, s);
     else launch_distance_t<16>(a, base, s);
     return hipGetLastError();
+}
+
+bool serve_loop_coop_fits(const LoopArgs& a)
+{
+    return (size_t)a.nk_lds + a.nde_lds + a.srtl_cap / 2u >= (size_t)2 * kCoopWords * kWave;
 }
 
 size_t serve_loop_lds(const LoopArgs& a)

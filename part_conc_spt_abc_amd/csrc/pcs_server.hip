@@ -176,6 +176,7 @@ struct pcs_server {
     uint32_t max_queue = 0;
     double cap_factor = 1.0;  // record slice estimate scale (pcs_server_set_record_capacity_factor)
     uint64_t reruns = 0;      // requests rerun after a record slice overflow
+    uint64_t group_walked = 0;  // device loop: completed requests whose tiles were walked a tile per group
 
     std::mutex mu;
     std::condition_variable cv;
@@ -507,6 +508,7 @@ uint32_t loop_collect(pcs_server* s, std::vector<pcs_completion>& out)
 {
     LoopHost& L = s->loop;
     std::vector<Pending> again;
+    uint64_t n_group = 0;
     out.clear();
     for (size_t i = 0; i < L.live.size();) {
         const uint32_t slot = L.live[i];
@@ -530,11 +532,13 @@ uint32_t loop_collect(pcs_server* s, std::vector<pcs_completion>& out)
         c.submit_us = b.req.submit_us;
         c.done_us = s->now_us();
         out.push_back(c);
+        n_group += d.group_walked != 0u ? 1u : 0u;
     }
     const uint32_t n_again = (uint32_t)again.size();
     push_back_reruns(s, again);
     if (!out.empty()) {
         std::lock_guard<std::mutex> lk(s->mu);
+        s->group_walked += n_group;
         for (auto& c : out) s->done.push_back(c);
         s->cv.notify_all();
     }
@@ -573,6 +577,13 @@ int loop_launch(pcs_server* s, uint32_t nsys)
     if (gpc < 1) return set_error(PCS_EINVAL, "serving loop does not fit a CU (n_k %u, n_de %u)", nk, nde);
     // group 0 admits requests, the others walk them
     const uint32_t grid = std::max(2u, (uint32_t)L.n_cu * (uint32_t)std::min(gpc, 8));
+    // Requests of up to one tile per CU, admitted while at most 12 earlier
+    // requests are unfinished, are walked a tile per group: a quiet GPU's
+    // polling groups (one per CU) claim them all at once. The exchange buffers
+    // need their LDS room.
+    a.coop_max_tiles = (!(s->cfg.flags & PCS_SERVE_LOOP_ONE_WAVE_TILES) && serve_loop_coop_fits(a))
+                           ? std::min((uint32_t)L.n_cu, (grid - 1u) / 4u) : 0u;
+    a.coop_max_open = 12u;
     e = hipMemsetAsync(&L.d_ctl->quit, 0, sizeof(uint32_t), L.stream);
     if (e == hipSuccess) e = launch_serve_loop(a, grid, nde, L.stream);
     if (e == hipSuccess) e = hipEventRecord(L.ev, L.stream);
@@ -741,13 +752,14 @@ int pcs_server_create(int device, const pcs_server_config* cfg, pcs_server** out
     if (!s->cfg.arena_bytes) s->cfg.arena_bytes = 1ull << 30;
     if (!s->cfg.max_inflight) s->cfg.max_inflight = 4096;
     if (!s->cfg.max_videos) s->cfg.max_videos = 64;
-    if (s->cfg.flags & ~PCS_SERVE_DEVICE_LOOP) {
+    const uint32_t flags = s->cfg.flags, max_videos = s->cfg.max_videos;
+    if (flags & ~(PCS_SERVE_DEVICE_LOOP | PCS_SERVE_LOOP_ONE_WAVE_TILES)) {
         delete s;
-        return set_error(PCS_EINVAL, "unknown server flags 0x%x", s->cfg.flags);
+        return set_error(PCS_EINVAL, "unknown server flags 0x%x", flags);
     }
-    if (s->cfg.max_videos > 4096) {
+    if (max_videos > 4096) {
         delete s;
-        return set_error(PCS_EINVAL, "max_videos %u above 4096", s->cfg.max_videos);
+        return set_error(PCS_EINVAL, "max_videos %u above 4096", max_videos);
     }
     s->max_queue = 4 * s->cfg.max_inflight;
     s->systems.reserve(kMaxSystems);  // the scheduler reads entries without the lock
@@ -879,6 +891,14 @@ int pcs_server_overflow_reruns(pcs_server* s, uint64_t* n)
     if (!s || !n) return set_error(PCS_EINVAL, "NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     *n = s->reruns;
+    return PCS_OK;
+}
+
+int pcs_server_group_walked(pcs_server* s, uint64_t* n)
+{
+    if (!s || !n) return set_error(PCS_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *n = s->group_walked;
     return PCS_OK;
 }
 

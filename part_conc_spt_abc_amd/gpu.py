@@ -255,12 +255,16 @@ def selftest_sqrt(lo_bits: int, hi_bits: int, device: int = 0) -> int:
 class Server:
     """Continuous-batching serving loop (pcs_server_*). device_loop=True
     selects the device-resident form (one persistent kernel fed from a ring in
-    pinned host memory) instead of host-driven launch groups."""
+    pinned host memory) instead of host-driven launch groups; with it,
+    one_wave_tiles=True walks every tile with one wave (a small request's
+    tiles are otherwise walked by a whole group each)."""
 
     def __init__(self, device: int = 0, arena_bytes: int = 0, max_inflight: int = 0, max_videos: int = 0,
-                 device_loop: bool = False, loop_idle_us: int = 0):
+                 device_loop: bool = False, loop_idle_us: int = 0, one_wave_tiles: bool = False):
         self.lib = _abi.load()
         flags = _abi.PCS_SERVE_DEVICE_LOOP if device_loop else 0
+        if one_wave_tiles:
+            flags |= _abi.PCS_SERVE_LOOP_ONE_WAVE_TILES
         cfg = _abi.pcs_server_config(arena_bytes, max_inflight, max_videos, flags, loop_idle_us)
         h = C.c_void_p()
         _abi.check(self.lib.pcs_server_create(device, C.byref(cfg), C.byref(h)), "pcs_server_create")
@@ -313,6 +317,12 @@ class Server:
     def overflow_reruns(self) -> int:
         n = C.c_uint64(0)
         _abi.check(self.lib.pcs_server_overflow_reruns(self.h, C.byref(n)), "pcs_server_overflow_reruns")
+        return int(n.value)
+
+    def group_walked(self) -> int:
+        """Completed requests whose tiles were each walked by a whole group."""
+        n = C.c_uint64(0)
+        _abi.check(self.lib.pcs_server_group_walked(self.h, C.byref(n)), "pcs_server_group_walked")
         return int(n.value)
 
     def now_us(self) -> float:

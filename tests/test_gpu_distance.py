@@ -80,10 +80,11 @@ def test_distance_over_record_counts(gpu_lib, n_k, n_de):
         assert dist[i] == pytest.approx(math.log10(ex), rel=0, abs=1e-12)
 
 
-@pytest.mark.parametrize("device_loop", [False, True])
-def test_serving_distance_over_record_counts(gpu_lib, device_loop):
+@pytest.mark.parametrize("device_loop,one_wave", [(False, False), (True, False), (True, True)])
+def test_serving_distance_over_record_counts(gpu_lib, device_loop, one_wave):
     """The serving forms on the same sweep at H 1000x500 (the request loop
-    sorts into LDS up to its own capacity and re-reads the records)."""
+    sorts into LDS up to its own capacity and re-reads the records; its
+    tiles walked by whole groups or by one wave each)."""
     model = cases.headline_model()
     obs = observed(model)
     props, f, n = sweep(model, n_sims=40)
@@ -94,7 +95,7 @@ def test_serving_distance_over_record_counts(gpu_lib, device_loop):
         dist, tracks, _ = b.read()
         exact = b.read_exact()
         b.close()
-    srv = Server(device=0, arena_bytes=2 << 30, max_inflight=64, device_loop=device_loop)
+    srv = Server(device=0, arena_bytes=2 << 30, max_inflight=64, device_loop=device_loop, one_wave_tiles=one_wave)
     sid = srv.add_system(model, obs)
     for i, p in enumerate(props):
         q = _abi.pcs_proposal.from_buffer_copy(p)

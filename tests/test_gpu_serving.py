@@ -256,3 +256,62 @@ def test_device_loop_empty_and_multi_video_requests():
     assert got == ref
     # the empty requests: no tracks
     assert all(got[t][1] == 0 for t in range(0, 12, 4))
+
+
+def test_device_loop_group_walked_tiles_at_chunk_edges():
+    """A request of few tiles admitted onto a quiet loop is walked a tile per
+    group (three waves produce each 4-step chunk's normals, one steps). Frame
+    counts around the chunk edges (0..9 frames: no step, partial chunks,
+    whole chunks) and particle counts around a tile (0, 1, 63, 64, 65, 200)
+    in all three model classes give the same results as the one-wave walk,
+    the host-driven loop and a resident batch. Submitted one at a time, every
+    request is group-walked; submitted all at once, those admitted behind
+    more than 12 unfinished ones are walked a wave per tile, with the same
+    results."""
+    rng = np.random.default_rng(33)
+    ch = chains(rng, 3)
+    reqs = []
+    for ci, (m, D, obs) in enumerate(ch):
+        p0 = proposal_record(m, D, list(np.full(m.n_species, 1e9 / m.n_species)), [2.0], 0, 0, 1)
+        for F in (0, 1, 2, 4, 5, 6, 9, 1000):
+            for N in (0, 1, 63, 64, 65, 200):
+                p = _abi.pcs_proposal.from_buffer_copy(p0)
+                p.sim_id = len(reqs)
+                reqs.append((ci, p, np.array([F], np.uint32), np.array([N], np.uint32), 900 + len(reqs)))
+    ref = host_loop_results(ch, reqs)
+    out = {}
+    for one_wave in (False, True):
+        print(f"device loop one_wave_tiles={one_wave}", flush=True)
+        srv = Server(device=0, arena_bytes=1 << 30, max_inflight=256, device_loop=True, one_wave_tiles=one_wave)
+        sids = [srv.add_system(m, obs) for m, _, obs in ch]
+        alone = {}
+        for tag, (ci, p, f, n, seed) in enumerate(reqs):
+            assert srv.submit(tag, sids[ci], p, f, n, seed)
+            alone.update(collect(srv, 1, timeout=30.0))
+        walked_alone = srv.group_walked()
+        for tag, (ci, p, f, n, seed) in enumerate(reqs):
+            assert srv.submit(tag, sids[ci], p, f, n, seed)
+        mixed = collect(srv, len(reqs), timeout=60.0)
+        walked_mixed = srv.group_walked() - walked_alone
+        srv.close()
+        print(f"group-walked: {walked_alone} alone, {walked_mixed} of {len(reqs)} mixed", flush=True)
+        if one_wave:
+            assert walked_alone == 0 and walked_mixed == 0
+        else:
+            assert walked_alone == len(reqs)
+        assert alone == mixed
+        out[one_wave] = alone
+    assert out[False] == ref
+    assert out[True] == ref
+    assert any(t > 0 for _, t in ref.values())
+    for ci, (m, D, obs) in enumerate(ch):
+        with Context(m, obs) as ctx:
+            for tag, (c2, p, f, n, seed) in enumerate(reqs):
+                if c2 != ci or int(f[0]) != 1000:
+                    continue
+                b = ctx.batch()
+                b.load([p], f, n)
+                b.run(seed, _abi.PCS_MODE_BATCHED)
+                dist, tracks, _ = b.read()
+                b.close()
+                assert (float(dist[0]), int(tracks[0])) == ref[tag], (ci, tag)

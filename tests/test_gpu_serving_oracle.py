@@ -5,10 +5,11 @@ The serving workload: 32 concurrent chains, each with its own box (lateral
 100-600 nm, H 1000 x 500 uint32, 1,000 frames, total concentration
 log-uniform on 1e8..1e10 /mL with at most 60,000 particles per request. Here
 the chains also cycle through the three model classes (periodic slab,
-reflecting z, Gaussian z detection). Both serving forms (host-driven launch
-groups and the device-resident loop) must give every request the same
-(exact distance sum, tracks) alone, mixed with the others and as a resident
-batch, and a sample of requests of every class must equal the oracle's
+reflecting z, Gaussian z detection). Every serving form (host-driven launch
+groups; the device-resident loop, whose small requests' tiles are each walked
+by a whole group, and the same loop with every tile walked by one wave) must
+give every request the same (exact distance sum, tracks) alone, mixed with the
+others and as a resident batch, and a sample of requests of every class must equal the oracle's
 orc_simulate_f32 + orc_distance. Record slices forced to overflow are rerun
 and still equal the oracle.
 """
@@ -76,8 +77,15 @@ def collect(srv, n, timeout):
     return got
 
 
-def serve(ch, reqs, device_loop, alone, cap_factor=None, arena=4 << 30):
-    srv = Server(device=0, arena_bytes=arena, max_inflight=1024, device_loop=device_loop)
+FORMS = ("host", "loop", "loop_one_wave")
+
+
+def server(form, **kw):
+    return Server(device=0, device_loop=form != "host", one_wave_tiles=form == "loop_one_wave", **kw)
+
+
+def serve(ch, reqs, form, alone, cap_factor=None, arena=4 << 30):
+    srv = server(form, arena_bytes=arena, max_inflight=1024)
     if cap_factor is not None:
         srv.set_record_capacity_factor(cap_factor)
     sids = [srv.add_system(m, obs) for m, _, obs in ch]
@@ -94,9 +102,13 @@ def serve(ch, reqs, device_loop, alone, cap_factor=None, arena=4 << 30):
                 time.sleep(0.0003)  # later requests arrive while earlier ones run
         got = collect(srv, len(reqs), 120.0)
     reruns = srv.overflow_reruns()
+    walked = srv.group_walked()
+    if form != "loop":
+        assert walked == 0
+    print(f"  {walked} of {len(reqs)} requests group-walked, {reruns} reruns", flush=True)
     assert srv.pending() == 0
     srv.close()
-    return got, reruns
+    return got, reruns, walked
 
 
 def oracle_result(ch, req):
@@ -140,11 +152,13 @@ def test_serving_workload_full_size_matches_oracle(gpu_lib):
           f"{int((npart == 60000).sum())} at the 60,000 cap", flush=True)
     assert int((npart == 60000).sum()) >= 3
     results = {}
-    for device_loop in (False, True):
+    for form in FORMS:
         for alone in (True, False):
-            print(f"device_loop={device_loop} alone={alone}", flush=True)
-            results[(device_loop, alone)], _ = serve(ch, reqs, device_loop, alone)
-    ref = results[(False, True)]
+            print(f"form={form} alone={alone}", flush=True)
+            results[(form, alone)], _, walked = serve(ch, reqs, form, alone)
+            if form == "loop" and alone:  # every request of at most 256 tiles (16,384 particles)
+                assert walked == sum(int(r[3][0]) <= 256 * 64 for r in reqs)
+    ref = results[("host", True)]
     for key, got in results.items():
         assert got == ref, key
     # each chain's requests as one resident batch of its model
@@ -168,15 +182,15 @@ def test_serving_workload_full_size_matches_oracle(gpu_lib):
         assert ref[i][2] == pytest.approx(math.log10(ex), rel=0, abs=1e-12)
 
 
-@pytest.mark.parametrize("device_loop", [False, True])
-def test_serving_overflow_reruns_match_oracle(gpu_lib, device_loop):
+@pytest.mark.parametrize("form", FORMS)
+def test_serving_overflow_reruns_match_oracle(gpu_lib, form):
     """Record slices sized far below the tracks (capacity factor 1e-4) all
     overflow; the server reruns each request with an exact slice before it
     completes, and the results equal the oracle's."""
     rng = np.random.default_rng(13)
     ch = serving_chains(rng, n=3, n_k=500, n_de=200)
     reqs = serving_requests(rng, ch, per_chain=3, frames=400, max_particles=6000)
-    got, reruns = serve(ch, reqs, device_loop, alone=False, cap_factor=1e-4, arena=1 << 30)
+    got, reruns, _ = serve(ch, reqs, form, alone=False, cap_factor=1e-4, arena=1 << 30)
     print(f"reruns {reruns}", flush=True)
     assert reruns > 0
     with cf.ThreadPoolExecutor(8) as ex:
