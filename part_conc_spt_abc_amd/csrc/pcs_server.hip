@@ -742,6 +742,9 @@ int pcs_server_create(int device, const pcs_server_config* cfg, pcs_server** out
 {
     if (!out) return set_error(PCS_EINVAL, "out is NULL");
     *out = nullptr;
+    if (cfg && (cfg->flags & ~(PCS_SERVE_DEVICE_LOOP | PCS_SERVE_LOOP_ONE_WAVE_TILES)))
+        return set_error(PCS_EINVAL, "unknown server flags 0x%x", cfg->flags);
+    if (cfg && cfg->max_videos > 4096) return set_error(PCS_EINVAL, "max_videos %u above 4096", cfg->max_videos);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PCS_ENODEV, "no GPU visible");
     if (device < 0 || device >= ndev) return set_error(PCS_EINVAL, "device %d outside [0, %d)", device, ndev);
@@ -752,15 +755,6 @@ int pcs_server_create(int device, const pcs_server_config* cfg, pcs_server** out
     if (!s->cfg.arena_bytes) s->cfg.arena_bytes = 1ull << 30;
     if (!s->cfg.max_inflight) s->cfg.max_inflight = 4096;
     if (!s->cfg.max_videos) s->cfg.max_videos = 64;
-    const uint32_t flags = s->cfg.flags, max_videos = s->cfg.max_videos;
-    if (flags & ~(PCS_SERVE_DEVICE_LOOP | PCS_SERVE_LOOP_ONE_WAVE_TILES)) {
-        delete s;
-        return set_error(PCS_EINVAL, "unknown server flags 0x%x", flags);
-    }
-    if (max_videos > 4096) {
-        delete s;
-        return set_error(PCS_EINVAL, "max_videos %u above 4096", max_videos);
-    }
     s->max_queue = 4 * s->cfg.max_inflight;
     s->systems.reserve(kMaxSystems);  // the scheduler reads entries without the lock
     s->arena_records = s->cfg.arena_bytes / 8;  // records + sort scratch

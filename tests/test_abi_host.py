@@ -76,6 +76,15 @@ def test_host_only_calls_without_gpu():
     us = C.c_double()
     assert lib.pcs_server_now_us(None, C.byref(us)) == _abi.PCS_EINVAL
     assert lib.pcs_server_set_record_capacity_factor(None, 1.0) == _abi.PCS_EINVAL
+    n64 = C.c_uint64()
+    assert lib.pcs_server_group_walked(None, C.byref(n64)) == _abi.PCS_EINVAL
+    assert lib.pcs_server_overflow_reruns(None, C.byref(n64)) == _abi.PCS_EINVAL
+    assert lib.pcs_batch_pmc_weights_async(None, one) == _abi.PCS_EINVAL
+    assert lib.pcs_batch_pmc_weights_wait(None, out) == _abi.PCS_EINVAL
+    # unknown server flags are refused before any device work
+    cfg = _abi.pcs_server_config(0, 0, 0, 0x80, 0)
+    assert lib.pcs_server_create(0, C.byref(cfg), C.byref(h)) == _abi.PCS_EINVAL
+    assert b"flags" in lib.pcs_last_error()
 
 
 def test_reference_hist_matches_oracle():
