@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 
 #include "pcs_device.h"
@@ -52,6 +53,22 @@ __device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Global-memory view of a pointer whose address space the compiler cannot
+// infer (one read out of memory): accesses through it compile to global
+// instructions, counted by the vector-memory counter alone, so the compiler
+// can wait for one load at a time. A flat access might touch LDS and complete
+// out of order, and makes it wait for every outstanding access.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gview(const T* p)
+{
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gview(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
 
 // Orders this wave's LDS accesses across lanes (staging buffer handoff).
 __device__ __forceinline__ void wave_lds_sync()
@@ -828,7 +845,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     uint64_t* red = L.red;
     const uint32_t tid = threadIdx.x;
     const uint32_t n_k = sy.n_k, n_de = sy.n_de;
-    const uint32_t* rec = pr.rec;
+    const auto* rec = gview(pr.rec);
 
     if (count > pr.rec_cap)  // record slice overflowed: counted, not stored
         return DistOut{0ull, 0ull, count, PCS_EOVERFLOW};
@@ -852,11 +869,17 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     constexpr uint32_t kNoRecord = ~0u;
     uint32_t held[KREG > 0 ? KREG : 1];
     uint32_t mr = 0u;
+    // Loads are issued for every slot at an index clamped into the slice and
+    // masked afterwards: a load under a lane mask would be waited for before
+    // the next one is issued.
 #pragma unroll
     for (int u = 0; u < KREG; ++u) {
         const uint32_t t = tid + (uint32_t)u * kDistThreads;
         held[u] = kNoRecord;
-        if ((uint32_t)u * kDistThreads < nheld) held[u] = (t < nheld) ? rec[t] : kNoRecord;
+        if ((uint32_t)u * kDistThreads < nheld) {
+            const uint32_t v = rec[min(t, nheld - 1u)];
+            held[u] = (t < nheld) ? v : kNoRecord;
+        }
     }
 #pragma unroll
     for (int u = 0; u < KREG; ++u) {
@@ -866,14 +889,12 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
             mr = max(mr, r);
         }
     }
-    // Past the held records: each thread keeps kDistLoads loads in flight.
+    // Past the held records: each thread keeps kDistLoads loads in flight
+    // (unmasked, at indices clamped into the slice).
     for (uint32_t t0 = nheld + tid; t0 < T; t0 += kDistLoads * kDistThreads) {
         uint32_t v[kDistLoads];
 #pragma unroll
-        for (int u = 0; u < kDistLoads; ++u) {
-            const uint32_t t = t0 + (uint32_t)u * kDistThreads;
-            v[u] = (t < T) ? rec[t] : 0u;
-        }
+        for (int u = 0; u < kDistLoads; ++u) v[u] = rec[min(t0 + (uint32_t)u * kDistThreads, T - 1u)];
 #pragma unroll
         for (int u = 0; u < kDistLoads; ++u) {
             if (t0 + (uint32_t)u * kDistThreads < T) {
@@ -905,7 +926,7 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     // Scatter bins by row; afterwards rowc[r] is one past the last slot of row r.
     const bool in_lds = T <= L.srtl_cap;
     uint16_t* srtl = L.srtl;
-    uint16_t* srtg = reinterpret_cast<uint16_t*>(pr.srt);
+    auto* srtg = gview(reinterpret_cast<uint16_t*>(pr.srt));
     auto put = [&](uint32_t v) {
         const uint32_t pos = atomicAdd(&rowc[v >> 16], 1u);
         if (in_lds) srtl[pos] = (uint16_t)v;
@@ -920,118 +941,174 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     for (uint32_t t0 = nheld + tid; t0 < T; t0 += kDistLoads * kDistThreads) {
         uint32_t v[kDistLoads];
 #pragma unroll
-        for (int u = 0; u < kDistLoads; ++u) {
-            const uint32_t t = t0 + (uint32_t)u * kDistThreads;
-            v[u] = (t < T) ? rec[t] : 0u;
-        }
+        for (int u = 0; u < kDistLoads; ++u) v[u] = rec[min(t0 + (uint32_t)u * kDistThreads, T - 1u)];
 #pragma unroll
         for (int u = 0; u < kDistLoads; ++u)
             if (t0 + (uint32_t)u * kDistThreads < T) put(v[u]);
     }
     __threadfence_block();
     __syncthreads();
-    auto bin_at = [&](uint32_t t) -> uint32_t { return in_lds ? (uint32_t)srtl[t] : (uint32_t)srtg[t]; };
-
     const uint32_t last = max(*L.smax, sy.maxrow_obs);  // < n_k
     const uint32_t j0 = tid * SEG;
-    uint32_t Ccol[SEG];
-#pragma unroll
-    for (int m = 0; m < SEG; ++m) Ccol[m] = 0u;
     uint64_t acc = 0ull, acc_hi = 0ull;
     const bool l1 = sy.dist_kind == PCS_DIST_L1;
-    const uint32_t* R = sy.obs_sat;
+    const auto* R = gview(sy.obs_sat);
     const uint32_t maxrow_obs = sy.maxrow_obs;
     const uint32_t lane = tid % kWave;
 
-    // Observed rows are loaded PF rows ahead of their use into a ring of
-    // register slots (slot d holds row i0 + d). Past the deepest observed row
-    // the table repeats that row, so a refill asks for row
-    // min(i + PF, maxrow_obs): no load leaves the table, and the rows past it
-    // are all served from one row in cache.
-    constexpr int PF = SEG >= 8 ? 2 : (SEG >= 4 ? 4 : 8);
-    uint32_t rb[PF][SEG];
-    auto load_row = [&](uint32_t (&dst)[SEG], uint32_t row) {
-        const uint32_t* Rrow = R + (uint64_t)row * n_de;
+    // The row sweep, compiled for bins sorted into LDS and for bins in HBM.
+    // Every row issues the same global loads on every path (the observed row
+    // PF rows ahead and, bins in HBM, the next row's sparse bins): the
+    // compiler can then wait for a row's own loads, issued rows earlier,
+    // instead of for every load in flight, which made each row of a lone
+    // distance cost a memory round trip.
+    auto sweep = [&](auto bins_in_lds) {
+        constexpr bool IN_LDS = decltype(bins_in_lds)::value;
+        auto bin_at = [&](uint32_t t) -> uint32_t {
+            if constexpr (IN_LDS) return (uint32_t)srtl[t];
+            else return (uint32_t)srtg[t];
+        };
+        uint32_t Ccol[SEG];
 #pragma unroll
-        for (int m = 0; m < SEG; ++m) dst[m] = (j0 + m < n_de) ? Rrow[j0 + m] : 0u;
-    };
-#pragma unroll
-    for (int d = 0; d < PF; ++d) load_row(rb[d], min((uint32_t)d, maxrow_obs));
+        for (int m = 0; m < SEG; ++m) Ccol[m] = 0u;
 
-    // This lane's bin of row r when the row has 1..kDistSparseRow records,
-    // else 0. It is read one row ahead of its use, so that a row's bins in
-    // HBM arrive while the row before is summed.
-    auto sparse_bin = [&](uint32_t r) -> uint32_t {
-        if (r > last) return 0u;
-        const uint32_t a = (r == 0u) ? 0u : rowc[r - 1u], n = rowc[r] - a;
-        return (n > 0u && n <= (uint32_t)kDistSparseRow && lane < n) ? bin_at(a + lane) : 0u;
-    };
-    uint32_t next_bin = sparse_bin(0u);
-    uint64_t racc_prev = 0ull, racc_prev_hi = 0ull;
-    for (uint32_t i0 = 0; i0 <= last; i0 += PF) {
+        // Observed rows are loaded PF rows ahead of their use into a ring of
+        // register slots (slot d holds row i0 + d). Past the deepest observed
+        // row the table repeats that row, so a refill asks for row
+        // min(i + PF, maxrow_obs): no load leaves the table, and the rows past
+        // it are all served from one row in cache. Columns past n_de load the
+        // last column and are not summed.
+        constexpr int PF = 2;
+        uint32_t rb[PF][SEG];
+        auto load_row = [&](uint32_t (&dst)[SEG], uint32_t row) {
+            const auto* Rrow = R + (uint64_t)row * n_de;
 #pragma unroll
-        for (int d = 0; d < PF; ++d) {
-            const uint32_t i = i0 + (uint32_t)d;
-            if (i > last) break;
-            const uint32_t mine = next_bin;
-            next_bin = sparse_bin(i + 1u);
-            const uint32_t s0 = (i == 0u) ? 0u : rowc[i - 1u];
-            const uint32_t s1 = rowc[i];
-            // Past the deepest observed row R repeats its row; a row without
-            // records repeats S's row, so it contributes what the row above did.
-            if (s1 == s0 && i > maxrow_obs && i != last) {
-                add128(acc, acc_hi, racc_prev, racc_prev_hi);
+            for (int m = 0; m < SEG; ++m) dst[m] = Rrow[min(j0 + (uint32_t)m, n_de - 1u)];
+        };
+#pragma unroll
+        for (int d = 0; d < PF; ++d) load_row(rb[d], min((uint32_t)d, maxrow_obs));
+
+        // Row r's slots [s0, s1) and this lane's bin of it when the row has
+        // 1..kDistSparseRow records (else 0), read one row ahead of their use,
+        // so that a row's bins in HBM arrive while the row before is summed
+        // (read every row, at an index clamped into the slice).
+        struct RowAhead {
+            uint32_t bin, s0, s1;
+        };
+        auto row_ahead = [&](uint32_t r) -> RowAhead {
+            const uint32_t rr = min(r, last);
+            const uint32_t a = uniform((rr == 0u) ? 0u : rowc[rr - 1u]), e = uniform(rowc[rr]);
+            const uint32_t n = e - a;
+            const bool take = r <= last && n > 0u && n <= (uint32_t)kDistSparseRow && lane < n;
+            uint32_t b;
+            if constexpr (IN_LDS) {
+                b = take ? bin_at(a + lane) : 0u;
             } else {
-                const uint32_t nr = s1 - s0;
-                if (nr > 0u && nr <= (uint32_t)kDistSparseRow) {
-                    // A few records: every wave holds them (one per lane) and
-                    // each thread adds those at or left of its columns.
-                    for (uint32_t q = 0u; q < nr; ++q) {
-                        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)q);
+                const uint32_t v = bin_at(min(a + lane, T - 1u));
+                b = take ? v : 0u;
+            }
+            return RowAhead{b, a, e};
+        };
+        uint64_t racc_prev = 0ull, racc_prev_hi = 0ull;
+        // Row i with records in slots [s0, s1) (wave-uniform) and this lane's
+        // sparse bin: the records go into the column counts, then the row's
+        // sum against observed row values rv; rows past `last` repeat row
+        // `last` in both tables.
+        auto do_row = [&](uint32_t i, uint32_t s0, uint32_t s1, uint32_t mine, const uint32_t (&rv)[SEG]) {
+            const uint32_t nr = s1 - s0;
+            if (nr > 0u && nr <= (uint32_t)kDistSparseRow) {
+                // A few records: every wave holds them (one per lane) and
+                // each thread adds those at or left of its columns.
+                for (uint32_t q = 0u; q < nr; ++q) {
+                    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)q);
 #pragma unroll
-                        for (int m = 0; m < SEG; ++m) Ccol[m] += (j0 + (uint32_t)m >= b) ? 1u : 0u;
-                    }
-                } else if (nr > 0u) {
-                    for (uint32_t t = s0 + tid; t < s1; t += kDistThreads) atomicAdd(&colh[bin_at(t)], 1u);
-                    __syncthreads();
-                    uint32_t pre[SEG];
-                    uint32_t run = 0u;
-#pragma unroll
-                    for (int m = 0; m < SEG; ++m) {
-                        run += (j0 + m < n_de) ? colh[j0 + m] : 0u;
-                        pre[m] = run;
-                    }
-                    uint32_t tot;
-                    const uint32_t ex = block_excl_scan(run, wsum, tot);
-#pragma unroll
-                    for (int m = 0; m < SEG; ++m) Ccol[m] += ex + pre[m];
-#pragma unroll
-                    for (int m = 0; m < SEG; ++m)
-                        if (j0 + m < n_de) colh[j0 + m] = 0u;
-                    __syncthreads();
+                    for (int m = 0; m < SEG; ++m) Ccol[m] += (j0 + (uint32_t)m >= b) ? 1u : 0u;
                 }
-                uint64_t racc = 0ull, racc_hi = 0ull;
+            } else if (nr > 0u) {
+                for (uint32_t t = s0 + tid; t < s1; t += kDistThreads) atomicAdd(&colh[bin_at(t)], 1u);
+                __syncthreads();
+                uint32_t pre[SEG];
+                uint32_t run = 0u;
 #pragma unroll
                 for (int m = 0; m < SEG; ++m) {
-                    if (j0 + m < n_de) {
-                        const uint32_t rv = rb[d][m];
-                        const uint32_t dd = Ccol[m] > rv ? Ccol[m] - rv : rv - Ccol[m];
-                        add128(racc, racc_hi, l1 ? (uint64_t)dd : (uint64_t)dd * dd, 0ull);
-                    }
+                    run += (j0 + m < n_de) ? colh[j0 + m] : 0u;
+                    pre[m] = run;
                 }
-                // Rows past `last` repeat row `last` in both tables.
-                if (i == last) {
-                    const uint64_t rep = (uint64_t)(n_k - last);
-                    add128(acc, acc_hi, racc * rep, __umul64hi(racc, rep) + racc_hi * rep);
-                } else {
-                    add128(acc, acc_hi, racc, racc_hi);
-                }
-                racc_prev = racc;
-                racc_prev_hi = racc_hi;
+                uint32_t tot;
+                const uint32_t ex = block_excl_scan(run, wsum, tot);
+#pragma unroll
+                for (int m = 0; m < SEG; ++m) Ccol[m] += ex + pre[m];
+#pragma unroll
+                for (int m = 0; m < SEG; ++m)
+                    if (j0 + m < n_de) colh[j0 + m] = 0u;
+                __syncthreads();
             }
-            load_row(rb[d], min(i + (uint32_t)PF, maxrow_obs));
+            uint64_t racc = 0ull, racc_hi = 0ull;
+#pragma unroll
+            for (int m = 0; m < SEG; ++m) {
+                if (j0 + m < n_de) {
+                    const uint32_t dd = Ccol[m] > rv[m] ? Ccol[m] - rv[m] : rv[m] - Ccol[m];
+                    add128(racc, racc_hi, l1 ? (uint64_t)dd : (uint64_t)dd * dd, 0ull);
+                }
+            }
+            if (i == last) {
+                const uint64_t rep = (uint64_t)(n_k - last);
+                add128(acc, acc_hi, racc * rep, __umul64hi(racc, rep) + racc_hi * rep);
+            } else {
+                add128(acc, acc_hi, racc, racc_hi);
+            }
+            racc_prev = racc;
+            racc_prev_hi = racc_hi;
+        };
+
+        // Rows 0..min(last, maxrow_obs): the observed table changes from row
+        // to row, so every row is summed.
+        const uint32_t p1 = min(last, maxrow_obs);
+        RowAhead next = row_ahead(0u);
+        for (uint32_t i0 = 0; i0 <= p1; i0 += PF) {
+#pragma unroll
+            for (int d = 0; d < PF; ++d) {
+                const uint32_t i = i0 + (uint32_t)d;
+                if (i > p1) break;
+                const RowAhead cur = next;
+                next = row_ahead(i + 1u);
+                do_row(i, cur.s0, cur.s1, cur.bin, rb[d]);
+                // From the deepest observed row on, every slot already holds it.
+                if (i < maxrow_obs) load_row(rb[d], min(i + (uint32_t)PF, maxrow_obs));
+            }
         }
-    }
+        // Rows maxrow_obs + 1..last: the observed table repeats row maxrow_obs
+        // (every ring slot holds it), so a row without records adds what the
+        // row above it did. The rows with records are found 64 at a time by
+        // one ballot over the row offsets and summed; each run of rows
+        // without records between them adds the previous row's sum once per
+        // row. Row `last` holds records (it is the deepest record row).
+        if (last > maxrow_obs) {
+            uint32_t prev = maxrow_obs;  // the last row summed
+            for (uint32_t base = maxrow_obs + 1u; base <= last; base += (uint32_t)kWave) {
+                const uint32_t r = base + lane;
+                const uint32_t rr = min(r, last);
+                const uint32_t e = rowc[rr], bg = rowc[rr - 1u];
+                uint64_t mask = ballot(r <= last && e != bg);
+                while (mask != 0ull) {
+                    const uint32_t l = (uint32_t)__builtin_ctzll(mask);
+                    mask &= mask - 1ull;
+                    const uint32_t i = base + l;
+                    const uint32_t s0 = (uint32_t)__builtin_amdgcn_readlane((int)bg, (int)l);
+                    const uint32_t s1 = (uint32_t)__builtin_amdgcn_readlane((int)e, (int)l);
+                    const uint64_t gap = (uint64_t)(i - prev - 1u);
+                    add128(acc, acc_hi, racc_prev * gap, __umul64hi(racc_prev, gap) + racc_prev_hi * gap);
+                    const uint32_t nr = s1 - s0;
+                    uint32_t mine = 0u;
+                    if (nr <= (uint32_t)kDistSparseRow && lane < nr) mine = bin_at(s0 + lane);
+                    do_row(i, s0, s1, mine, rb[0]);
+                    prev = i;
+                }
+            }
+        }
+    };
+    if (in_lds) sweep(std::true_type{});
+    else sweep(std::false_type{});
 
     // 128-bit sum over the block: across each wave by shuffles, then the
     // waves' partial sums in LDS.
@@ -1061,8 +1138,12 @@ constexpr int kDistHeld = 32;
 // Dynamic LDS: row counts (n_k_max words, rounded up to 4), one row's column
 // histogram (n_de_max words), then srtl_cap 16-bit sorted bins; sized to the
 // models of the launch and to the occupancy the registers allow.
+// Occupancy targets per column-segment width: 8 waves per SIMD (64 VGPRs)
+// for histograms up to 512 columns and 6 for up to 1,024, which the register
+// allocation meets without spills; wider segments take what they need.
 template <int SEG>
-__global__ __launch_bounds__(kDistThreads) void distance_kernel(DistArgs a)
+__global__ __launch_bounds__(kDistThreads) __attribute__((amdgpu_waves_per_eu(SEG <= 2 ? 8 : (SEG == 4 ? 6 : 1))))
+void distance_kernel(DistArgs a)
 {
     extern __shared__ uint32_t dist_lds[];
     uint32_t* rowc = dist_lds;
@@ -1340,13 +1421,32 @@ __attribute__((noinline)) __device__ void walk_tile_coop_call(const DevProp* gp,
     }
 }
 
+// A request's distance in the serving loop, as a function of its own: its
+// row sweep keeps its registers (inlined, the loop's live state pushed the
+// sweep into spill reloads on every row). The working set is rebuilt here
+// from the dynamic LDS base, so that its accesses stay LDS accesses; the
+// proposal and the model are read from their HBM copies.
 template <int SEG>
-__global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per_eu(6, 6))) void serve_loop_kernel(LoopArgs a)
+__attribute__((noinline)) __device__ DistOut serve_distance(const DevProp* gp, const DevSys* gs, uint32_t count,
+                                                            uint32_t rowc_off, uint32_t nk_lds, uint32_t nde_lds,
+                                                            uint32_t srtl_cap)
 {
     extern __shared__ uint32_t loop_lds[];
     __shared__ uint32_t wsum[kDistThreads / kWave];
     __shared__ uint32_t smax;
     __shared__ uint64_t red[kDistRed];
+    uint32_t* rowc = loop_lds + rowc_off;
+    const DistLds L{rowc, rowc + nk_lds, wsum, &smax, red, reinterpret_cast<uint16_t*>(rowc + nk_lds + nde_lds),
+                    srtl_cap};
+    const DevProp pr = load_uniform(gp);
+    const DevSys sy = load_uniform(gs);
+    return distance_block<SEG, 0>(pr, sy, count, L);
+}
+
+template <int SEG>
+__global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per_eu(6, 6))) void serve_loop_kernel(LoopArgs a)
+{
+    extern __shared__ uint32_t loop_lds[];
     __shared__ uint64_t cmd_r;
     __shared__ uint32_t cmd[6];  // kind, t0, nt, last, sleep units, coop
 
@@ -1364,9 +1464,8 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
     load_angle_table(tabf);
     __syncthreads();
     const float2* tab = reinterpret_cast<const float2*>(tabf);
-    uint32_t* rowc = loop_lds + kLoopWaves * a.stage_cap + kAngleTabFloats;
-    const DistLds L{rowc, rowc + a.nk_lds, wsum, &smax, red, reinterpret_cast<uint16_t*>(rowc + a.nk_lds + a.nde_lds),
-                    a.srtl_cap};
+    const uint32_t rowc_off = kLoopWaves * a.stage_cap + kAngleTabFloats;
+    uint32_t* rowc = loop_lds + rowc_off;  // the distance's LDS, also the group walk's exchange buffers
     const uint32_t vw = 3u * (a.maxv + 1u);
     // Polling: one idle group per CU holds the CU's poller token and polls
     // every sleep unit (~3.4 us); the CU's other idle groups back off from 1
@@ -1481,10 +1580,10 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
         __syncthreads();
         if (uniform(cmd[3]) != 0u) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // records of every group
-            const DevProp pr = load_uniform(&S.prop);
-            const DevSys sy = load_uniform(&a.sys[pr.sys_id]);
+            const uint32_t sys_id = uniform(S.prop.sys_id);
             const uint32_t count = uniform(ld_agent32(&S.rec_count));
-            const DistOut o = distance_block<SEG, 0>(pr, sy, count, L);
+            const DistOut o =
+                serve_distance<SEG>(&S.prop, &a.sys[sys_id], count, rowc_off, a.nk_lds, a.nde_lds, a.srtl_cap);
             if (tid == 0u) {
                 LoopDone& d = a.done[slot];
                 d.exact = dist_exact(o);

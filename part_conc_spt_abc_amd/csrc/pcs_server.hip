@@ -396,11 +396,18 @@ void retire_group(pcs_server* s, Group& g)
     s->cv.notify_all();
 }
 
+// A scheduler thread keeps polling (yielding its core to other threads) for
+// this long after its last work before it sleeps on the condition variable:
+// a request arriving meanwhile is published at once instead of after the
+// thread's wake-up (~10 us).
+constexpr auto kSchedSpin = std::chrono::microseconds(2000);
+
 void scheduler(pcs_server* s)
 {
     hipSetDevice(s->device);
     std::vector<Pending> batch;
     std::vector<uint64_t> offs;
+    auto last_work = std::chrono::steady_clock::now();
     for (;;) {
         bool idle = true;
         // Retire finished groups.
@@ -464,10 +471,12 @@ void scheduler(pcs_server* s)
             for (Group& g : s->groups) any_busy |= g.busy;
             std::unique_lock<std::mutex> lk(s->mu);
             if (s->stop) return;
-            if (!any_busy && s->queue.empty())
+            if (!any_busy && s->queue.empty() && std::chrono::steady_clock::now() - last_work > kSchedSpin)
                 s->cv.wait_for(lk, std::chrono::milliseconds(2));
             else
                 lk.unlock(), std::this_thread::yield();
+        } else {
+            last_work = std::chrono::steady_clock::now();
         }
     }
 }
@@ -607,10 +616,16 @@ void scheduler_loop(pcs_server* s)
     hipSetDevice(s->device);
     LoopHost& L = s->loop;
     std::vector<pcs_completion> out;
+    auto last_work = std::chrono::steady_clock::now();
+    auto last_query = last_work;
     for (;;) {
         bool progress = loop_collect(s, out) > 0;
         // Has the kernel left (idle limit, or the stop flag of a restart)?
-        if (L.running) {
+        // Asked every 100 us at most: the results above are read from host
+        // memory, and a HIP call per pass would delay noticing them.
+        const auto now = std::chrono::steady_clock::now();
+        if (L.running && (L.stopping || now - last_query >= std::chrono::microseconds(100))) {
+            last_query = now;
             const hipError_t q = hipEventQuery(L.ev);
             if (q == hipSuccess) {
                 L.running = false;
@@ -674,10 +689,12 @@ void scheduler_loop(pcs_server* s)
         if (!progress) {
             std::unique_lock<std::mutex> lk(s->mu);
             if (s->stop) return;
-            if (L.live.empty() && s->queue.empty())
+            if (L.live.empty() && s->queue.empty() && std::chrono::steady_clock::now() - last_work > kSchedSpin)
                 s->cv.wait_for(lk, std::chrono::milliseconds(2));
             else
                 lk.unlock(), std::this_thread::yield();
+        } else {
+            last_work = std::chrono::steady_clock::now();
         }
     }
 }
