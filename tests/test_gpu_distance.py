@@ -111,3 +111,48 @@ def test_serving_distance_over_record_counts(gpu_lib, device_loop, one_wave):
     srv.close()
     for i in range(len(props)):
         assert got[i] == (float(dist[i]), int(tracks[i]), int(exact[i])), i
+
+
+@pytest.mark.parametrize("obs_kind", ["empty", "row3"])
+def test_distance_past_the_deepest_observed_row(gpu_lib, obs_kind):
+    """Past the deepest observed row the sweep visits only the rows holding
+    records (found 64 rows at a time) and adds each run of empty rows as a
+    multiple of the row above. An empty observed histogram (every row is past
+    it) and one with tracks only in row 3 put every kind of row there: dense
+    rows (more than 32 records) of the large simulations, sparse rows, and
+    gaps of every length; batched and in the request loop."""
+    model = cases.headline_model()
+    obs = np.zeros((model.n_k, model.n_de), np.uint32)
+    if obs_kind == "row3":
+        obs[3, :40] = np.arange(1, 41, dtype=np.uint32)
+    props, f, n = sweep(model, n_sims=32)
+    with Context(model, obs) as ctx:
+        b = ctx.batch()
+        b.load(props, f, n)
+        b.run(47, _abi.PCS_MODE_BATCHED)
+        dist, tracks, H = b.read(hist=True)
+        exact = b.read_exact().copy()
+        b.close()
+    dense = 0
+    for i in range(len(props)):
+        ex, _ = O.distance(H[i], obs, model.dist_kind)
+        assert int(exact[i]) == ex, f"simulation {i} ({tracks[i]} records): {int(exact[i])} != {ex}"
+        dense += int((H[i].sum(axis=1)[4:] > 32).sum())
+    print(f"{obs_kind}: records {tracks.min()}..{tracks.max()}, dense rows past row 3: {dense}", flush=True)
+    assert dense > 0
+    srv = Server(device=0, arena_bytes=2 << 30, max_inflight=64, device_loop=True)
+    sid = srv.add_system(model, obs)
+    for i, p in enumerate(props):
+        q = _abi.pcs_proposal.from_buffer_copy(p)
+        q.first_video = 0
+        assert srv.submit(i, sid, q, f[i:i + 1], n[i:i + 1], 47)
+    got = {}
+    t0 = time.time()
+    while len(got) < len(props):
+        for c in srv.poll(256, wait_us=2000):
+            assert c.status == 0
+            got[c.tag] = (int(c.tracks), int(c.exact))
+        assert time.time() - t0 < 120.0, f"only {len(got)} of {len(props)} completions"
+    srv.close()
+    for i in range(len(props)):
+        assert got[i] == (int(tracks[i]), int(exact[i])), i
