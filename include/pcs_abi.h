@@ -348,7 +348,9 @@ int pcs_pmc_weights(int device, const double* theta_new, uint32_t n_new, const d
  *                          requests have a wave walk each tile.
  *   PCS_SERVE_LOOP_ONE_WAVE_TILES  with PCS_SERVE_DEVICE_LOOP: every tile is
  *                          walked by one wave, as in a large request.
- * All give the same result for the same request. */
+ * All give the same result for the same request. In either form the host
+ * thread keeps polling (yielding its core) for 2 ms after its last work before
+ * it sleeps, so that a request arriving meanwhile is published at once. */
 #define PCS_SERVE_DEVICE_LOOP 1u
 #define PCS_SERVE_LOOP_ONE_WAVE_TILES 2u
 
