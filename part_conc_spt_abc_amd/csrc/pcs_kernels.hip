@@ -1245,27 +1245,40 @@ __device__ bool loop_admit(const LoopArgs& a, uint64_t q)
     LoopSlot& S = a.slots[slot];
     const bool l0 = lane_id() == 0u;
     const uint64_t expect = q < a.R ? 0ull : 4ull * (q - a.R) + 3ull;
-    if (bcast64(l0 ? ld_agent64(&S.state) : 0ull) != expect) return false;
     const LoopReq& Q = a.ring[slot];
-    if (bcast64(l0 ? ld_system64(&Q.seq) : 0ull) != q + 1ull) return false;
+    // The slot's state and the ring entry's sequence number, both loads in
+    // flight at once: an idle poll costs one round trip to host memory.
+    uint64_t st = 0ull, sq = 0ull;
+    if (l0) {
+        st = ld_agent64(&S.state);
+        sq = ld_system64(&Q.seq);
+    }
+    if (bcast64(st) != expect || bcast64(sq) != q + 1ull) return false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the ring entry was written before its seq
     if (l0) st_agent64(&S.state, 4ull * q + 1ull);
-    // One round of host reads: the proposal (36 words) and n_videos, tiles.
+    // One round of host reads: the proposal (36 words), n_videos, tiles and,
+    // in the wave's other lanes, the first words of the video table (all of
+    // it for up to 7 videos); a longer table is read in a second round.
     constexpr uint32_t kPropWords = sizeof(DevProp) / 4u;
+    constexpr uint32_t kVid0 = kPropWords + 2u;  // first lane holding a video-table word
+    static_assert(kVid0 < (uint32_t)kWave, "proposal words and counts fit one wave");
     const uint32_t* src = reinterpret_cast<const uint32_t*>(&Q.prop);
     uint32_t* dst = reinterpret_cast<uint32_t*>(&S.prop);
     const uint32_t lane = lane_id();
+    const uint32_t vcap = 3u * (a.maxv + 1u);  // words of a slot's video table
+    const uint32_t* vs = a.ring_vid + (uint64_t)slot * vcap;
+    uint32_t* vd = a.slot_vid + (uint64_t)slot * vcap;
     uint32_t w = 0u;
     if (lane < kPropWords) w = src[lane];
     else if (lane == kPropWords) w = Q.n_videos;
     else if (lane == kPropWords + 1u) w = Q.tiles;
+    else if (lane - kVid0 < vcap) w = vs[lane - kVid0];
     if (lane < kPropWords) dst[lane] = w;
     const uint32_t nv = min(__shfl(w, (int)kPropWords, kWave), a.maxv);
     const uint32_t tiles = __shfl(w, (int)kPropWords + 1, kWave);
     const uint32_t vw = 3u * (nv + 1u);
-    const uint32_t* vs = a.ring_vid + (uint64_t)slot * 3u * (a.maxv + 1u);
-    uint32_t* vd = a.slot_vid + (uint64_t)slot * 3u * (a.maxv + 1u);
-    for (uint32_t i = lane; i < vw; i += kWave) vd[i] = vs[i];
+    if (lane >= kVid0 && lane - kVid0 < vw) vd[lane - kVid0] = w;
+    for (uint32_t i = (uint32_t)kWave - kVid0 + lane; i < vw; i += kWave) vd[i] = vs[i];
     // A request of few tiles admitted onto a quiet loop has each tile walked by
     // a whole group: a lone wave's walk is bound by its own instruction
     // stream, and the group's walk issues more instructions per tile, which a
