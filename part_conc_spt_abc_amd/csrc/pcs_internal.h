@@ -139,6 +139,14 @@ struct LoopReq {
 };
 static_assert(sizeof(LoopReq) == 160, "LoopReq layout");
 
+// PCS_TRACE = 1 (the tracing build, `make trace`): every request's completion
+// carries the device wall clock (100 MHz) at its admission, its first tile
+// claim, and the start and end of its distance, and the host prints one line
+// per completed request to stderr (see latency_anatomy.py).
+#ifndef PCS_TRACE
+#define PCS_TRACE 0
+#endif
+
 struct LoopDone {
     uint64_t seq;
     double dist;
@@ -147,8 +155,11 @@ struct LoopDone {
     int32_t status;
     uint32_t group_walked;  // the request's tiles were claimed a tile per group
     uint32_t pad;
+#if PCS_TRACE
+    uint64_t t_admit, t_claim, t_dist, t_done;
+#endif
 };
-static_assert(sizeof(LoopDone) == 40, "LoopDone layout");
+static_assert(sizeof(LoopDone) == (PCS_TRACE ? 72 : 40), "LoopDone layout");
 
 struct LoopSlot {
     uint64_t state;
@@ -156,8 +167,11 @@ struct LoopSlot {
     uint32_t tiles_done, rec_count, n_videos, pad;
     uint64_t claim;  // (r mod 2^32) << 32 | next unclaimed tile
     DevProp prop;
+#if PCS_TRACE
+    uint64_t t_admit, t_claim;
+#endif
 };
-static_assert(sizeof(LoopSlot) == 184, "LoopSlot layout");
+static_assert(sizeof(LoopSlot) == (PCS_TRACE ? 200 : 184), "LoopSlot layout");
 
 // CU index of a group: XCC id << 8 | SE/SH/CU id of HW_ID, below 2048.
 constexpr uint32_t kLoopCuSlots = 2048;

@@ -764,15 +764,26 @@ constexpr int kDistSparseRow = 32;
 // Record loads each thread keeps in flight while reading a slice.
 constexpr int kDistLoads = 8;
 
+// Inclusive sum over the wave by data-parallel-primitive moves (VALU, no
+// LDS round trip): shifts by 1, 2, 4, 8 lanes within each 16-lane row (a lane
+// without a source adds 0), then lane 15 of rows 0 and 2 added to rows 1 and
+// 3, then lane 31 added to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return (uint32_t)x;
+}
+
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total)
 {
     const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, kWave);
-        if (lane >= (uint32_t)o) x += y;
-    }
+    const uint32_t x = wave_incl_scan(v);
     if (lane == kWave - 1) wsum[wave] = x;
     __syncthreads();
     uint32_t base = 0u, tot = 0u;
@@ -889,14 +900,17 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
             mr = max(mr, r);
         }
     }
-    // Past the held records: each thread keeps kDistLoads loads in flight
-    // (unmasked, at indices clamped into the slice).
-    for (uint32_t t0 = nheld + tid; t0 < T; t0 += kDistLoads * kDistThreads) {
-        uint32_t v[kDistLoads];
+    // Past the held records: each thread keeps kLoads loads in flight
+    // (unmasked, at indices clamped into the slice); with no records held in
+    // registers (a serving request's distance, alone on the GPU) 12, which
+    // still fits its registers without spills.
+    constexpr int kLoads = KREG == 0 ? 12 : kDistLoads;
+    for (uint32_t t0 = nheld + tid; t0 < T; t0 += kLoads * kDistThreads) {
+        uint32_t v[kLoads];
 #pragma unroll
-        for (int u = 0; u < kDistLoads; ++u) v[u] = rec[min(t0 + (uint32_t)u * kDistThreads, T - 1u)];
+        for (int u = 0; u < kLoads; ++u) v[u] = rec[min(t0 + (uint32_t)u * kDistThreads, T - 1u)];
 #pragma unroll
-        for (int u = 0; u < kDistLoads; ++u) {
+        for (int u = 0; u < kLoads; ++u) {
             if (t0 + (uint32_t)u * kDistThreads < T) {
                 const uint32_t r = v[u] >> 16;
                 atomicAdd(&rowc[r], 1u);
@@ -938,12 +952,12 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
         asm volatile("" : "+v"(v));
         if (v != kNoRecord) put(v);
     }
-    for (uint32_t t0 = nheld + tid; t0 < T; t0 += kDistLoads * kDistThreads) {
-        uint32_t v[kDistLoads];
+    for (uint32_t t0 = nheld + tid; t0 < T; t0 += kLoads * kDistThreads) {
+        uint32_t v[kLoads];
 #pragma unroll
-        for (int u = 0; u < kDistLoads; ++u) v[u] = rec[min(t0 + (uint32_t)u * kDistThreads, T - 1u)];
+        for (int u = 0; u < kLoads; ++u) v[u] = rec[min(t0 + (uint32_t)u * kDistThreads, T - 1u)];
 #pragma unroll
-        for (int u = 0; u < kDistLoads; ++u)
+        for (int u = 0; u < kLoads; ++u)
             if (t0 + (uint32_t)u * kDistThreads < T) put(v[u]);
     }
     __threadfence_block();
@@ -1025,7 +1039,16 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
                     for (int m = 0; m < SEG; ++m) Ccol[m] += (j0 + (uint32_t)m >= b) ? 1u : 0u;
                 }
             } else if (nr > 0u) {
-                for (uint32_t t = s0 + tid; t < s1; t += kDistThreads) atomicAdd(&colh[bin_at(t)], 1u);
+                // four bins per thread read (at indices clamped into the row)
+                // before their atomics
+                for (uint32_t t = s0 + tid; t < s1; t += 4u * kDistThreads) {
+                    uint32_t b[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) b[u] = bin_at(min(t + (uint32_t)u * kDistThreads, s1 - 1u));
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (t + (uint32_t)u * kDistThreads < s1) atomicAdd(&colh[b[u]], 1u);
+                }
                 __syncthreads();
                 uint32_t pre[SEG];
                 uint32_t run = 0u;
@@ -1287,6 +1310,10 @@ __device__ bool loop_admit(const LoopArgs& a, uint64_t q)
     if (l0 && a.coop_max_tiles != 0u && tiles <= a.coop_max_tiles)
         coop = q - ld_agent64(&a.ctl->finished) <= (uint64_t)a.coop_max_open ? 1u : 0u;
     if (l0) {
+#if PCS_TRACE
+        S.t_admit = wall_clock64();
+        S.t_claim = 0ull;
+#endif
         S.tiles = tiles;
         S.coop = coop;
         S.n_videos = nv;
@@ -1558,6 +1585,9 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
         const bool coop = uniform(cmd[5]) != 0u;
         const uint32_t slot = (uint32_t)(r % a.R);
         LoopSlot& S = a.slots[slot];
+#if PCS_TRACE
+        if (tid == 0u && t0 == 0u) S.t_claim = wall_clock64();
+#endif
         if (coop || wave < nt) {
             const uint32_t t = t0 + (coop ? 0u : wave);
             const uint32_t* vid = a.slot_vid + (uint64_t)slot * vw;
@@ -1595,10 +1625,19 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // records of every group
             const uint32_t sys_id = uniform(S.prop.sys_id);
             const uint32_t count = uniform(ld_agent32(&S.rec_count));
+#if PCS_TRACE
+            const uint64_t t_dist = wall_clock64();
+#endif
             const DistOut o =
                 serve_distance<SEG>(&S.prop, &a.sys[sys_id], count, rowc_off, a.nk_lds, a.nde_lds, a.srtl_cap);
             if (tid == 0u) {
                 LoopDone& d = a.done[slot];
+#if PCS_TRACE
+                d.t_admit = S.t_admit;
+                d.t_claim = S.t_claim;
+                d.t_dist = t_dist;
+                d.t_done = wall_clock64();
+#endif
                 d.exact = dist_exact(o);
                 d.dist = dist_log10(o);
                 d.tracks = o.tracks;

@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -152,6 +153,9 @@ struct LoopHost {
     struct Book {
         bool busy = false;
         uint64_t r = 0, off = 0;
+#if PCS_TRACE
+        double publish_us = 0;
+#endif
         Pending req;  // kept for an exact rerun of an overflowed record slice
     };
     std::vector<Book> book;         // per slot
@@ -540,6 +544,14 @@ uint32_t loop_collect(pcs_server* s, std::vector<pcs_completion>& out)
         c.status = d.status;
         c.submit_us = b.req.submit_us;
         c.done_us = s->now_us();
+#if PCS_TRACE
+        // host clock (us): submit, publish to the ring, result seen; device
+        // wall clock (ticks of 10 ns): admitted, first claim, distance, done
+        std::fprintf(stderr, "pcs-trace %llu %.3f %.3f %.3f %llu %llu %llu %llu %u\n", (unsigned long long)c.tag,
+                     c.submit_us, b.publish_us, c.done_us, (unsigned long long)d.t_admit,
+                     (unsigned long long)d.t_claim, (unsigned long long)d.t_dist, (unsigned long long)d.t_done,
+                     d.group_walked);
+#endif
         out.push_back(c);
         n_group += d.group_walked != 0u ? 1u : 0u;
     }
@@ -667,6 +679,9 @@ void scheduler_loop(pcs_server* s)
                 b.r = L.next_r;
                 b.off = off;
                 loop_publish(s, slot, L.next_r, p, off);
+#if PCS_TRACE
+                b.publish_us = s->now_us();
+#endif
                 b.req = std::move(p);
                 L.live.push_back(slot);
                 ++L.next_r;
