@@ -28,6 +28,18 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(_abi.PROTOTYPES), set(names) ^ set(_abi.PROTOTYPES)
 
 
+def test_tracing_build_exports_the_same_entry_points():
+    """lib/trace/libpcs.so (make trace) is a drop-in for the default library."""
+    path = ROOT / "part_conc_spt_abc_amd" / "lib" / "trace" / "libpcs.so"
+    if not path.exists():
+        import pytest
+        pytest.skip("the tracing build was not made")
+    lib = C.CDLL(str(path))
+    for n in declared():
+        assert hasattr(lib, n), n
+    assert lib.pcs_abi_version() == 1
+
+
 def test_struct_layouts_match_header(tmp_path):
     """sizeof/offsetof of every ABI struct from the C compiler vs ctypes."""
     import subprocess
