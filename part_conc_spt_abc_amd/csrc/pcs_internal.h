@@ -141,7 +141,8 @@ static_assert(sizeof(LoopReq) == 160, "LoopReq layout");
 
 // PCS_TRACE = 1 (the tracing build, `make trace`): every request's completion
 // carries the device wall clock (100 MHz) at its admission, its first tile
-// claim, and the start and end of its distance, and the host prints one line
+// claim, the start of its distance, the start of the distance's row sweep
+// (after the record passes) and the end, and the host prints one line
 // per completed request to stderr (see latency_anatomy.py).
 #ifndef PCS_TRACE
 #define PCS_TRACE 0
@@ -156,10 +157,10 @@ struct LoopDone {
     uint32_t group_walked;  // the request's tiles were claimed a tile per group
     uint32_t pad;
 #if PCS_TRACE
-    uint64_t t_admit, t_claim, t_dist, t_done;
+    uint64_t t_admit, t_claim, t_dist, t_sweep, t_done;
 #endif
 };
-static_assert(sizeof(LoopDone) == (PCS_TRACE ? 72 : 40), "LoopDone layout");
+static_assert(sizeof(LoopDone) == (PCS_TRACE ? 80 : 40), "LoopDone layout");
 
 struct LoopSlot {
     uint64_t state;

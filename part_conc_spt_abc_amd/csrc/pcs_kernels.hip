@@ -70,6 +70,16 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gview(T* p)
     return (__attribute__((address_space(1))) T*)p;
 }
 
+// Pointers typed in the LDS (3) and global (1) address spaces, for the
+// arguments of a function of its own: a generic pointer argument compiles to
+// flat accesses (which take the vector-memory path to LDS as well and are
+// waited for on both counters); one converted from a typed argument inside
+// the function compiles to LDS or global instructions, also in the functions
+// inlined into it.
+typedef const __attribute__((address_space(3))) float2* lds_cf2;
+typedef __attribute__((address_space(3))) uint32_t* lds_u32;
+typedef __attribute__((address_space(1))) uint32_t* glb_u32;
+
 // Orders this wave's LDS accesses across lanes (staging buffer handoff).
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -247,9 +257,10 @@ __device__ __forceinline__ void stage_flush(Stage& st)
     uint32_t base = 0u;
     if (lane_id() == 0u) base = atomicAdd(st.counter, st.cnt);
     base = uniform(base);
+    auto* rec = gview(st.rec);  // record slices are in HBM
     for (uint32_t i = lane_id(); i < st.cnt; i += kWave) {
         const uint32_t idx = base + i;
-        if (idx < st.slice_cap) st.rec[idx] = st.buf[i];
+        if (idx < st.slice_cap) rec[idx] = st.buf[i];
     }
     wave_lds_sync();
     st.cnt = 0u;
@@ -499,6 +510,51 @@ __device__ __forceinline__ void step_particle(Particle& q, uint64_t m_active, ui
     m_prev = m_det;
 }
 
+// A full 4-step chunk of particle q, for a wave alone on its SIMD: the four
+// steps' moves and detection masks first (they do not depend on the track
+// state), then each step's track logic as step_particle runs it. Every
+// value and the order of the staged records are step_particle's; the moves
+// and comparisons of later steps no longer wait behind each step's branch.
+template <bool GAUSS, bool ZREF>
+__device__ __forceinline__ void step_chunk4(Particle& q, uint64_t m_active, uint64_t& m_prev, const float (&d)[12],
+                                            const float (&r2)[4], const uint32_t (&wdet)[4], const WalkConsts& K,
+                                            Stage& st)
+{
+    constexpr bool ZBOX = !ZREF;
+    uint64_t m_det[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        q.x = fract_box(__builtin_fmaf(q.sx, d[2 * s], q.x));
+        q.y = fract_box(__builtin_fmaf(q.sy, d[2 * s + 1], q.y));
+        if (ZBOX) {
+            q.z = fract_box(__builtin_fmaf(q.sz, d[8 + s], q.z));
+        } else {
+            const float zn = __builtin_fmaf(q.sz, d[8 + s], q.z);
+            q.z = wrap_reflect(zn, K.hz);
+        }
+        const bool cx = q.x <= K.wx, cy = q.y <= K.wy;
+        const bool cz = GAUSS ? gauss_detect(ZBOX ? q.z - 0.5f : q.z, K.cg, wdet[s])
+                              : (ZBOX ? q.z <= q.wz : __builtin_fabsf(q.z) <= q.wz);
+        m_det[s] = ballot(cx) & ballot(cy) & ballot(cz);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int32_t f1 = s + 1;
+        const uint64_t m_end = m_active & m_prev & ~m_det[s];
+        if (m_end != 0ull) {
+            const uint64_t m_closing = m_end & ballot(q.ts <= f1 - (int32_t)K.kmin);
+            if (m_closing != 0ull) {
+                const uint32_t k = (uint32_t)(f1 - q.ts);
+                stage_push2<true>(st, m_closing, k, __float_as_uint(track_de(q)), K.bs);
+            }
+        }
+        const float de_n = q.de + r2[s];
+        q.de = __builtin_amdgcn_inverse_ballot_w64(m_det[s] & m_prev) ? de_n : 0.0f;
+        q.ts = __builtin_amdgcn_inverse_ballot_w64(m_det[s] & ~m_prev) ? f1 : q.ts;
+        m_prev = m_det[s];
+    }
+}
+
 // Tracks still open in the last frame (src/cpu/simulate.jl:114-121), which is
 // frame 0 after the last chunk's rebase: k = 1 - ts.
 template <bool TRACKS>
@@ -604,16 +660,19 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
 
 // Cooperative walk of one tile by the four waves of a serving group, for a
 // request's latency: a lone wave issues one instruction per few cycles, so
-// its tile takes as long as its whole instruction stream. Waves 0..2 produce
-// each 4-step chunk's unit normals - wave w the Box-Muller pairs 2w and 2w + 1
-// from the Philox blocks they need, wave 2 also the Gaussian-detection words -
-// into an LDS double buffer; wave 3 holds the particles, steps them and
-// stages their tracks. Round i produces chunk i into buffer i % 2 while chunk
-// i - 1 is stepped from the other; the group's barrier after each round hands
-// the buffers over. The values and their order are walk_tile's, so the
-// results are the same. xbuf: 2 * kCoopWords * 64 words (lane-minor, so a
-// wave's accesses to one word hit 64 consecutive words).
-constexpr int kCoopWords = 20;  // per lane and chunk: d[12], r^2 of pairs 0..3, detection words 0..3
+// its tile takes as long as its whole instruction stream. The chunk's work is
+// pipelined over three stages with one group barrier per round: in round i,
+// waves 0 and 1 run Philox blocks 1 and 2 of chunk i (one each, the 64-bit
+// multiplies that dominate a block) into an LDS word buffer; waves 0..2 turn
+// chunk i - 1's words into its unit normals - wave w the Box-Muller pairs 2w
+// and 2w + 1, wave 2 also the Gaussian-detection words - into a normal
+// buffer; wave 3 holds the particles, steps chunk i - 2 and stages its tracks.
+// Both buffers are double (chunk parity). The values and their order are
+// walk_tile's, so the results are the same. xbuf: kCoopBufWords words
+// (lane-minor, so a wave's accesses to one word hit 64 consecutive words).
+constexpr int kCoopWords = 20;     // normals per lane and chunk: d[12], r^2 of pairs 0..3, detection words 0..3
+constexpr int kCoopRawWords = 8;   // Philox words per lane and chunk: blocks 1 and 2
+constexpr int kCoopBufWords = 2 * (kCoopWords + kCoopRawWords) * kWave;
 
 template <bool GAUSS, bool ZREF>
 __device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& sy, uint32_t F, uint32_t N,
@@ -630,6 +689,7 @@ __device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& 
     const uint32_t id = p0 + lane;
     const uint32_t steps = F > 0u ? F - 1u : 0u;
     const uint32_t nc = (steps + 3u) / 4u;
+    uint32_t* raw = xbuf + 2 * kCoopWords * kWave;
 
     Particle q;
     uint64_t m_active = 0ull, m_prev = 0ull;
@@ -639,23 +699,31 @@ __device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& 
         m_active = ballot(q.active);
         m_prev = ballot(q.det0);
     }
-    for (uint32_t it = 0; it <= nc; ++it) {
-        if (wave < 3u && it < nc) {
-            const uint32_t c = it;
-            float* xb = reinterpret_cast<float*>(xbuf) + (c & 1u) * (uint32_t)(kCoopWords * kWave) + lane;
-            const U4 b1 = philox10(B * c + 2u, id, video, sim, k0, k1);
-            uint32_t wr0, wr1, wa;
-            if (wave < 2u) {
-                const U4 b0 = philox10(B * c + 1u, id, video, sim, k0, k1);
-                wr0 = wave == 0u ? b0.x : b0.z;
-                wr1 = wave == 0u ? b0.y : b0.w;
-                wa = b1.z;
-            } else {
-                wr0 = b1.x;
-                wr1 = b1.y;
-                wa = b1.w;
-            }
+    for (uint32_t it = 0; it <= nc + 1u; ++it) {
+        if (wave < 3u) {
+            // Pair p's radius is word p of blocks 1, 2 (words 0..7), its angle
+            // word 6 + p / 4. Every stage runs in every round, so that the
+            // compiler can interleave this wave's Philox block with its
+            // Box-Muller pairs: in round 0 the pairs take words no round
+            // wrote and write normals to the half that chunk 1 overwrites in
+            // round 2, before the stepping wave first reads it in round 3; in
+            // rounds nc and nc + 1 the blocks and pairs are those of chunks
+            // past the last, written to halves whose chunks were read in the
+            // round before. Any word gives finite normals and table indices
+            // below 256.
+            const uint32_t c = it - 1u;  // the chunk whose normals this round makes
             const int pa = 2 * (int)wave;
+            const uint32_t* rr = raw + (c & 1u) * (uint32_t)(kCoopRawWords * kWave) + lane;
+            const uint32_t wr0 = rr[pa * kWave], wr1 = rr[(pa + 1) * kWave], wa = rr[(6 + pa / 4) * kWave];
+            if (wave < 2u) {
+                const U4 b = philox10(B * it + 1u + wave, id, video, sim, k0, k1);
+                uint32_t* rw = raw + (it & 1u) * (uint32_t)(kCoopRawWords * kWave) + 4u * wave * kWave + lane;
+                rw[0] = b.x;
+                rw[kWave] = b.y;
+                rw[2 * kWave] = b.z;
+                rw[3 * kWave] = b.w;
+            }
+            float* xb = reinterpret_cast<float*>(xbuf) + (c & 1u) * (uint32_t)(kCoopWords * kWave) + lane;
             float d0, d1, y0, d2, d3, y1;
             bm_pair(wr0, wa, pa, tab, d0, d1, y0);
             bm_pair(wr1, wa, pa + 1, tab, d2, d3, y1);
@@ -674,8 +742,8 @@ __device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& 
                 xb[18 * kWave] = __uint_as_float(b2.z);
                 xb[19 * kWave] = __uint_as_float(b2.w);
             }
-        } else if (wave == 3u && it > 0u) {
-            const uint32_t c = it - 1u;
+        } else if (it >= 2u) {
+            const uint32_t c = it - 2u;
             const uint32_t nst = min(4u, steps - 4u * c);
             const float* xb = reinterpret_cast<const float*>(xbuf) + (c & 1u) * (uint32_t)(kCoopWords * kWave) + lane;
             float d[12], r2[4];
@@ -687,11 +755,15 @@ __device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& 
                 r2[k] = xb[(12 + k) * kWave];
                 wdet[k] = GAUSS ? __float_as_uint(xb[(16 + k) * kWave]) : 0u;
             }
+            if (nst == 4u) {
+                step_chunk4<GAUSS, ZREF>(q, m_active, m_prev, d, r2, wdet, K, st);
+            } else {
 #pragma unroll
-            for (uint32_t s = 0; s < 4u; ++s) {
-                if (s >= nst) break;
-                step_particle<GAUSS, ZREF, false>(q, m_active, m_prev, d[2 * s], d[2 * s + 1], d[8 + s], r2[s],
-                                                  wdet[s], s, K, st);
+                for (uint32_t s = 0; s < 4u; ++s) {
+                    if (s >= nst) break;
+                    step_particle<GAUSS, ZREF, false>(q, m_active, m_prev, d[2 * s], d[2 * s + 1], d[8 + s], r2[s],
+                                                      wdet[s], s, K, st);
+                }
             }
             q.ts -= (int32_t)nst;  // the chunk's last frame becomes frame 0
         }
@@ -761,6 +833,8 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a)
 constexpr int kDistThreads = 256;
 // Rows with at most this many records skip the row histogram and block scan.
 constexpr int kDistSparseRow = 32;
+
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
 // Record loads each thread keeps in flight while reading a slice.
 constexpr int kDistLoads = 8;
 
@@ -820,6 +894,9 @@ struct DistOut {
     uint64_t lo, hi;
     uint32_t tracks;
     int32_t status;
+#if PCS_TRACE
+    uint64_t t_sweep;  // device wall clock when the row sweep began
+#endif
 };
 
 // 128-bit accumulation of the distance sum (it passes 2^64 when cumulative
@@ -962,13 +1039,16 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     }
     __threadfence_block();
     __syncthreads();
-    const uint32_t last = max(*L.smax, sy.maxrow_obs);  // < n_k
+    const uint32_t last = uniform(max(*L.smax, sy.maxrow_obs));  // < n_k
     const uint32_t j0 = tid * SEG;
     uint64_t acc = 0ull, acc_hi = 0ull;
     const bool l1 = sy.dist_kind == PCS_DIST_L1;
     const auto* R = gview(sy.obs_sat);
     const uint32_t maxrow_obs = sy.maxrow_obs;
     const uint32_t lane = tid % kWave;
+#if PCS_TRACE
+    const uint64_t t_sweep = wall_clock64();
+#endif
 
     // The row sweep, compiled for bins sorted into LDS and for bins in HBM.
     // Every row issues the same global loads on every path (the observed row
@@ -1002,42 +1082,54 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
 #pragma unroll
         for (int d = 0; d < PF; ++d) load_row(rb[d], min((uint32_t)d, maxrow_obs));
 
-        // Row r's slots [s0, s1) and this lane's bin of it when the row has
-        // 1..kDistSparseRow records (else 0), read one row ahead of their use,
-        // so that a row's bins in HBM arrive while the row before is summed
-        // (read every row, at an index clamped into the slice).
+        // Row r's slots [s0, s1) and the lane's slot s0 + lane of it (clamped
+        // into the slice), read one row ahead of their use, so that a row's
+        // bins arrive while the row before is summed. Lanes past the row's
+        // records hold some other bin: only lanes below the count are read,
+        // and only in a row of at most kDistSparseRow records. No slice, no
+        // read (the LDS form may then have no room for bins at all).
         struct RowAhead {
             uint32_t bin, s0, s1;
         };
         auto row_ahead = [&](uint32_t r) -> RowAhead {
             const uint32_t rr = min(r, last);
             const uint32_t a = uniform((rr == 0u) ? 0u : rowc[rr - 1u]), e = uniform(rowc[rr]);
-            const uint32_t n = e - a;
-            const bool take = r <= last && n > 0u && n <= (uint32_t)kDistSparseRow && lane < n;
-            uint32_t b;
-            if constexpr (IN_LDS) {
-                b = take ? bin_at(a + lane) : 0u;
-            } else {
-                const uint32_t v = bin_at(min(a + lane, T - 1u));
-                b = take ? v : 0u;
-            }
+            uint32_t b = 0u;
+            if (T != 0u) b = bin_at(min(a + lane, T - 1u));
             return RowAhead{b, a, e};
         };
-        uint64_t racc_prev = 0ull, racc_prev_hi = 0ull;
+        // Column pairs (j0 + 2p, j0 + 2p + 1) as 16-bit halves (columns and
+        // per-row counts stay below 2^15): jp holds j0 + 1 and j0 + 2, and
+        // pair p compares them with the bin less 2p.
+        constexpr int NP = (SEG + 1) / 2;
+        const u16x2 jp{(uint16_t)(j0 + 1u), (uint16_t)(j0 + 2u)};
+        struct RowSum {
+            uint64_t lo, hi;
+        };
         // Row i with records in slots [s0, s1) (wave-uniform) and this lane's
-        // sparse bin: the records go into the column counts, then the row's
-        // sum against observed row values rv; rows past `last` repeat row
-        // `last` in both tables.
-        auto do_row = [&](uint32_t i, uint32_t s0, uint32_t s1, uint32_t mine, const uint32_t (&rv)[SEG]) {
+        // slot of it: the records go into the column counts, then the row's
+        // sum against observed row values rv is returned.
+        auto do_row = [&](uint32_t s0, uint32_t s1, uint32_t mine, const uint32_t (&rv)[SEG]) -> RowSum {
             const uint32_t nr = s1 - s0;
             if (nr > 0u && nr <= (uint32_t)kDistSparseRow) {
                 // A few records: every wave holds them (one per lane) and
-                // each thread adds those at or left of its columns.
+                // each thread counts those at or left of its columns, for a
+                // column pair at once: bin - (column + 1) is negative, the
+                // half's top bit, exactly when the bin is at or left of the
+                // column (it lies in [-2^12, 2^12) as a 16-bit difference).
+                u16x2 h[NP];
+#pragma unroll
+                for (int p = 0; p < NP; ++p) h[p] = u16x2{0, 0};
                 for (uint32_t q = 0u; q < nr; ++q) {
                     const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)q);
 #pragma unroll
-                    for (int m = 0; m < SEG; ++m) Ccol[m] += (j0 + (uint32_t)m >= b) ? 1u : 0u;
+                    for (int p = 0; p < NP; ++p) {
+                        const uint16_t bp = (uint16_t)(b - 2u * (uint32_t)p);
+                        h[p] += (u16x2)(u16x2{bp, bp} - jp) >> (uint16_t)15;
+                    }
                 }
+#pragma unroll
+                for (int m = 0; m < SEG; ++m) Ccol[m] += (m & 1) ? (uint32_t)h[m / 2].y : (uint32_t)h[m / 2].x;
             } else if (nr > 0u) {
                 // four bins per thread read (at indices clamped into the row)
                 // before their atomics
@@ -1074,19 +1166,22 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
                     add128(racc, racc_hi, l1 ? (uint64_t)dd : (uint64_t)dd * dd, 0ull);
                 }
             }
-            if (i == last) {
-                const uint64_t rep = (uint64_t)(n_k - last);
-                add128(acc, acc_hi, racc * rep, __umul64hi(racc, rep) + racc_hi * rep);
-            } else {
-                add128(acc, acc_hi, racc, racc_hi);
-            }
-            racc_prev = racc;
-            racc_prev_hi = racc_hi;
+            return RowSum{racc, racc_hi};
+        };
+        // A row's sum counted for w rows: acc += w r in three 32 x 32 -> 64-bit
+        // multiply-adds (r.hi < SEG: a row sums SEG squares below 2^64).
+        auto add_rows = [&](const RowSum& r, uint32_t w) {
+            const uint64_t p0 = (uint64_t)(uint32_t)r.lo * w;
+            const uint64_t p1 = (uint64_t)(uint32_t)(r.lo >> 32) * w + (p0 >> 32);
+            const uint64_t ph = (uint64_t)(uint32_t)r.hi * w + (p1 >> 32);
+            add128(acc, acc_hi, (p1 << 32) | (uint32_t)p0, ph);
         };
 
         // Rows 0..min(last, maxrow_obs): the observed table changes from row
-        // to row, so every row is summed.
+        // to row, so every row is summed; rows past `last` repeat row `last`
+        // in both tables.
         const uint32_t p1 = min(last, maxrow_obs);
+        RowSum rs{0ull, 0ull};
         RowAhead next = row_ahead(0u);
         for (uint32_t i0 = 0; i0 <= p1; i0 += PF) {
 #pragma unroll
@@ -1095,7 +1190,9 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
                 if (i > p1) break;
                 const RowAhead cur = next;
                 next = row_ahead(i + 1u);
-                do_row(i, cur.s0, cur.s1, cur.bin, rb[d]);
+                rs = do_row(cur.s0, cur.s1, cur.bin, rb[d]);
+                if (i == last) add_rows(rs, n_k - last);
+                else add128(acc, acc_hi, rs.lo, rs.hi);
                 // From the deepest observed row on, every slot already holds it.
                 if (i < maxrow_obs) load_row(rb[d], min(i + (uint32_t)PF, maxrow_obs));
             }
@@ -1103,31 +1200,49 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
         // Rows maxrow_obs + 1..last: the observed table repeats row maxrow_obs
         // (every ring slot holds it), so a row without records adds what the
         // row above it did. The rows with records are found 64 at a time by
-        // one ballot over the row offsets and summed; each run of rows
-        // without records between them adds the previous row's sum once per
-        // row. Row `last` holds records (it is the deepest record row).
+        // one ballot over the row offsets, and each one's slots are read
+        // while the one before it is summed. A row's sum is added once the
+        // next row with records is known, counted for itself and the rows
+        // without records after it (row maxrow_obs, already added once, for
+        // those only). Row `last` holds records (it is the deepest record
+        // row), so T > 0 here and every search ends at or before it.
         if (last > maxrow_obs) {
-            uint32_t prev = maxrow_obs;  // the last row summed
-            for (uint32_t base = maxrow_obs + 1u; base <= last; base += (uint32_t)kWave) {
-                const uint32_t r = base + lane;
-                const uint32_t rr = min(r, last);
-                const uint32_t e = rowc[rr], bg = rowc[rr - 1u];
-                uint64_t mask = ballot(r <= last && e != bg);
-                while (mask != 0ull) {
-                    const uint32_t l = (uint32_t)__builtin_ctzll(mask);
-                    mask &= mask - 1ull;
-                    const uint32_t i = base + l;
-                    const uint32_t s0 = (uint32_t)__builtin_amdgcn_readlane((int)bg, (int)l);
-                    const uint32_t s1 = (uint32_t)__builtin_amdgcn_readlane((int)e, (int)l);
-                    const uint64_t gap = (uint64_t)(i - prev - 1u);
-                    add128(acc, acc_hi, racc_prev * gap, __umul64hi(racc_prev, gap) + racc_prev_hi * gap);
-                    const uint32_t nr = s1 - s0;
-                    uint32_t mine = 0u;
-                    if (nr <= (uint32_t)kDistSparseRow && lane < nr) mine = bin_at(s0 + lane);
-                    do_row(i, s0, s1, mine, rb[0]);
-                    prev = i;
+            struct RecRow {
+                uint32_t i, s0, s1, bin;
+            };
+            uint32_t base = maxrow_obs + 1u - (uint32_t)kWave, bg = 0u, e = 0u;
+            uint64_t mask = 0ull;
+            auto next_rec_row = [&]() -> RecRow {
+                while (mask == 0ull && base + (uint32_t)kWave <= last) {
+                    base += (uint32_t)kWave;
+                    const uint32_t r = base + lane, rr = min(r, last);
+                    e = rowc[rr];
+                    bg = rowc[rr - 1u];
+                    mask = ballot(r <= last && e != bg);
                 }
+                if (mask == 0ull) return RecRow{last, T, T, 0u};  // not reached
+                const uint32_t l = (uint32_t)__builtin_ctzll(mask);
+                mask &= mask - 1ull;
+                const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)bg, (int)l);
+                const uint32_t z = (uint32_t)__builtin_amdgcn_readlane((int)e, (int)l);
+                return RecRow{base + l, a, z, bin_at(min(a + lane, T - 1u))};
+            };
+            uint32_t prev = maxrow_obs, already = 1u;
+            RecRow cur = next_rec_row();
+            for (;;) {
+                // the row and its slots are wave-uniform (kept in SGPRs, so
+                // that the branches on them stay scalar)
+                const uint32_t i = uniform(cur.i), s0 = uniform(cur.s0), s1 = uniform(cur.s1);
+                RecRow nxt = cur;
+                if (i < last) nxt = next_rec_row();
+                add_rows(rs, i - prev - already);
+                rs = do_row(s0, s1, cur.bin, rb[0]);
+                prev = i;
+                already = 0u;
+                if (i >= last) break;
+                cur = nxt;
             }
+            add_rows(rs, n_k - last);
         }
     };
     if (in_lds) sweep(std::true_type{});
@@ -1150,7 +1265,11 @@ __device__ __forceinline__ DistOut distance_block(const DevProp& pr, const DevSy
     for (uint32_t w = 0; w < nw; ++w) add128(lo, hi, red[w], red[nw + w]);
     __syncthreads();
     __builtin_amdgcn_s_setprio(0);
-    return DistOut{lo, hi, T, PCS_OK};
+    DistOut o{lo, hi, T, PCS_OK};
+#if PCS_TRACE
+    o.t_sweep = t_sweep;
+#endif
+    return o;
 }
 
 // Records a distance workgroup holds in registers between the row count and
@@ -1443,11 +1562,30 @@ __device__ uint32_t loop_decide(const LoopArgs& a, uint64_t& r_out, uint32_t& t0
 // model classes raised the loop's register pressure and spill traffic on the
 // path every request takes). The proposal and the model are read from their
 // HBM copies and the stage is passed by value (empty on entry and return), so
-// that nothing of the caller's has its address taken.
+// that nothing of the caller's has its address taken. The angle table, the
+// exchange buffer and the stage are in LDS, the record slice and its counter
+// in HBM.
 __attribute__((noinline)) __device__ void walk_tile_coop_call(const DevProp* gp, const DevSys* gs, uint32_t F,
                                                               uint32_t N, uint32_t video, uint32_t p0, Stage st,
-                                                              const float2* tab, uint32_t* xbuf)
+                                                              lds_cf2 tab_l, lds_u32 xbuf_l, lds_u32 buf_l,
+                                                              glb_u32 rec_g, glb_u32 counter_g)
 {
+    // Scalar arguments arrive in VGPRs, as if they could differ between lanes;
+    // read once, they are wave-uniform again (the chunk loop, the step count
+    // and the stage fill then branch on the scalar unit, and the wave masks of
+    // the walk stay in SGPRs).
+    F = uniform(F);
+    N = uniform(N);
+    p0 = uniform(p0);
+    video = uniform(video);
+    st.cap = uniform(st.cap);
+    st.cnt = uniform(st.cnt);
+    st.slice_cap = uniform(st.slice_cap);
+    const float2* tab = (const float2*)tab_l;
+    uint32_t* xbuf = (uint32_t*)xbuf_l;
+    st.buf = (uint32_t*)buf_l;
+    st.rec = (uint32_t*)rec_g;
+    st.counter = (uint32_t*)counter_g;
     const DevProp pr = load_uniform(gp);
     const DevSys sy = load_uniform(gs);
     const bool gauss = sy.detect_model == PCS_DETECT_GAUSS_Z;
@@ -1475,6 +1613,12 @@ __attribute__((noinline)) __device__ DistOut serve_distance(const DevProp* gp, c
     __shared__ uint32_t wsum[kDistThreads / kWave];
     __shared__ uint32_t smax;
     __shared__ uint64_t red[kDistRed];
+    // wave-uniform again (scalar arguments arrive in VGPRs)
+    count = uniform(count);
+    rowc_off = uniform(rowc_off);
+    nk_lds = uniform(nk_lds);
+    nde_lds = uniform(nde_lds);
+    srtl_cap = uniform(srtl_cap);
     uint32_t* rowc = loop_lds + rowc_off;
     const DistLds L{rowc, rowc + nk_lds, wsum, &smax, red, reinterpret_cast<uint16_t*>(rowc + nk_lds + nde_lds),
                     srtl_cap};
@@ -1606,7 +1750,8 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
             const bool gauss = sy.detect_model == PCS_DETECT_GAUSS_Z;
             const bool zref = sy.z_boundary == PCS_Z_REFLECT;
             if (coop) {  // exchange buffers in the distance's room, idle while a tile is walked
-                walk_tile_coop_call(&S.prop, &a.sys[pr.sys_id], F, N, lo, p0, st, tab, rowc);
+                walk_tile_coop_call(&S.prop, &a.sys[pr.sys_id], F, N, lo, p0, st, (lds_cf2)tab, (lds_u32)rowc,
+                                    (lds_u32)st.buf, (glb_u32)st.rec, (glb_u32)st.counter);
             } else if (gauss) {
                 if (zref) walk_tile<1, true, true, false>(pr, sy, F, N, lo, p0, st, tab, 0u, 0u);
                 else walk_tile<1, true, false, false>(pr, sy, F, N, lo, p0, st, tab, 0u, 0u);
@@ -1636,6 +1781,7 @@ __global__ __launch_bounds__(kLoopWaves * kWave) __attribute__((amdgpu_waves_per
                 d.t_admit = S.t_admit;
                 d.t_claim = S.t_claim;
                 d.t_dist = t_dist;
+                d.t_sweep = o.t_sweep;
                 d.t_done = wall_clock64();
 #endif
                 d.exact = dist_exact(o);
@@ -2331,7 +2477,7 @@ hipError_t launch_distance(const DistArgs& args, uint32_t n_k_max, uint32_t n_de
 
 bool serve_loop_coop_fits(const LoopArgs& a)
 {
-    return (size_t)a.nk_lds + a.nde_lds + a.srtl_cap / 2u >= (size_t)2 * kCoopWords * kWave;
+    return (size_t)a.nk_lds + a.nde_lds + a.srtl_cap / 2u >= (size_t)kCoopBufWords;
 }
 
 size_t serve_loop_lds(const LoopArgs& a)

@@ -546,11 +546,13 @@ uint32_t loop_collect(pcs_server* s, std::vector<pcs_completion>& out)
         c.done_us = s->now_us();
 #if PCS_TRACE
         // host clock (us): submit, publish to the ring, result seen; device
-        // wall clock (ticks of 10 ns): admitted, first claim, distance, done
-        std::fprintf(stderr, "pcs-trace %llu %.3f %.3f %.3f %llu %llu %llu %llu %u\n", (unsigned long long)c.tag,
-                     c.submit_us, b.publish_us, c.done_us, (unsigned long long)d.t_admit,
-                     (unsigned long long)d.t_claim, (unsigned long long)d.t_dist, (unsigned long long)d.t_done,
-                     d.group_walked);
+        // wall clock (ticks of 10 ns): admitted, first claim, distance, its
+        // row sweep, done; then whether the tiles were group-walked, and the
+        // records
+        std::fprintf(stderr, "pcs-trace %llu %.3f %.3f %.3f %llu %llu %llu %llu %llu %u %u\n",
+                     (unsigned long long)c.tag, c.submit_us, b.publish_us, c.done_us, (unsigned long long)d.t_admit,
+                     (unsigned long long)d.t_claim, (unsigned long long)d.t_dist, (unsigned long long)d.t_sweep,
+                     (unsigned long long)d.t_done, d.group_walked, d.tracks);
 #endif
         out.push_back(c);
         n_group += d.group_walked != 0u ? 1u : 0u;

@@ -6,7 +6,8 @@ serving loop of the tracing build of the library (lib/trace/libpcs.so,
 `make -C part_conc_spt_abc_amd/csrc trace`), which prints per completion the
 host clock at submit, at publication to the request ring and when the result
 is seen, and the device wall clock at admission, first tile claim, and the
-start and end of the distance. Prints the median of each stage.
+start of the distance, of its row sweep and its end. Prints the median of each
+stage and of the request's track records.
 
     python -m part_conc_spt_abc_amd.latency_anatomy [--particles 64 6400] [--frames 250 1000]
         [--requests 50] [--gap-ms 1] [--one-wave-tiles]
@@ -81,18 +82,20 @@ def main(argv=None):
             cases.append((int(f[1]), int(f[2]), int(f[3])))
         elif f[:1] == ["pcs-trace"]:
             rows[int(f[1])] = [float(x) for x in f[2:]]
-    print(f"{'particles':>9} {'frames':>6} | {'submit->publish':>15} {'admit->claim':>12} {'walk':>8} "
-          f"{'distance':>8} {'ring+return':>11} {'total':>8} | group-walked  (median us)")
+    print(f"{'particles':>9} {'frames':>6} {'records':>7} | {'submit->publish':>15} {'admit->claim':>12} "
+          f"{'walk':>8} {'distance':>8} {'(records':>8} {'sweep)':>7} {'ring+return':>11} {'total':>8} | "
+          f"group-walked  (median us)")
     for n, f, first in cases:
         tags = range(first + 10, first + 10 + args.requests)  # the first 10 of each case warm up
         r = np.array([rows[t] for t in tags])
-        sub, pub, seen, ta, tc, td, tn, gw = r.T
+        sub, pub, seen, ta, tc, td, ts, tn, gw, nrec = r.T
         dev = lambda a, b: (b - a) * TICK_US
-        stages = np.stack([pub - sub, dev(ta, tc), dev(tc, td), dev(td, tn),
+        stages = np.stack([pub - sub, dev(ta, tc), dev(tc, td), dev(td, tn), dev(td, ts), dev(ts, tn),
                            (seen - pub) - dev(ta, tn), seen - sub], axis=1)
         med = np.median(stages, axis=0)
-        print(f"{n:9d} {f:6d} | {med[0]:15.1f} {med[1]:12.1f} {med[2]:8.1f} {med[3]:8.1f} {med[4]:11.1f} "
-              f"{med[5]:8.1f} | {int(gw.sum())}/{len(tags)}")
+        print(f"{n:9d} {f:6d} {int(np.median(nrec)):7d} | {med[0]:15.1f} {med[1]:12.1f} {med[2]:8.1f} "
+              f"{med[3]:8.1f} {med[4]:8.1f} {med[5]:7.1f} {med[6]:11.1f} {med[7]:8.1f} | "
+              f"{int(gw.sum())}/{len(tags)}")
 
 
 if __name__ == "__main__":
