@@ -662,16 +662,16 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
 // request's latency: a lone wave issues one instruction per few cycles, so
 // its tile takes as long as its whole instruction stream. The chunk's work is
 // pipelined over three stages with one group barrier per round: in round i,
-// waves 0 and 1 run Philox blocks 1 and 2 of chunk i (one each, the 64-bit
-// multiplies that dominate a block) into an LDS word buffer; waves 0..2 turn
-// chunk i - 1's words into its unit normals - wave w the Box-Muller pairs 2w
-// and 2w + 1, wave 2 also the Gaussian-detection words - into a normal
-// buffer; wave 3 holds the particles, steps chunk i - 2 and stages its tracks.
+// waves 0, 1 and 2 run Philox blocks 1, 2 and (Gaussian detection) 3 of
+// chunk i into an LDS word buffer, one each; waves 0..2 turn chunk i - 1's
+// words into its unit normals - wave w the Box-Muller pairs 2w and 2w + 1,
+// wave 2 also hands on the detection words - into a normal buffer; wave 3
+// holds the particles, steps chunk i - 2 and stages its tracks.
 // Both buffers are double (chunk parity). The values and their order are
 // walk_tile's, so the results are the same. xbuf: kCoopBufWords words
 // (lane-minor, so a wave's accesses to one word hit 64 consecutive words).
 constexpr int kCoopWords = 20;     // normals per lane and chunk: d[12], r^2 of pairs 0..3, detection words 0..3
-constexpr int kCoopRawWords = 8;   // Philox words per lane and chunk: blocks 1 and 2
+constexpr int kCoopRawWords = 12;  // Philox words per lane and chunk: blocks 1, 2 and 3 (or a spare)
 constexpr int kCoopBufWords = 2 * (kCoopWords + kCoopRawWords) * kWave;
 
 template <bool GAUSS, bool ZREF>
@@ -702,27 +702,28 @@ __device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& 
     for (uint32_t it = 0; it <= nc + 1u; ++it) {
         if (wave < 3u) {
             // Pair p's radius is word p of blocks 1, 2 (words 0..7), its angle
-            // word 6 + p / 4. Every stage runs in every round, so that the
-            // compiler can interleave this wave's Philox block with its
-            // Box-Muller pairs: in round 0 the pairs take words no round
-            // wrote and write normals to the half that chunk 1 overwrites in
-            // round 2, before the stepping wave first reads it in round 3; in
-            // rounds nc and nc + 1 the blocks and pairs are those of chunks
-            // past the last, written to halves whose chunks were read in the
-            // round before. Any word gives finite normals and table indices
-            // below 256.
+            // word 6 + p / 4. Every stage runs in every round and this wave's
+            // Philox block is not under a branch, so that the compiler can
+            // interleave it with the wave's Box-Muller pairs in one basic
+            // block: in round 0 the pairs take words no round wrote and write
+            // normals to the half that chunk 1 overwrites in round 2, before
+            // the stepping wave first reads it in round 3; in rounds nc and
+            // nc + 1 the blocks and pairs are those of chunks past the last,
+            // written to halves whose chunks were read in the round before.
+            // Any word gives finite normals and table indices below 256.
             const uint32_t c = it - 1u;  // the chunk whose normals this round makes
             const int pa = 2 * (int)wave;
             const uint32_t* rr = raw + (c & 1u) * (uint32_t)(kCoopRawWords * kWave) + lane;
             const uint32_t wr0 = rr[pa * kWave], wr1 = rr[(pa + 1) * kWave], wa = rr[(6 + pa / 4) * kWave];
-            if (wave < 2u) {
-                const U4 b = philox10(B * it + 1u + wave, id, video, sim, k0, k1);
-                uint32_t* rw = raw + (it & 1u) * (uint32_t)(kCoopRawWords * kWave) + 4u * wave * kWave + lane;
-                rw[0] = b.x;
-                rw[kWave] = b.y;
-                rw[2 * kWave] = b.z;
-                rw[3 * kWave] = b.w;
-            }
+            // block 1 + wave of chunk it: with Gaussian detection wave 2's is
+            // block 3, the detection words, handed on in the next round;
+            // without it, wave 2's block (chunk it + 1's first) goes unused
+            const U4 b = philox10(B * it + 1u + wave, id, video, sim, k0, k1);
+            uint32_t* rw = raw + (it & 1u) * (uint32_t)(kCoopRawWords * kWave) + 4u * wave * kWave + lane;
+            rw[0] = b.x;
+            rw[kWave] = b.y;
+            rw[2 * kWave] = b.z;
+            rw[3 * kWave] = b.w;
             float* xb = reinterpret_cast<float*>(xbuf) + (c & 1u) * (uint32_t)(kCoopWords * kWave) + lane;
             float d0, d1, y0, d2, d3, y1;
             bm_pair(wr0, wa, pa, tab, d0, d1, y0);
@@ -736,11 +737,8 @@ __device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& 
                 xb[(13 + pa) * kWave] = y1;
             }
             if (GAUSS && wave == 2u) {
-                const U4 b2 = philox10(B * c + 3u, id, video, sim, k0, k1);
-                xb[16 * kWave] = __uint_as_float(b2.x);
-                xb[17 * kWave] = __uint_as_float(b2.y);
-                xb[18 * kWave] = __uint_as_float(b2.z);
-                xb[19 * kWave] = __uint_as_float(b2.w);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xb[(16 + j) * kWave] = __uint_as_float(rr[(8 + j) * kWave]);
             }
         } else if (it >= 2u) {
             const uint32_t c = it - 2u;
