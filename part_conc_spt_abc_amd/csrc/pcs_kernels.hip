@@ -666,7 +666,8 @@ __device__ __forceinline__ void walk_tile(const DevProp& pr, const DevSys& sy, u
 // chunk i into an LDS word buffer, one each; waves 0..2 turn chunk i - 1's
 // words into its unit normals - wave w the Box-Muller pairs 2w and 2w + 1,
 // wave 2 also hands on the detection words - into a normal buffer; wave 3
-// holds the particles, steps chunk i - 2 and stages its tracks.
+// holds the particles, reads chunk i - 2's normals and steps chunk i - 3,
+// whose normals it read in the round before, and stages its tracks.
 // Both buffers are double (chunk parity). The values and their order are
 // walk_tile's, so the results are the same. xbuf: kCoopBufWords words
 // (lane-minor, so a wave's accesses to one word hit 64 consecutive words).
@@ -691,24 +692,74 @@ __device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& 
     const uint32_t nc = (steps + 3u) / 4u;
     uint32_t* raw = xbuf + 2 * kCoopWords * kWave;
 
-    Particle q;
-    uint64_t m_active = 0ull, m_prev = 0ull;
+    // The stepping wave and the producers run loops of their own, with one
+    // group barrier per round each and the same number of rounds (the branch
+    // on the wave index is wave-uniform), so that neither holds the other's
+    // registers.
     if (wave == 3u) {
+        Particle q;
         q.id = id;
         init_particle<GAUSS, ZREF>(q, N, video, sim, k0, k1, pr, K);
-        m_active = ballot(q.active);
-        m_prev = ballot(q.det0);
-    }
-    for (uint32_t it = 0; it <= nc + 1u; ++it) {
-        if (wave < 3u) {
+        const uint64_t m_active = ballot(q.active);
+        uint64_t m_prev = ballot(q.det0);
+        float d[12], r2[4];  // the chunk read a round ahead of its steps
+        uint32_t wdet[4];
+        for (uint32_t it = 0; it <= nc + 2u; ++it) {
+            // Chunk it - 2's values (made in round it - 1) are read at the start
+            // of the round; the chunk read in the round before, it - 3, is
+            // stepped while they arrive.
+            const bool rd = it >= 2u && it - 2u < nc;
+            float dq[12], r2q[4];
+            uint32_t wq[4];
+            if (rd) {
+                const uint32_t c = it - 2u;
+                const float* xb = reinterpret_cast<const float*>(xbuf) + (c & 1u) * (uint32_t)(kCoopWords * kWave) + lane;
+#pragma unroll
+                for (int k = 0; k < 12; ++k) dq[k] = xb[k * kWave];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    r2q[k] = xb[(12 + k) * kWave];
+                    wq[k] = GAUSS ? __float_as_uint(xb[(16 + k) * kWave]) : 0u;
+                }
+            }
+            if (it >= 3u) {
+                const uint32_t c = it - 3u;
+                const uint32_t nst = min(4u, steps - 4u * c);
+                if (nst == 4u) {
+                    step_chunk4<GAUSS, ZREF>(q, m_active, m_prev, d, r2, wdet, K, st);
+                } else {
+#pragma unroll
+                    for (uint32_t s = 0; s < 4u; ++s) {
+                        if (s >= nst) break;
+                        step_particle<GAUSS, ZREF, false>(q, m_active, m_prev, d[2 * s], d[2 * s + 1], d[8 + s],
+                                                          r2[s], wdet[s], s, K, st);
+                    }
+                }
+                q.ts -= (int32_t)nst;  // the chunk's last frame becomes frame 0
+            }
+            if (rd) {
+#pragma unroll
+                for (int k = 0; k < 12; ++k) d[k] = dq[k];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    r2[k] = r2q[k];
+                    wdet[k] = wq[k];
+                }
+            }
+            __syncthreads();
+        }
+        close_open_tracks<false>(q, m_active, m_prev, K, st);
+        stage_flush_binned(st, K.bs);
+    } else {
+        for (uint32_t it = 0; it <= nc + 2u; ++it) {
             // Pair p's radius is word p of blocks 1, 2 (words 0..7), its angle
             // word 6 + p / 4. Every stage runs in every round and this wave's
             // Philox block is not under a branch, so that the compiler can
             // interleave it with the wave's Box-Muller pairs in one basic
             // block: in round 0 the pairs take words no round wrote and write
             // normals to the half that chunk 1 overwrites in round 2, before
-            // the stepping wave first reads it in round 3; in rounds nc and
-            // nc + 1 the blocks and pairs are those of chunks past the last,
+            // the stepping wave first reads it in round 3; in rounds nc to
+            // nc + 2 the blocks and pairs are those of chunks past the last,
             // written to halves whose chunks were read in the round before.
             // Any word gives finite normals and table indices below 256.
             const uint32_t c = it - 1u;  // the chunk whose normals this round makes
@@ -740,36 +791,8 @@ __device__ __forceinline__ void walk_tile_coop(const DevProp& pr, const DevSys& 
 #pragma unroll
                 for (int j = 0; j < 4; ++j) xb[(16 + j) * kWave] = __uint_as_float(rr[(8 + j) * kWave]);
             }
-        } else if (it >= 2u) {
-            const uint32_t c = it - 2u;
-            const uint32_t nst = min(4u, steps - 4u * c);
-            const float* xb = reinterpret_cast<const float*>(xbuf) + (c & 1u) * (uint32_t)(kCoopWords * kWave) + lane;
-            float d[12], r2[4];
-            uint32_t wdet[4];
-#pragma unroll
-            for (int k = 0; k < 12; ++k) d[k] = xb[k * kWave];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                r2[k] = xb[(12 + k) * kWave];
-                wdet[k] = GAUSS ? __float_as_uint(xb[(16 + k) * kWave]) : 0u;
-            }
-            if (nst == 4u) {
-                step_chunk4<GAUSS, ZREF>(q, m_active, m_prev, d, r2, wdet, K, st);
-            } else {
-#pragma unroll
-                for (uint32_t s = 0; s < 4u; ++s) {
-                    if (s >= nst) break;
-                    step_particle<GAUSS, ZREF, false>(q, m_active, m_prev, d[2 * s], d[2 * s + 1], d[8 + s], r2[s],
-                                                      wdet[s], s, K, st);
-                }
-            }
-            q.ts -= (int32_t)nst;  // the chunk's last frame becomes frame 0
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    if (wave == 3u) {
-        close_open_tracks<false>(q, m_active, m_prev, K, st);
-        stage_flush_binned(st, K.bs);
     }
 }
 
