@@ -43,13 +43,50 @@ def child(args):
     for n in args.particles:
         for f in args.frames:
             print(f"case {n} {f} {tag}", file=sys.stderr, flush=True)
-            for k in range(args.requests + 10):
+            for k in range(args.requests + WARMUP):
                 assert srv.submit(tag, sid, p, np.array([f], np.uint32), np.array([n], np.uint32), 100 + k)
                 tag += 1
                 while not srv.poll(4, wait_us=5000):
                     pass
                 time.sleep(args.gap_ms * 1e-3)
     srv.close()
+
+
+# A pcs-trace line of the tracing build (pcs_server.hip): tag, host clock at
+# submit, publish and completion seen (us); device wall clock at admission,
+# first claim, distance start, row-sweep start and end (ticks); whether the
+# tiles were group-walked; the request's track records.
+TRACE_FIELDS = ("submit", "publish", "seen", "admit", "claim", "dist", "sweep", "done", "group_walked", "records")
+WARMUP = 10  # requests of each case that are not counted
+
+
+def parse(text):
+    """The cases ("case particles frames first_tag") and the pcs-trace rows
+    (tag -> the TRACE_FIELDS values) of a child's stderr."""
+    cases, rows = [], {}
+    for line in text.splitlines():
+        f = line.split()
+        if f[:1] == ["case"]:
+            cases.append((int(f[1]), int(f[2]), int(f[3])))
+        elif f[:1] == ["pcs-trace"]:
+            if len(f) != 2 + len(TRACE_FIELDS):
+                raise ValueError(f"pcs-trace line with {len(f) - 2} fields, not {len(TRACE_FIELDS)}: {line!r}")
+            rows[int(f[1])] = [float(x) for x in f[2:]]
+    return cases, rows
+
+
+def summarize(rows, first, requests):
+    """Median records and median stages (us) of the case whose tags start at
+    `first`, past its warm-up: submit->publish, admit->claim, walk, distance,
+    its record passes, its sweep, ring + return, total; and how many of the
+    requests were group-walked, of how many."""
+    tags = range(first + WARMUP, first + WARMUP + requests)
+    r = np.array([rows[t] for t in tags])
+    sub, pub, seen, ta, tc, td, ts, tn, gw, nrec = r.T
+    dev = lambda a, b: (b - a) * TICK_US
+    stages = np.stack([pub - sub, dev(ta, tc), dev(tc, td), dev(td, tn), dev(td, ts), dev(ts, tn),
+                       (seen - pub) - dev(ta, tn), seen - sub], axis=1)
+    return int(np.median(nrec)), np.median(stages, axis=0), int(gw.sum()), len(tags)
 
 
 def main(argv=None):
@@ -75,28 +112,15 @@ def main(argv=None):
     if p.returncode != 0:
         sys.stderr.write(p.stdout + p.stderr[-4000:])
         raise SystemExit(p.returncode)
-    cases, rows = [], {}
-    for line in p.stderr.splitlines():
-        f = line.split()
-        if f[:1] == ["case"]:
-            cases.append((int(f[1]), int(f[2]), int(f[3])))
-        elif f[:1] == ["pcs-trace"]:
-            rows[int(f[1])] = [float(x) for x in f[2:]]
+    cases, rows = parse(p.stderr)
     print(f"{'particles':>9} {'frames':>6} {'records':>7} | {'submit->publish':>15} {'admit->claim':>12} "
           f"{'walk':>8} {'distance':>8} {'(records':>8} {'sweep)':>7} {'ring+return':>11} {'total':>8} | "
           f"group-walked  (median us)")
     for n, f, first in cases:
-        tags = range(first + 10, first + 10 + args.requests)  # the first 10 of each case warm up
-        r = np.array([rows[t] for t in tags])
-        sub, pub, seen, ta, tc, td, ts, tn, gw, nrec = r.T
-        dev = lambda a, b: (b - a) * TICK_US
-        stages = np.stack([pub - sub, dev(ta, tc), dev(tc, td), dev(td, tn), dev(td, ts), dev(ts, tn),
-                           (seen - pub) - dev(ta, tn), seen - sub], axis=1)
-        med = np.median(stages, axis=0)
-        print(f"{n:9d} {f:6d} {int(np.median(nrec)):7d} | {med[0]:15.1f} {med[1]:12.1f} {med[2]:8.1f} "
+        rec, med, walked, count = summarize(rows, first, args.requests)
+        print(f"{n:9d} {f:6d} {rec:7d} | {med[0]:15.1f} {med[1]:12.1f} {med[2]:8.1f} "
               f"{med[3]:8.1f} {med[4]:8.1f} {med[5]:7.1f} {med[6]:11.1f} {med[7]:8.1f} | "
-              f"{int(gw.sum())}/{len(tags)}")
-
+              f"{walked}/{count}")
 
 if __name__ == "__main__":
     main()
