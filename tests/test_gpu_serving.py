@@ -261,8 +261,9 @@ def test_device_loop_empty_and_multi_video_requests():
 def test_device_loop_group_walked_tiles_at_chunk_edges():
     """A request of few tiles admitted onto a quiet loop is walked a tile per
     group (three waves produce each 4-step chunk's normals, one steps). Frame
-    counts around the chunk edges (0..9 frames: no step, partial chunks,
-    whole chunks) and particle counts around a tile (0, 1, 63, 64, 65, 200)
+    counts around the chunk edges (0..14 frames: no step, partial chunks,
+    whole chunks, up to four chunks in the group's pipeline) and particle
+    counts around a tile (0, 1, 63, 64, 65, 200)
     in all three model classes give the same results as the one-wave walk,
     the host-driven loop and a resident batch. Submitted one at a time, every
     request is group-walked; submitted all at once, those admitted behind
@@ -273,7 +274,7 @@ def test_device_loop_group_walked_tiles_at_chunk_edges():
     reqs = []
     for ci, (m, D, obs) in enumerate(ch):
         p0 = proposal_record(m, D, list(np.full(m.n_species, 1e9 / m.n_species)), [2.0], 0, 0, 1)
-        for F in (0, 1, 2, 4, 5, 6, 9, 1000):
+        for F in (0, 1, 2, 4, 5, 6, 9, 10, 13, 14, 1000):
             for N in (0, 1, 63, 64, 65, 200):
                 p = _abi.pcs_proposal.from_buffer_copy(p0)
                 p.sim_id = len(reqs)
